@@ -1,0 +1,678 @@
+// hk_kernels.hip -- CDNA4 (gfx950) kernels of the path-tracing core.
+//
+//  * trace_ray<ANY>: 4-wide BVH traversal, stack of (ref, tnear) pairs with the
+//    top kLdsStack entries in LDS and the rest in private memory, exact FP32
+//    Moller-Trumbore in leaves (replaces Acceleration::RayIntersect).
+//  * k_intersect: batched rayIntersect over hk_ray / hk_hit arrays.
+//  * k_bsdf_batch: batched BSDF::Sample / Eval / Pdf for parity tests.
+//  * wavefront pathMIS (PathMISIntegrator::Li, src/integrator/
+//    PathMISIntegrator.cpp:16-157): k_camera -> [k_extend -> k_shade ->
+//    k_shadow]* -> k_resolve.  Path state is path-indexed SoA; ray, hit and
+//    shadow queues are compacted with wave ballots (one atomic per wave).
+//    Film accumulation is owned per output pixel (gather in k_resolve), so
+//    results are deterministic and independent of scheduling.
+#include <hip/hip_runtime.h>
+
+#include "hk_bsdf.h"
+#include "hk_kernels.h"
+
+namespace hk {
+
+constexpr int kBlock = 256;          // threads per block for every kernel here
+constexpr int kLdsStack = 16;        // stack entries per lane kept in LDS
+constexpr int kPrivStack = 80;       // further entries in private memory
+constexpr float kInfF = __builtin_huge_valf();
+
+
+// ------------------------------------------------------------------ traversal
+struct TraceStack {
+  int2 *lds;  // kLdsStack x kBlock, entry (ref, tnear bits)
+  int tid;
+  int sp;
+  int2 priv[kPrivStack];
+  __device__ __forceinline__ void push(int32_t ref, float t) {
+    int2 e = make_int2(ref, __float_as_int(t));
+    if (sp < kLdsStack) lds[sp * kBlock + tid] = e;
+    else priv[sp - kLdsStack] = e;
+    sp++;
+  }
+  __device__ __forceinline__ int2 pop() {
+    sp--;
+    return sp < kLdsStack ? lds[sp * kBlock + tid] : priv[sp - kLdsStack];
+  }
+};
+
+template <bool ANY>
+__device__ __forceinline__ bool trace_ray(const Bvh4Node *__restrict__ nodes, const TraceTri *__restrict__ tris,
+                                          int32_t root, int2 *lds_stack, vec3 o, vec3 d, float tmin, float tmax,
+                                          float &bt, float &bu, float &bv, int32_t &bprim) {
+  float ix = fabsf(d.x) > 1e-30f ? 1.0f / d.x : copysignf(1e30f, d.x);
+  float iy = fabsf(d.y) > 1e-30f ? 1.0f / d.y : copysignf(1e30f, d.y);
+  float iz = fabsf(d.z) > 1e-30f ? 1.0f / d.z : copysignf(1e30f, d.z);
+  TraceStack st;
+  st.lds = lds_stack;
+  st.tid = threadIdx.x;
+  st.sp = 0;
+  bt = tmax;
+  bprim = -1;
+  bu = bv = 0.0f;
+  // Slab tests are widened by 4 ulp-relative (tnear <= tfar * (1 + 2^-21)),
+  // which covers the rounding of (lo - o) * (1/d); triangle boxes are padded
+  // on the host for the inexactness of the triangle test itself.
+  const float kWiden = 1.0000005f;
+  int32_t ref = root;
+  for (;;) {
+    if (ref >= 0) {
+      const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
+      float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+      int4 ch = reinterpret_cast<const int4 *>(np)[6];
+      float tn[4];
+      const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+      const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+      const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+      const int32_t cha[4] = {ch.x, ch.y, ch.z, ch.w};
+      float tlim = bt * kWiden;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        float t0x = (lxa[c] - o.x) * ix, t1x = (hxa[c] - o.x) * ix;
+        float t0y = (lya[c] - o.y) * iy, t1y = (hya[c] - o.y) * iy;
+        float t0z = (lza[c] - o.z) * iz, t1z = (hza[c] - o.z) * iz;
+        float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+        float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tlim));
+        bool hit = cha[c] != kEmptyRef && a <= b * kWiden;
+        tn[c] = hit ? a : kInfF;
+      }
+      // order children near -> far (sorting network on (t, ref))
+      int32_t r[4] = {cha[0], cha[1], cha[2], cha[3]};
+#define HK_CSWAP(i, j)                                    \
+  if (tn[j] < tn[i]) {                                    \
+    float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;          \
+    int32_t rr = r[i]; r[i] = r[j]; r[j] = rr;            \
+  }
+      HK_CSWAP(0, 1) HK_CSWAP(2, 3) HK_CSWAP(0, 2) HK_CSWAP(1, 3) HK_CSWAP(1, 2)
+#undef HK_CSWAP
+      // push the far ones, continue with the nearest
+      if (tn[3] != kInfF) st.push(r[3], tn[3]);
+      if (tn[2] != kInfF) st.push(r[2], tn[2]);
+      if (tn[1] != kInfF) st.push(r[1], tn[1]);
+      if (tn[0] != kInfF) {
+        ref = r[0];
+        continue;
+      }
+    } else {
+      uint32_t code = uint32_t(~ref);
+      uint32_t first = code >> 3, cnt = (code & 7u) + 1u;
+      for (uint32_t k = 0; k < cnt; k++) {
+        const float4 *tp = reinterpret_cast<const float4 *>(tris + first + k);
+        float4 a = tp[0], b = tp[1], c = tp[2];
+        float u, v, t;
+        if (mt_test(o, d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(b.w, c.x, c.y), tmin, bt, u, v, t)) {
+          int32_t prim = __float_as_int(a.w);
+          if (ANY) {
+            bt = t; bu = u; bv = v; bprim = prim;
+            return true;
+          }
+          // closest hit; among equal t the largest prim (brute-force order)
+          if (t < bt || prim > bprim) {
+            bt = t; bu = u; bv = v; bprim = prim;
+          }
+        }
+      }
+    }
+    // pop the next entry that can still hold a hit at t <= bt
+    bool found = false;
+    while (st.sp > 0) {
+      int2 e = st.pop();
+      if (__int_as_float(e.y) <= bt * kWiden) {
+        ref = e.x;
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  return bprim >= 0;
+}
+
+// ------------------------------------------------------------------ batched rayIntersect
+__global__ __launch_bounds__(kBlock) void k_intersect(const Bvh4Node *__restrict__ nodes,
+                                                      const TraceTri *__restrict__ tris, int32_t root,
+                                                      const float4 *__restrict__ rays, float4 *__restrict__ hits,
+                                                      uint32_t n, int any) {
+  __shared__ int2 stack[kLdsStack * kBlock];
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    float4 a = rays[2 * i], b = rays[2 * i + 1];
+    float t, u, v;
+    int32_t prim;
+    bool h = any ? trace_ray<true>(nodes, tris, root, stack, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, t, u,
+                                   v, prim)
+                 : trace_ray<false>(nodes, tris, root, stack, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, t,
+                                    u, v, prim);
+    hits[i] = h ? make_float4(t, u, v, __int_as_float(prim)) : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+  }
+}
+
+// ------------------------------------------------------------------ BSDF batch
+__global__ __launch_bounds__(kBlock) void k_bsdf_batch(const DevMaterial *__restrict__ mats,
+                                                       const float *__restrict__ rft, int idx, uint32_t n,
+                                                       const float *__restrict__ wi, const float *__restrict__ wo,
+                                                       const int32_t *__restrict__ meas,
+                                                       const float *__restrict__ u, float *out_eval, float *out_pdf,
+                                                       float *out_f, float *out_wo, int32_t *out_meas,
+                                                       float *out_eta, float *out_spdf) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    BRec r;
+    r.wi = ld3(wi + 3 * i);
+    r.wo = ld3(wo + 3 * i);
+    r.measure = meas[i];
+    r.eta = 1.0f;
+    vec3 e = bsdf_eval(mats, rft, idx, r);
+    out_eval[3 * i] = e.x; out_eval[3 * i + 1] = e.y; out_eval[3 * i + 2] = e.z;
+    out_pdf[i] = bsdf_pdf(mats, rft, idx, r);
+    BRec s;
+    s.wi = r.wi;
+    s.wo = v3(0.0f);
+    s.eta = 1.0f;
+    s.measure = kUnknownMeasure;
+    U1 u1{nullptr, u + 4 * i + 2, 0, 2};
+    vec3 f = bsdf_sample(mats, rft, idx, s, u[4 * i], u[4 * i + 1], u1);
+    out_f[3 * i] = f.x; out_f[3 * i + 1] = f.y; out_f[3 * i + 2] = f.z;
+    out_wo[3 * i] = s.wo.x; out_wo[3 * i + 1] = s.wo.y; out_wo[3 * i + 2] = s.wo.z;
+    out_meas[i] = s.measure;
+    out_eta[i] = s.eta;
+    out_spdf[i] = bsdf_pdf(mats, rft, idx, s);
+  }
+}
+
+// ------------------------------------------------------------------ wavefront helpers
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// Wave-aggregated append: one atomic per wave, slots in lane order.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+  unsigned long long m = __ballot(pred);
+  if (m == 0ull) return 0xffffffffu;
+  uint32_t lane = lane_id();
+  uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader);
+  unsigned long long below = lane == 0 ? 0ull : (m & ((1ull << lane) - 1ull));
+  return pred ? base + (uint32_t)__popcll(below) : 0xffffffffu;
+}
+
+__device__ __forceinline__ uint32_t find_item(const WorkItem *items, uint32_t n, uint32_t id) {
+  uint32_t lo = 0, hi = n;  // largest i with items[i].path_begin <= id
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (items[mid].path_begin <= id) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ vec3 mat3_mul(const float *m, vec3 v) {  // row-major 3x3
+  return v3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+            m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+
+// ---- emitters
+struct ERec {
+  vec3 p, n, wi;
+  float pdf, dist;
+};
+
+// DiscretePDF1D::SampleContinuous / SampleDiscrete (src/core/DiscretePDF.cpp:32-78)
+__device__ __forceinline__ int cdf_find(const float *cdf, int n, float s) {
+  int lo = 0, cnt = n + 1;  // lower_bound over n + 1 entries
+  while (cnt > 0) {
+    int step = cnt >> 1;
+    if (cdf[lo + step] < s) { lo += step + 1; cnt -= step + 1; }
+    else cnt = step;
+  }
+  int i = lo - 1;
+  return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+}
+__device__ __forceinline__ float cdf_sample_cont(const float *cdf, const float *func, int n, float integral, float s,
+                                                 float &pdf, int &idx) {
+  int i = cdf_find(cdf, n, s);
+  idx = i;
+  float du = s - cdf[i];
+  if (cdf[i + 1] - cdf[i] > 0.0f) du /= (cdf[i + 1] - cdf[i]);
+  pdf = func[i] / integral;
+  return ((float)i + du) / (float)n;
+}
+
+// AreaLight::Sample with Mesh::SamplePosition (src/emitter/AreaLight.cpp:14-35,
+// src/core/Mesh.cpp:273-303)
+__device__ __forceinline__ vec3 area_sample(const DevScene &sc, const DevMesh &m, vec3 ref, float sx, float sy,
+                                            float s1, ERec &er) {
+  int tri = cdf_find(sc.cdf + m.cdf_off, (int)m.nt, s1);
+  float sq = sqrtf(1.0f - sx);
+  float al = 1.0f - sq, be = sy * sq, ga = 1.0f - al - be;
+  const uint32_t *f = sc.F + 3 * size_t(m.first_prim + tri);
+  vec3 p0 = ld3(sc.V + 3 * size_t(f[0])), p1 = ld3(sc.V + 3 * size_t(f[1])), p2 = ld3(sc.V + 3 * size_t(f[2]));
+  er.p = (ga * p0 + al * p1) + be * p2;
+  if (m.has_n) {
+    er.n = (ga * ld3(sc.N + 3 * size_t(f[0])) + al * ld3(sc.N + 3 * size_t(f[1]))) + be * ld3(sc.N + 3 * size_t(f[2]));
+  } else {
+    er.n = normalized(cross(p1 - p0, p2 - p0));
+  }
+  vec3 wi = er.p - ref;
+  er.dist = norm(wi);
+  er.wi = normalized(wi);
+  float gd = fabsf(dot(-er.wi, er.n));
+  er.pdf = gd == 0.0f ? 0.0f : m.inv_area * ((er.dist * er.dist) / gd);
+  if (er.pdf == 0.0f || isinf(er.pdf)) return v3(0.0f);
+  vec3 le = dot(er.n, er.wi) < 0.0f ? ld3(m.radiance) : v3(0.0f);
+  return le / er.pdf;
+}
+
+constexpr float kTwoPiF = 6.28318548202514648f;
+constexpr float kPi2F = 9.8696044010893586f;  // float(pi * pi)
+
+// EnvironmentLight::Sample (src/emitter/EnvironmentLight.cpp:40-76)
+__device__ __forceinline__ vec3 env_sample(const DevScene &sc, vec3 ref, float dist, float sx, float sy, ERec &er) {
+  float py, px;
+  int iy, ix;
+  float v = cdf_sample_cont(sc.env_marg_cdf, sc.env_marg_func, sc.env_h, sc.env_marg_int, sy, py, iy);
+  float u = cdf_sample_cont(sc.env_row_cdf + size_t(iy) * (sc.env_w + 1), sc.env_row_func + size_t(iy) * sc.env_w,
+                            sc.env_w, sc.env_row_int[iy], sx, px, ix);
+  float map_pdf = px * py;
+  float theta = v * kPiF;
+  float st = sinf(theta);
+  if (map_pdf == 0.0f) return v3(0.0f);
+  if (st == 0.0f) return v3(0.0f);
+  er.pdf = map_pdf / (2.0f * kPi2F * st);
+  float phi = u * kTwoPiF;
+  vec3 dir = v3(st * cosf(phi), st * sinf(phi), cosf(theta));
+  er.wi = mat3_mul(sc.env_to_world, dir);
+  er.n = -er.wi;
+  er.dist = dist;
+  er.p = ref + er.wi * dist * 2.0f;
+  vec3 rad = ld3(sc.env_tex + 3 * (size_t(iy) * sc.env_w + ix));
+  if (!color_valid(rad)) return v3(0.0f);
+  return rad * sc.env_scale / er.pdf;
+}
+// EnvironmentLight::Eval (src/emitter/EnvironmentLight.cpp:92-111)
+__device__ __forceinline__ vec3 env_eval(const DevScene &sc, vec3 wi) {
+  vec3 l = mat3_mul(sc.env_to_local, wi);
+  float theta = acosf(l.z), phi = atan2f(l.y, l.x);
+  if (phi < 0.0f) phi += 2.0f * kPiF;
+  if (isnan(theta) || isnan(phi)) return v3(0.0f);
+  float X = clampf(phi / kTwoPiF * (float)sc.env_w, 0.0f, (float)sc.env_w - 1.0f);
+  float Y = clampf(theta / kPiF * (float)sc.env_h, 0.0f, (float)sc.env_h - 1.0f);
+  vec3 rad = ld3(sc.env_tex + 3 * (size_t((int)Y) * sc.env_w + (int)X));
+  if (!color_valid(rad)) return v3(0.0f);
+  return rad * sc.env_scale;
+}
+
+// Triangle::PostIntersect (src/core/Mesh.cpp:67-172): position and shading frame.
+__device__ __forceinline__ void post_intersect(const DevScene &sc, int32_t prim, float u, float v, vec3 &P, Frame &sh,
+                                               uint32_t &mesh_id) {
+  mesh_id = sc.prim_mesh[prim];
+  const DevMesh &m = sc.meshes[mesh_id];
+  float b0 = 1.0f - (u + v), b1 = u, b2 = v;
+  const uint32_t *f = sc.F + 3 * size_t(prim);
+  uint32_t i0 = f[0], i1 = f[1], i2 = f[2];
+  vec3 p0 = ld3(sc.V + 3 * size_t(i0)), p1 = ld3(sc.V + 3 * size_t(i1)), p2 = ld3(sc.V + 3 * size_t(i2));
+  P = (b0 * p0 + b1 * p1) + b2 * p2;
+  if (m.has_n) {
+    vec3 ns = (b0 * ld3(sc.N + 3 * size_t(i0)) + b1 * ld3(sc.N + 3 * size_t(i1))) + b2 * ld3(sc.N + 3 * size_t(i2));
+    if (m.has_uv) {
+      vec3 dp1 = p1 - p0, dp2 = p2 - p0;
+      if (norm(cross(dp1, dp2)) == 0.0f) {
+        sh = frame_from_n(normalized(cross(p1 - p0, p2 - p0)));
+        return;
+      }
+      const float *uv = sc.UV;
+      float du1x = uv[2 * i1] - uv[2 * i0], du1y = uv[2 * i1 + 1] - uv[2 * i0 + 1];
+      float du2x = uv[2 * i2] - uv[2 * i0], du2y = uv[2 * i2 + 1] - uv[2 * i0 + 1];
+      float det = du1x * du2y - du1y * du2x;
+      if (det == 0.0f) {
+        sh = frame_from_n(normalized(ns));
+      } else {
+        float idet = 1.0f / det;
+        vec3 dpdu = (du2y * dp1 - du1y * dp2) * idet;
+        float iln = 1.0f / norm(ns);
+        sh = frame_from_n_dpdu(ns * iln, dpdu);
+      }
+    } else {
+      sh = frame_from_n(normalized(ns));
+    }
+  } else {
+    sh = frame_from_n(normalized(cross(p1 - p0, p2 - p0)));
+  }
+}
+
+constexpr uint32_t kFlagPrevDiscrete = 1u;
+
+// ------------------------------------------------------------------ k_camera
+// RenderBlock's per-sample prologue (src/Main.cpp:22-31) + PerspectiveCamera::
+// SampleRay (src/camera/PerspectiveCamera.cpp:77-102).
+__global__ __launch_bounds__(kBlock) void k_camera(RenderArgs A) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[0] = A.n_paths;
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < A.n_paths; id += stride) {
+    const WorkItem it = A.items[find_item(A.items, A.n_items, id)];
+    const DevRequest &rq = A.reqs[it.req];
+    uint32_t local = id - it.path_begin;
+    uint32_t q = rq.pix0 + local % it.npix, s = it.s0 + local / it.npix;
+    uint32_t x = q % (uint32_t)rq.width, y = q / (uint32_t)rq.width;
+    Rng rng = rng_for_path(rq.seed, q, s);
+    float jx = rng.next(), jy = rng.next();
+    rng.next();
+    rng.next();  // aperture sample, unused by the pinhole camera
+    float px = (float)x + jx, py = (float)y + jy;
+    float X = px * rq.inv_w, Y = py * rq.inv_h;
+    const float *m = rq.s2c;
+    float hx = m[0] * X + m[1] * Y + m[3], hy = m[4] * X + m[5] * Y + m[7];
+    float hz = m[8] * X + m[9] * Y + m[11], hw = m[12] * X + m[13] * Y + m[15];
+    vec3 dir = normalized(v3(hx / hw, hy / hw, hz / hw));
+    float inv_z = 1.0f / dir.z;
+    const float *c = rq.c2w;
+    vec3 o = v3(c[3] / c[15], c[7] / c[15], c[11] / c[15]);
+    vec3 d = v3(c[0] * dir.x + c[1] * dir.y + c[2] * dir.z, c[4] * dir.x + c[5] * dir.y + c[6] * dir.z,
+                c[8] * dir.x + c[9] * dir.y + c[10] * dir.z);
+    A.q_path[0][id] = id;
+    A.q_o[0][id] = make_float4(o.x, o.y, o.z, rq.near_clip * inv_z);
+    A.q_d[0][id] = make_float4(d.x, d.y, d.z, rq.far_clip * inv_z);
+    A.ps_beta[id] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    A.ps_L[id] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    A.ps_rng[id] = rng.state;
+    A.ps_pix[id] = q;
+    A.ps_meta[id] = 0u;
+  }
+}
+
+// ------------------------------------------------------------------ k_extend
+__global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
+  __shared__ int2 stack[kLdsStack * kBlock];
+  uint32_t count = A.counters[qi];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    A.counters[qi ^ 1] = 0u;
+    A.counters[2] = 0u;
+    atomicAdd(&A.stats[0], (unsigned long long)count);
+  }
+  uint32_t stride = gridDim.x * blockDim.x;
+  const float4 *qo = A.q_o[qi], *qd = A.q_d[qi];
+  float4 *qh = A.q_hit[qi];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    float4 a = qo[i], b = qd[i];
+    float t, u, v;
+    int32_t prim;
+    bool h = trace_ray<false>(A.sc.nodes, A.sc.tris, A.sc.root_ref, stack, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z),
+                              a.w, b.w, t, u, v, prim);
+    qh[i] = h ? make_float4(t, u, v, __int_as_float(prim)) : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+  }
+}
+
+// ------------------------------------------------------------------ k_shade
+// One bounce of PathMISIntegrator::Li for every queued path.
+__global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
+  const DevScene &sc = A.sc;
+  uint32_t count = A.counters[qi];
+  int qn = qi ^ 1;
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    bool active = i < count;
+    bool cont = false;
+    uint32_t smask = 0, path = 0;
+    vec3 P = v3(0.0f), nd = v3(0.0f);
+    if (active) {
+      path = A.q_path[qi][i];
+      float4 o4 = A.q_o[qi][i], d4 = A.q_d[qi][i], h4 = A.q_hit[qi][i];
+      float4 b4 = A.ps_beta[path], l4 = A.ps_L[path];
+      uint32_t meta = A.ps_meta[path];
+      Rng rng;
+      rng.state = A.ps_rng[path];
+      rng.inc = rng_inc(A.ps_pix[path]);
+      uint32_t depth = meta & 0xffffu, flags = meta >> 16;
+      vec3 beta = v3(b4.x, b4.y, b4.z), L = v3(l4.x, l4.y, l4.z);
+      float pdf_b = b4.w, w_mats = l4.w;
+      vec3 ro = v3(o4.x, o4.y, o4.z), rd = v3(d4.x, d4.y, d4.z);
+      int32_t prim = __float_as_int(h4.w);
+      if (prim < 0) {  // escaped (PathMISIntegrator.cpp:33-66)
+        if (depth == 0) {
+          if (sc.has_env && !sc.force_background) L = L + beta * env_eval(sc, rd) / 1.0f;
+          else L = ld3(sc.background);
+        } else if (sc.has_env) {
+          L = L + beta * env_eval(sc, rd) / 1.0f;
+        }
+      } else {
+        Frame fr;
+        uint32_t mesh_id;
+        post_intersect(sc, prim, h4.y, h4.z, P, fr, mesh_id);
+        const DevMesh &mesh = sc.meshes[mesh_id];
+        int mat = (int)mesh.material;
+        if (mesh.emitter >= 0) {  // :73-81 (+ the MATS weight of :122-140)
+          vec3 wi = P - ro;
+          float dist = norm(wi);
+          wi = wi / dist;
+          if (depth > 0) {
+            float gd = fabsf(dot(-wi, fr.n));
+            float pl = gd == 0.0f ? 0.0f : mesh.inv_area * ((dist * dist) / gd);
+            if (pdf_b + pl != 0.0f) w_mats = pdf_b / (pdf_b + pl);
+            if (flags & kFlagPrevDiscrete) w_mats = 1.0f;
+          }
+          vec3 le = dot(fr.n, wi) < 0.0f ? ld3(mesh.radiance) : v3(0.0f);
+          L = L + beta * w_mats * le;
+        }
+        vec3 wi_loc = to_local(fr, -1.0f * rd);
+        // emitter sampling over every emitter (:83-109)
+        for (int e = 0; e < sc.n_emitters; e++) {
+          float sx = rng.next(), sy = rng.next(), s1 = rng.next();
+          DevEmitter em = sc.emitters[e];
+          ERec er;
+          vec3 ld = em.kind == 0 ? area_sample(sc, sc.meshes[em.mesh], P, sx, sy, s1, er)
+                                 : env_sample(sc, P, sc.bbox_radius, sx, sy, er);
+          if (!is_zero_fuzzy(ld)) {
+            BRec br;
+            br.wi = wi_loc;
+            br.wo = to_local(fr, er.wi);
+            br.measure = kSolidAngle;
+            br.eta = 1.0f;
+            float pb = bsdf_pdf(sc.mats, sc.rft, mat, br);
+            float w_ems = (er.pdf + pb != 0.0f) ? er.pdf / (er.pdf + pb) : 1.0f;
+            vec3 c = beta * bsdf_eval(sc.mats, sc.rft, mat, br) * fabsf(br.wo.z) * ld * w_ems;
+            if (!(c.x == 0.0f && c.y == 0.0f && c.z == 0.0f)) {
+              size_t slot = size_t(e) * A.n_paths + path;
+              vec3 sd = er.p - P;
+              A.sh_o[slot] = make_float4(P.x, P.y, P.z, 0.0f);
+              A.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, 0.0f);
+              A.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.0f);
+              smask |= 1u << e;
+            }
+          }
+        }
+        // BSDF sampling (:111-120), MIS bookkeeping (:122-140), RR (:143-151)
+        BRec br;
+        br.wi = wi_loc;
+        br.wo = v3(0.0f);
+        br.eta = 1.0f;
+        br.measure = kUnknownMeasure;
+        float sx = rng.next(), sy = rng.next();
+        U1 u1{&rng, nullptr, 0, 0};
+        vec3 f = bsdf_sample(sc.mats, sc.rft, mat, br, sx, sy, u1);
+        nd = to_world(fr, br.wo);
+        beta = beta * f;
+        if (!is_zero_fuzzy(beta)) {
+          pdf_b = bsdf_pdf(sc.mats, sc.rft, mat, br);
+          flags = 0u;
+          if (br.measure == kDiscrete) {
+            w_mats = 1.0f;
+            flags = kFlagPrevDiscrete;
+          }
+          if (rng.next() < 0.95f) {
+            beta = beta * (1.0f / 0.95f);
+            depth++;
+            cont = depth < (uint32_t)sc.max_depth;
+          }
+        }
+      }
+      A.ps_beta[path] = make_float4(beta.x, beta.y, beta.z, pdf_b);
+      A.ps_L[path] = make_float4(L.x, L.y, L.z, w_mats);
+      A.ps_rng[path] = rng.state;
+      A.ps_meta[path] = (depth & 0xffffu) | (flags << 16);
+    }
+    uint32_t slot = wave_append(&A.counters[qn], cont);
+    if (cont) {
+      A.q_path[qn][slot] = path;
+      A.q_o[qn][slot] = make_float4(P.x, P.y, P.z, 1e-4f);
+      A.q_d[qn][slot] = make_float4(nd.x, nd.y, nd.z, kInfF);
+    }
+    uint32_t ss = wave_append(&A.counters[2], smask != 0u);
+    if (smask != 0u) {
+      A.s_path[ss] = path;
+      A.s_mask[ss] = smask;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ k_shadow
+// Scene::ShadowRayIntersect (src/core/Scene.cpp:122-126) for the queued
+// emitter samples; one lane owns a path, so its radiance needs no atomics.
+__global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A) {
+  __shared__ int2 stack[kLdsStack * kBlock];
+  uint32_t count = A.counters[2];
+  uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t traced = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    uint32_t path = A.s_path[i], mask = A.s_mask[i];
+    float4 l4 = A.ps_L[path];
+    vec3 L = v3(l4.x, l4.y, l4.z);
+    while (mask) {
+      int e = __ffs((int)mask) - 1;
+      mask &= mask - 1u;
+      size_t slot = size_t(e) * A.n_paths + path;
+      float4 o = A.sh_o[slot], d = A.sh_d[slot], c = A.sh_c[slot];
+      float t, u, v;
+      int32_t prim;
+      traced++;
+      bool occ = trace_ray<true>(A.sc.nodes, A.sc.tris, A.sc.root_ref, stack, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z),
+                                 1e-4f, 1.0f - 1e-4f, t, u, v, prim);
+      if (!occ) L = L + v3(c.x, c.y, c.z);
+    }
+    A.ps_L[path] = make_float4(L.x, L.y, L.z, l4.w);
+  }
+  // one atomic per wave for the statistics
+  for (int off = 32; off > 0; off >>= 1) traced += __shfl_down(traced, off);
+  if (lane_id() == 0 && traced) atomicAdd(&A.stats[1], (unsigned long long)traced);
+}
+
+// ------------------------------------------------------------------ k_resolve
+// ImageBlock::Put (src/core/Block.cpp:95-134) as a gather: each output pixel
+// owns its accumulator and sums the filtered samples of its neighbours.
+__global__ __launch_bounds__(kBlock) void k_resolve(RenderArgs A, uint32_t item_idx) {
+  const WorkItem it = A.items[item_idx];
+  const DevRequest &rq = A.reqs[it.req];
+  int W = rq.width, H = rq.height;
+  float R = rq.filter_radius;
+  int rad = (int)floorf(R + 0.5f);
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < it.npix; p += stride) {
+    int x = (int)(p % (uint32_t)W), y = (int)(p / (uint32_t)W);
+    float4 *accum = reinterpret_cast<float4 *>(rq.accum);
+    float4 acc = accum[p];
+    for (uint32_t s = 0; s < it.k; s++) {
+      for (int qy = max(0, y - rad); qy <= min(H - 1, y + rad); qy++) {
+        for (int qx = max(0, x - rad); qx <= min(W - 1, x + rad); qx++) {
+          uint32_t q = (uint32_t)(qy * W + qx);
+          if (q >= it.npix) continue;
+          Rng rng = rng_for_path(rq.seed, q, it.s0 + s);
+          float jx = rng.next(), jy = rng.next();
+          float cx = ((float)qx + jx) - 0.5f, cy = ((float)qy + jy) - 0.5f;
+          if (x < (int)ceilf(cx - R) || x > (int)floorf(cx + R) || y < (int)ceilf(cy - R) || y > (int)floorf(cy + R))
+            continue;
+          uint32_t path = it.path_begin + s * it.npix + q;
+          float4 l4 = A.ps_L[path];
+          vec3 L = v3(l4.x, l4.y, l4.z);
+          if (!color_valid(L)) continue;  // invalid samples are dropped (Block.cpp:97-101)
+          float wx = rq.filter[(int)(fabsf((float)x - cx) * rq.lookup)];
+          float wy = rq.filter[(int)(fabsf((float)y - cy) * rq.lookup)];
+          acc.x += L.x * wx * wy;
+          acc.y += L.y * wx * wy;
+          acc.z += L.z * wx * wy;
+          acc.w += 1.0f * wx * wy;
+        }
+      }
+    }
+    accum[p] = acc;
+  }
+}
+
+// Final ToBitmap (src/core/Block.cpp:61-72): rgb / w.
+__global__ __launch_bounds__(kBlock) void k_finalize(const float4 *accum, float *rgb, uint32_t n) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    float4 a = accum[p];
+    bool z = a.w == 0.0f;
+    rgb[3 * p] = z ? 0.0f : a.x / a.w;
+    rgb[3 * p + 1] = z ? 0.0f : a.y / a.w;
+    rgb[3 * p + 2] = z ? 0.0f : a.z / a.w;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ host launchers
+static uint32_t grid_for(uint32_t n, uint32_t cap) {
+  uint32_t g = (n + kBlock - 1) / kBlock;
+  if (g > cap) g = cap;
+  return g < 1 ? 1 : g;
+}
+
+hipError_t launch_intersect(const Bvh4Node *nodes, const TraceTri *tris, int32_t root, const void *rays, void *hits,
+                            uint32_t n, int any, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_intersect, dim3(grid_for(n, 1u << 20)), dim3(kBlock), 0, st, nodes, tris, root,
+                     (const float4 *)rays, (float4 *)hits, n, any);
+  return hipGetLastError();
+}
+
+hipError_t launch_bsdf_batch(const DevMaterial *mats, const float *rft, int idx, uint32_t n, const float *wi,
+                             const float *wo, const int32_t *meas, const float *u, float *out_eval, float *out_pdf,
+                             float *out_f, float *out_wo, int32_t *out_meas, float *out_eta, float *out_spdf,
+                             hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bsdf_batch, dim3(grid_for(n, 1u << 16)), dim3(kBlock), 0, st, mats, rft, idx, n, wi, wo, meas,
+                     u, out_eval, out_pdf, out_f, out_wo, out_meas, out_eta, out_spdf);
+  return hipGetLastError();
+}
+
+hipError_t launch_camera(const RenderArgs &A, uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_camera, dim3(grid_for(A.n_paths, grid)), dim3(kBlock), 0, st, A);
+  return hipGetLastError();
+}
+hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_extend, dim3(grid), dim3(kBlock), 0, st, A, qi);
+  return hipGetLastError();
+}
+hipError_t launch_shade(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, st, A, qi);
+  return hipGetLastError();
+}
+hipError_t launch_shadow(const RenderArgs &A, uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(kBlock), 0, st, A);
+  return hipGetLastError();
+}
+hipError_t launch_resolve(const RenderArgs &A, uint32_t item, uint32_t npix, hipStream_t st) {
+  hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, 1u << 16)), dim3(kBlock), 0, st, A, item);
+  return hipGetLastError();
+}
+hipError_t launch_finalize(const void *accum, float *rgb, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(n, 1u << 16)), dim3(kBlock), 0, st, (const float4 *)accum, rgb, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace hk

@@ -1,0 +1,148 @@
+// hk_runtime.h -- host-side objects behind the opaque C-ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hikari_amd.h"
+#include "hk_shared.h"
+
+namespace hk {
+
+struct HkError {
+  int code;
+  std::string msg;
+  HkError(int c, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+};
+void hip_check(hipError_t e, const char *what);
+std::string default_data_dir();
+
+// Owned device allocation.
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() {}
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void alloc(size_t n);
+  void release();
+  void upload(const void *src, size_t n, hipStream_t st);
+  template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct DevBvh {
+  DBuf nodes, tris;
+  uint32_t n_nodes = 0, n_tris = 0, n_leaves = 0, max_depth = 0;
+};
+
+constexpr uint32_t kDefaultMaxPaths = 1u << 22;
+constexpr int kMaxEmitters = 32;
+
+// Wavefront buffers, grown on demand and reused across renders of a context.
+struct Workspace {
+  uint32_t cap_paths = 0, cap_emitters = 0;
+  DBuf ps_beta, ps_L, ps_rng, ps_pix, ps_meta;
+  DBuf q_path[2], q_o[2], q_d[2], q_hit[2];
+  DBuf s_path, s_mask, sh_o, sh_d, sh_c;
+  DBuf counters, stats;
+  DBuf reqs, items;
+  void ensure(uint32_t paths, uint32_t emitters);
+  void release();
+};
+
+}  // namespace hk
+
+struct hk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int n_cu = 0;
+  std::string arch;
+  uint32_t max_paths = hk::kDefaultMaxPaths;
+  uint32_t grid = 0;  // persistent grid for queue kernels (0: automatic)
+  bool profile = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  uint32_t *host_counters = nullptr;  // pinned
+  hk::Workspace ws;
+};
+
+struct hk_accel {
+  hk_ctx *ctx = nullptr;
+  hk::DevBvh bvh;
+  double build_ms = 0;
+};
+
+struct hk_scene {
+  hk_ctx *ctx = nullptr;
+  hk::DevScene dev;
+  hk::DevBvh bvh;
+  hk::DBuf V, N, UV, F, prim_mesh, meshes, mats, rft, emitters, cdf;
+  hk::DBuf env_tex, env_row_cdf, env_row_func, env_row_int, env_marg_cdf, env_marg_func;
+  hk_scene_desc desc;             // camera / film defaults (pointers not retained)
+  uint32_t n_tris = 0;
+  uint64_t bytes = 0;
+};
+
+namespace hk {
+// 4x4 helpers in double (the camera / env transforms are set up once).
+struct Mat4 {
+  double m[4][4];
+  static Mat4 from(const float *a) {
+    Mat4 r;
+    for (int i = 0; i < 16; i++) r.m[i / 4][i % 4] = a[i];
+    return r;
+  }
+  static Mat4 identity() {
+    Mat4 r;
+    for (int i = 0; i < 16; i++) r.m[i / 4][i % 4] = (i / 4 == i % 4);
+    return r;
+  }
+  Mat4 operator*(const Mat4 &b) const {
+    Mat4 r;
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) {
+        double s = 0;
+        for (int k = 0; k < 4; k++) s += m[i][k] * b.m[k][j];
+        r.m[i][j] = s;
+      }
+    return r;
+  }
+  Mat4 inverse() const {
+    double a[4][8];
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) { a[i][j] = m[i][j]; a[i][j + 4] = i == j; }
+    for (int c = 0; c < 4; c++) {
+      int p = c;
+      for (int r = c + 1; r < 4; r++)
+        if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+      if (a[p][c] == 0.0) throw HkError(HK_ERR_INVALID, "singular transform");
+      for (int j = 0; j < 8; j++) std::swap(a[c][j], a[p][j]);
+      double d = a[c][c];
+      for (int j = 0; j < 8; j++) a[c][j] /= d;
+      for (int r = 0; r < 4; r++)
+        if (r != c) {
+          double f = a[r][c];
+          for (int j = 0; j < 8; j++) a[r][j] -= f * a[c][j];
+        }
+    }
+    Mat4 r;
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) r.m[i][j] = a[i][j + 4];
+    return r;
+  }
+  void to_float(float *out) const {
+    for (int i = 0; i < 16; i++) out[i] = float(m[i / 4][i % 4]);
+  }
+};
+
+void build_cdf(const float *func, int n, float *cdf, float &integral);
+void camera_setup(const float *c2w, float fov, int w, int h, float near_clip, float far_clip, DevRequest &r);
+void filter_setup(const hk_scene_desc &d, DevRequest &r);
+void upload_bvh(hk_ctx *ctx, const float *tri9, uint32_t nt, DevBvh &out);
+void build_materials(const float *recs, int n, const char *rft_dir, std::vector<DevMaterial> &out,
+                     std::vector<float> &rft_pool);
+}  // namespace hk

@@ -1,0 +1,185 @@
+/*
+ * hikari_amd.h -- C ABI of the MI355X path-tracing core (libhikari_amd.so).
+ *
+ * Drop-in boundary for Hikari's pathMIS hot path.  Each entry point names the
+ * reference interface it replaces.  Conventions:
+ *   - every call returns HK_OK (0) or a negative HK_ERR_* code; the message of
+ *     the last failure on a context is returned by hk_last_error();
+ *   - host pointers are caller-owned and only read/written during the call;
+ *     *_device variants take device pointers (HBM-resident inputs/outputs);
+ *   - one context = one device + one HIP stream; calls on one context are
+ *     serialised by the caller (the reference's Scene/Accel are read-only
+ *     after Activate(), include/core/Integrator.hpp:35; the same holds here).
+ *   - no CPU fallback: if no HIP device is usable, hk_ctx_create fails.
+ */
+#ifndef HIKARI_AMD_H
+#define HIKARI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HK_OK 0
+#define HK_ERR_INVALID (-1)
+#define HK_ERR_HIP (-2)
+#define HK_ERR_OOM (-3)
+#define HK_ERR_UNSUPPORTED (-4)
+#define HK_ERR_IO (-5)
+#define HK_ERR_BUDGET (-6)
+
+typedef struct hk_ctx hk_ctx;
+typedef struct hk_accel hk_accel;
+typedef struct hk_scene hk_scene;
+
+/* ------------------------------------------------------------ context */
+/* Opens `device` and creates the context's stream.  No reference counterpart
+ * (the reference is single-process CPU code, src/Main.cpp:151-205). */
+int hk_ctx_create(int device, hk_ctx **out);
+void hk_ctx_destroy(hk_ctx *ctx);
+/* Message of the last failed call on ctx (or of the last failed
+ * hk_ctx_create when ctx is NULL). */
+const char *hk_last_error(const hk_ctx *ctx);
+const char *hk_version(void);
+/* Synchronises the context stream. */
+int hk_ctx_sync(hk_ctx *ctx);
+
+/* ------------------------------------------------------------ acceleration */
+/* One ray: origin, tmin, direction (need not be normalised), tmax. */
+typedef struct hk_ray {
+  float ox, oy, oz, tmin;
+  float dx, dy, dz, tmax;
+} hk_ray;
+/* Closest (or any) hit: t, barycentric u, v of Moller-Trumbore, and the
+ * global triangle index (mesh-major, face-minor, as the reference's shape list,
+ * src/core/Acceleration.cpp:13-28); prim = -1 on a miss. */
+typedef struct hk_hit {
+  float t, u, v;
+  int32_t prim;
+} hk_hit;
+
+typedef struct hk_accel_info {
+  uint32_t n_triangles, n_nodes, n_leaves, max_depth;
+  uint64_t node_bytes, triangle_bytes;
+  double build_ms;
+} hk_accel_info;
+
+/* Replaces Acceleration::AddMesh + Build (include/core/Acceleration.hpp:27-45,
+ * src/acceleration/BVHAcceleration.cpp:46-248): builds a 4-wide SAH BVH over
+ * nt triangles (idx: nt x 3 into v: nv x 3) and uploads it to HBM. */
+int hk_accel_build(hk_ctx *ctx, const float *v, uint32_t nv, const uint32_t *idx, uint32_t nt, hk_accel **out);
+void hk_accel_destroy(hk_accel *accel);
+int hk_accel_get_info(const hk_accel *accel, hk_accel_info *info);
+
+/* Replaces Acceleration::RayIntersect(ray, isect, bShadowRay)
+ * (include/core/Acceleration.hpp:57, src/core/Acceleration.cpp:45-80) for a
+ * batch of n rays.  Closest hit (any_hit = 0) returns the Moller-Trumbore hit
+ * of the reference bit for bit; among equal t the largest prim wins (the
+ * reference's brute-force order, where the last tested triangle wins).  Any
+ * hit (any_hit = 1) reports some hit in [tmin, tmax] (prim >= 0) or a miss.
+ * Host buffers. */
+int hk_accel_intersect(hk_accel *accel, const hk_ray *rays, uint32_t n, hk_hit *hits, int any_hit);
+/* Same on device buffers (d_rays, d_hits in HBM) on the context stream;
+ * asynchronous. */
+int hk_accel_intersect_device(hk_accel *accel, const hk_ray *d_rays, uint32_t n, hk_hit *d_hits, int any_hit);
+
+/* ------------------------------------------------------------ BSDFs */
+/* Material records: 32 floats each.  [0] type (0 diffuse, 1 conductor,
+ * 2 dielectric, 3 plastic, 4 roughConductor, 5 roughDielectric, 6 roughPlastic,
+ * 7 coating, 8 roughCoating), [1] distribution (0 beckmann, 1 ggx), [2] nested
+ * material index (coatings), [3] flags (1 nonlinear, 2 anisotropic),
+ * [4..6] colour A, [8..10] colour B, [20] intIOR, [21] extIOR, [22] alpha or
+ * alphaU, [23] alphaV, [24] thickness.  Colour A/B: diffuse albedo/-;
+ * (rough)conductor ks/k; (rough)dielectric ksReflect/ksRefract; (rough)plastic
+ * ks/kd; (rough)coating sigmaA/ks.  The XML properties of the src/bsdf classes. */
+#define HK_MATERIAL_STRIDE 32
+
+/* Batched BSDF::Eval / Pdf / Sample (include/core/BSDF.hpp:67-94) of
+ * material `index` on the GPU, for parity tests.  wi, wo: n x 3 (local frame);
+ * measure: n (1 solid angle, 2 discrete); u: n x 4 uniforms (u[0], u[1] the
+ * Sample() point, u[2], u[3] the extra draws Sample() makes from the sampler).
+ * Outputs (n x 3 / n): eval, pdf, and for Sample(): f, wo, measure, eta, and
+ * Pdf() of the sampled record.  rft_dir: directory holding
+ * BeckmannRFTData.bin / GGXRFTData.bin (NULL: the library's data dir). */
+int hk_bsdf_batch(hk_ctx *ctx, const float *materials, uint32_t n_materials, uint32_t index, const char *rft_dir,
+                  uint32_t n, const float *wi, const float *wo, const int32_t *measure, const float *u,
+                  float *out_eval, float *out_pdf, float *out_f, float *out_wo, int32_t *out_measure,
+                  float *out_eta, float *out_spdf);
+
+/* ------------------------------------------------------------ scenes */
+typedef struct hk_mesh_desc {
+  const float *v; uint32_t nv;       /* nv x 3 positions (world space) */
+  const float *n; const float *uv;   /* nullable: nv x 3 normals, nv x 2 uv */
+  const uint32_t *f; uint32_t nt;    /* nt x 3 vertex indices */
+  int32_t material;                  /* index into the material table */
+  int32_t is_emitter; float radiance[3]; /* area light (src/emitter/AreaLight.cpp) */
+} hk_mesh_desc;
+
+typedef struct hk_scene_desc {
+  const hk_mesh_desc *meshes; int32_t n_meshes;
+  const float *materials; int32_t n_materials;
+  const float *env; int32_t env_w, env_h;      /* nullable lat-long RGB32F map, rows top to bottom */
+  float env_scale; float env_to_world[16];      /* row-major 4x4 */
+  float cam_to_world[16];                       /* row-major 4x4 */
+  float fov, near_clip, far_clip; int32_t width, height;
+  int32_t filter;                               /* 0 box, 1 gaussian, 2 tent, 3 mitchell */
+  float filter_radius, filter_stddev, filter_b, filter_c;
+  int32_t max_depth; float background[3]; int32_t force_background;
+  const char *rft_dir;                          /* nullable */
+} hk_scene_desc;
+
+/* Replaces Scene::Activate (src/core/Scene.cpp:128-165) plus the Activate()
+ * of every mesh / BSDF / emitter / camera: uploads geometry, BVH, material
+ * table (with reduced rough-transmittance tables), emitter CDFs and the
+ * environment map to HBM.  Emitters are ordered area lights in mesh order,
+ * then the environment. */
+int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *desc, hk_scene **out);
+void hk_scene_destroy(hk_scene *scene);
+/* Bytes of HBM the scene holds. */
+uint64_t hk_scene_device_bytes(const hk_scene *scene);
+
+/* ------------------------------------------------------------ rendering */
+/* One render request: camera pose and film of the request; spp samples per
+ * pixel numbered [spp_offset, spp_offset + spp); `seed` keys the per-path
+ * pcg32 streams together with pixel and sample index. */
+typedef struct hk_request {
+  float cam_to_world[16]; float fov; int32_t width, height;
+  uint32_t spp, spp_offset, seed;
+  int32_t use_scene_camera;                     /* 1: take pose/fov/size from the scene */
+} hk_request;
+
+typedef struct hk_render_stats {
+  uint64_t paths, extension_rays, shadow_rays, bounces_launched;
+  double gpu_ms;          /* device time of the render (HIP events) */
+  double trace_ms;        /* device time inside the closest-hit kernel */
+  uint64_t trace_launches;
+} hk_render_stats;
+
+/* Whole-image render with the gpuPathMIS integrator (the wavefront form of
+ * PathMISIntegrator::Li, src/integrator/PathMISIntegrator.cpp:16-157, driven
+ * as src/Main.cpp:11-147).  out_rgb: host, height x width x 3, the filtered
+ * image (ImageBlock::ToBitmap).  Synchronous. */
+int hk_render(hk_scene *scene, const hk_request *req, float *out_rgb, hk_render_stats *stats);
+
+/* Per-sample radiance of one pixel for samples [s0, s0 + n) (the t-test
+ * harness of src/core/StudentTTest.cpp:137-175): out n x 5 (r, g, b, px, py). */
+int hk_render_pixel_samples(hk_scene *scene, uint32_t seed, uint32_t x, uint32_t y, uint32_t s0, uint32_t n,
+                            float *out);
+
+/* Batched renders: n requests over one resident scene share one wavefront
+ * (paths tagged by request).  out_rgb[i]: host buffer of request i
+ * (height x width x 3).  Results equal the requests rendered alone, bit for
+ * bit.  max_paths bounds the wavefront (0: automatic). */
+int hk_render_batch(hk_scene *scene, const hk_request *reqs, uint32_t n, float **out_rgb, uint32_t max_paths,
+                    hk_render_stats *stats);
+
+/* Tuning knobs (0 keeps the default): wavefront capacity in paths and
+ * samples per pixel per pass. */
+int hk_set_option(hk_ctx *ctx, const char *name, int64_t value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIKARI_AMD_H */

@@ -1,0 +1,85 @@
+"""Shared helpers for the parity tests: material tables, ray batches, statistics."""
+import numpy as np
+
+from hikari_amd import desc as D
+
+# One record per BSDF configuration exercised by the parity tests (index ->
+# record); coatings refer to nested records by index inside the same table.
+def material_table():
+    recs = [
+        D.diffuse((0.7, 0.5, 0.3)),                                            # 0
+        D.conductor(int_ior=0.47, ext_ior=1.0, k=(3.98, 2.39, 1.60)),           # 1
+        D.dielectric(int_ior=1.5, ext_ior=1.0),                                 # 2
+        D.plastic(int_ior=1.5, ext_ior=1.0, kd=(0.2, 0.5, 0.8)),                # 3
+        D.rough_conductor(alpha=0.3, dist=D.GGX, int_ior=0.47, ext_ior=1.0, k=(3.98, 2.39, 1.60)),  # 4
+        D.rough_conductor(alpha=0.1, dist=D.BECKMANN, k=(3.9, 2.4, 2.1)),       # 5
+        D.rough_dielectric(alpha=0.3, dist=D.BECKMANN, int_ior=1.5, ext_ior=1.0),  # 6
+        D.rough_dielectric(alpha=0.15, dist=D.GGX, int_ior=1.5, ext_ior=1.0),   # 7
+        D.rough_plastic(alpha=0.3, dist=D.GGX, kd=(0.6, 0.3, 0.2)),             # 8
+        D.rough_plastic(alpha=0.15, dist=D.BECKMANN, kd=0.5, nonlinear=True),   # 9
+        D.coating(4, int_ior=1.5),                                              # 10 coated rough conductor
+        D.rough_coating(0, alpha=0.2, dist=D.GGX, sigma_a=(0.1, 0.2, 0.3), thickness=1.0),  # 11
+        D.rough_conductor(alpha=0.1, dist=D.GGX, alpha_v=0.4, k=(3.9, 2.4, 2.1)),  # 12 anisotropic
+        D.coating(0, int_ior=1.49, sigma_a=(0.2, 0.1, 0.05)),                   # 13 coated diffuse
+    ]
+    return np.stack(recs).astype(np.float32)
+
+
+def random_dirs(rng, n, upper=None):
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    if upper is True:
+        d[:, 2] = np.abs(d[:, 2])
+    return d.astype(np.float32)
+
+
+def scene_triangles(scene):
+    tris = []
+    for m in scene.meshes:
+        tris.append(m.v[m.f].reshape(-1, 9))
+    return np.concatenate(tris).astype(np.float32)
+
+
+def scene_vf(scene):
+    vs, fs, off = [], [], 0
+    for m in scene.meshes:
+        vs.append(m.v)
+        fs.append(m.f.astype(np.int64) + off)
+        off += len(m.v)
+    return np.concatenate(vs).astype(np.float32), np.concatenate(fs).astype(np.uint32)
+
+
+def box_rays(rng, n, lo, hi, tmin=1e-4, tmax=np.inf):
+    o = rng.uniform(lo, hi, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros((n, 8), np.float32)
+    r[:, :3], r[:, 3], r[:, 4:7], r[:, 7] = o, tmin, d, tmax
+    return r
+
+
+def grazing_rays(rng, tris, n):
+    """Rays aimed at triangle edges and vertices from random directions (edge cases of the M-T test)."""
+    k = rng.integers(0, len(tris), n)
+    t = tris[k].reshape(n, 3, 3).astype(np.float64)
+    a = rng.integers(0, 3, n)
+    b = (a + 1) % 3
+    w = rng.choice([0.0, 0.5, 1.0, rng.uniform()], size=n)
+    target = t[np.arange(n), a] * (1 - w)[:, None] + t[np.arange(n), b] * w[:, None]
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = target - d * rng.uniform(0.05, 3.0, size=(n, 1))
+    r = np.zeros((n, 8), np.float32)
+    r[:, :3], r[:, 3], r[:, 4:7], r[:, 7] = o, 1e-4, d, np.inf
+    return r
+
+
+def lum(rgb):
+    rgb = np.asarray(rgb, np.float64)
+    return rgb[..., 0] * 0.212671 + rgb[..., 1] * 0.715160 + rgb[..., 2] * 0.072169
+
+
+def rel_mse(img, ref, eps=1e-2):
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))
