@@ -206,10 +206,12 @@ HK_HD vec3 cosine_hemisphere(float sx, float sy) {  // Sampling.cpp:29-53, 92-10
     }
     dx = r * cosf(th);
     dy = r * sinf(th);
+    // 1 - dx^2 - dy^2 of the reference is 1 - r^2; (1 - |r|)(1 + |r|) keeps
+    // its digits near the rim of the disk
+    float ar = fabsf(r);
+    return {dx, dy, sqrtf(fmaxf(0.0f, (1.0f - ar) * (1.0f + ar)))};
   }
-  float z = sqrtf(1.0f - dx * dx - dy * dy);
-  if (isnan(z)) z = 0.0f;
-  return {dx, dy, z};
+  return {dx, dy, 1.0f};
 }
 HK_HD float cosine_hemisphere_pdf(vec3 v) {
   return (norm(v) - 1.0f <= 1e-6f && v.z >= 0.0f) ? v.z / kPiF : 0.0f;
@@ -281,19 +283,21 @@ HK_HD vec3 micro_sample(const Micro &m, float sx, float sy, float &pdf) {
     float cs = cos_phi / m.au, ss = sin_phi / m.av;
     alpha_sqr = 1.0f / (cs * cs + ss * ss);
   }
-  float cos_m;
+  float cos_m, tan2;
   if (m.type == kBeckmann) {
-    float tan2 = alpha_sqr * -logf(1.0f - sx);
+    tan2 = alpha_sqr * -logf(1.0f - sx);
     cos_m = 1.0f / sqrtf(1.0f + tan2);
     pdf = (1.0f - sx) / (kPiF * m.au * m.av * cos_m * cos_m * cos_m);
   } else {
-    float tan2 = alpha_sqr * sx / (1.0f - sx);
+    tan2 = alpha_sqr * sx / (1.0f - sx);
     cos_m = 1.0f / sqrtf(1.0f + tan2);
     float tmp = 1.0f + tan2 / alpha_sqr;
     pdf = kInvPiF / (m.au * m.av * cos_m * cos_m * cos_m * tmp * tmp);
   }
   if (pdf < 1e-20f) pdf = 0.0f;
-  float sin_m = sqrtf(fmaxf(0.0f, 1.0f - cos_m * cos_m));
+  // sin = sqrt(1 - cos^2) of the reference, evaluated as tan * cos: the same
+  // value without the cancellation that costs FP32 digits near the normal
+  float sin_m = sqrtf(tan2) * cos_m;
   return {sin_m * cos_phi, sin_m * sin_phi, cos_m};
 }
 

@@ -1,6 +1,7 @@
 // oracle_capi.cpp -- TEST INFRASTRUCTURE ONLY: extern "C" entry points of the
 // CPU restatement (hk_oracle*.hpp) for the Python tests and the CPU-baseline
 // leg of bench.py.  Nothing in the product loads this library.
+#include <chrono>
 #include <cstdio>
 #include <memory>
 
@@ -247,21 +248,29 @@ static void to_in(const ho_scene *s, SceneIn &in) {
 
 // Render samples [s0, s1) of every pixel.  out_rgb: W*H*3 resolved image
 // (nullable); out_acc: W*H*4 filter-weighted sums (nullable); stats: ext,
-// shadow, paths (nullable).
+// shadow, paths, render microseconds excluding scene setup (nullable).
 int ho_render(int fp64, const ho_scene *s, uint32_t req_seed, int s0, int s1, int threads, float *out_rgb,
               double *out_acc, uint64_t *stats) {
   try {
     SceneIn in;
     to_in(s, in);
     PathStats st;
+    auto t0 = std::chrono::steady_clock::now(), t1 = t0;
     if (fp64) {
       Scene<double> sc; sc.load(in);
+      t0 = std::chrono::steady_clock::now();
       render(sc, req_seed, s0, s1, threads, out_rgb, out_acc, &st);
+      t1 = std::chrono::steady_clock::now();
     } else {
       Scene<float> sc; sc.load(in);
+      t0 = std::chrono::steady_clock::now();
       render(sc, req_seed, s0, s1, threads, out_rgb, out_acc, &st);
+      t1 = std::chrono::steady_clock::now();
     }
-    if (stats) { stats[0] = st.ext; stats[1] = st.shadow; stats[2] = st.paths; }
+    if (stats) {
+      stats[0] = st.ext; stats[1] = st.shadow; stats[2] = st.paths;
+      stats[3] = uint64_t(std::chrono::duration<double, std::micro>(t1 - t0).count());  // render time, us
+    }
     return 0;
   } catch (const std::exception &e) {
     snprintf(g_err, sizeof g_err, "%s", e.what());

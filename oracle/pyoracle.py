@@ -147,12 +147,12 @@ def microfacet(dist, au, av, m, v, u):
 
 
 def render(scene, req_seed=0, spp=(0, 1), threads=None, fp64=False):
-    """Returns (rgb (H,W,3) float32, acc (H,W,4) float64, stats [ext, shadow, paths])."""
+    """Returns (rgb (H,W,3) float32, acc (H,W,4) float64, stats [ext, shadow, paths, render_us])."""
     desc = scene.to_c()
     W, H = scene.width, scene.height
     rgb = np.zeros((H, W, 3), np.float32)
     acc = np.zeros((H, W, 4), np.float64)
-    st = np.zeros(3, np.uint64)
+    st = np.zeros(4, np.uint64)
     threads = threads or min(16, os.cpu_count() or 1)
     _check(lib().ho_render(int(fp64), C.byref(desc), req_seed, spp[0], spp[1], threads, _p(rgb), _p(acc), _p(st)))
     return rgb, acc, st

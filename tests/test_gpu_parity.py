@@ -101,21 +101,39 @@ def test_bsdf_eval_pdf_vs_fp64(ctx, index):
 
 @pytest.mark.parametrize("index", list(range(14)))
 def test_bsdf_sample_vs_fp64(ctx, index):
-    """Sample(): direction, weight, measure and pdf agree with the FP64 oracle on the same uniforms."""
+    """Sample() on identical uniforms: within 1e-5 relative of the FP64 oracle, or within the FP64
+    result's own change when a uniform moves by 2^-22 (the absolute resolution of FP32 values in
+    [0, 1) that the warps remap, e.g. (u - p) / (1 - p)), where the warp amplifies it."""
     rng = np.random.default_rng(200 + index)
     mats = material_table()
     wi, wo, measure, u = _bsdf_inputs(rng, 20000)
     g = ctx.bsdf_batch(mats, index, wi, wo, measure, u)
     r = O.bsdf_batch(mats, index, wi, wo, measure, u, fp64=True)
-    same = (g["measure"] == r["measure"]) & (np.abs(g["wo"] - r["wo"]).max(axis=1) < 1e-4)
-    frac_branch = 1.0 - same.mean()
-    e_f = _rel_err(g["f"], r["f"]).max(axis=1)[same]
-    e_wo = np.abs(g["wo"] - r["wo"]).max(axis=1)[same]
-    print(f"mat {index}: branch mismatches {frac_branch:.2e}, f p99.9 {np.quantile(e_f, 0.999):.2e}, "
-          f"wo max {e_wo.max():.2e}")
-    assert frac_branch <= 1e-3
-    assert np.quantile(e_f, 0.999) <= 1e-5
-    assert np.quantile(e_wo, 0.999) <= 1e-5
+    # sensitivity of the FP64 mapping to the last bit of each uniform
+    sens_f = np.zeros(len(u))
+    sens_wo = np.zeros(len(u))
+    for k in range(2):
+        for direction in (np.inf, -np.inf):
+            up = u.copy()
+            step = np.float32(2.0 ** -22) * (1 if direction > 0 else -1)
+            up[:, k] = np.clip(u[:, k] + step, 0.0, np.nextafter(np.float32(1), np.float32(0)))
+            rp = O.bsdf_batch(mats, index, wi, wo, measure, up, fp64=True)
+            sens_f = np.maximum(sens_f, np.abs(rp["f"] - r["f"]).max(axis=1))
+            sens_wo = np.maximum(sens_wo, np.abs(rp["wo"] - r["wo"]).max(axis=1))
+    same = g["measure"] == r["measure"]
+    tol_f = np.maximum(1e-5 * np.maximum(np.abs(r["f"]).max(axis=1), 1e-3), 4 * sens_f)
+    tol_wo = np.maximum(1e-5, 4 * sens_wo)
+    ok_f = np.abs(g["f"] - r["f"]).max(axis=1) <= tol_f
+    ok_wo = np.abs(g["wo"] - r["wo"]).max(axis=1) <= tol_wo
+    ok = same & ok_f & ok_wo
+    print(f"mat {index}: measure mismatches {int((~same).sum())}, f outside tol {int((~ok_f).sum())}, "
+          f"wo outside tol {int((~ok_wo).sum())} of {len(ok)}")
+    if ok.mean() < 0.999:
+        import os
+        os.makedirs("gpurun_out", exist_ok=True)
+        np.savez(f"gpurun_out/bsdf_sample_fail_{index}.npz", wi=wi, wo=wo, measure=measure, u=u, ok=ok,
+                 **{"g_" + k: v for k, v in g.items()}, **{"r_" + k: v for k, v in r.items()})
+    assert ok.mean() >= 0.999
 
 
 def _furnace(albedo):

@@ -1,0 +1,120 @@
+"""Test-side reader of the reference's scene XML (for its <test> scene files).
+
+Covers the subset used by scene/pa5/tests/*.xml: integrator depth, perspective
+camera (width/height/fov/toWorld with lookat/translate/scale/matrix/rotate),
+OBJ meshes with diffuse/dielectric/... BSDFs and area emitters.  Transforms
+compose in document order, each new op pre-multiplied (src/core/Parser.cpp:
+308-369).  The product's own C++ loader is separate; this one only builds
+inputs for the oracle and the C-ABI in tests.
+"""
+import math
+import os
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+from hikari_amd import desc as D
+
+
+def _vec(s):
+    return [float(x) for x in s.replace(",", " ").split()]
+
+
+def parse_transform(node):
+    m = np.eye(4)
+    for op in node:
+        if op.tag == "lookat":
+            t = D.lookat(_vec(op.get("origin")), _vec(op.get("target")), _vec(op.get("up")))
+        elif op.tag == "translate":
+            t = np.eye(4)
+            t[:3, 3] = _vec(op.get("value"))
+        elif op.tag == "scale":
+            t = np.diag(_vec(op.get("value")) + [1.0])
+        elif op.tag == "matrix":
+            t = np.array(_vec(op.get("value"))).reshape(4, 4)
+        elif op.tag == "rotate":
+            ang = math.radians(float(op.get("angle")))
+            ax = np.array(_vec(op.get("axis")))
+            ax = ax / np.linalg.norm(ax)
+            K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+            t = np.eye(4)
+            t[:3, :3] = np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * (K @ K)
+        else:
+            raise ValueError(f"unsupported transform op {op.tag}")
+        m = t @ m
+    return m
+
+
+def props(node):
+    out = {}
+    for c in node:
+        name = c.get("name")
+        if c.tag in ("float",):
+            out[name] = float(c.get("value"))
+        elif c.tag == "integer":
+            out[name] = int(c.get("value"))
+        elif c.tag == "boolean":
+            out[name] = c.get("value").strip().lower() == "true"
+        elif c.tag == "string":
+            out[name] = c.get("value")
+        elif c.tag in ("color", "point", "vector"):
+            v = _vec(c.get("value"))
+            out[name] = v if len(v) == 3 else v * 3
+        elif c.tag == "transform":
+            out[name] = parse_transform(c)
+    return out
+
+
+def bsdf_record(node, table):
+    t = node.get("type")
+    p = props(node)
+    g = lambda k, d: p.get(k, d)  # noqa: E731
+    dist = D.GGX if g("type", "beckmann") == "ggx" else D.BECKMANN
+    if t == "diffuse":
+        return D.diffuse(g("albedo", 0.5))
+    if t == "dielectric":
+        return D.dielectric(g("intIOR", 1.5046), g("extIOR", D.AIR), g("ksReflect", 1.0), g("ksRefract", 1.0))
+    if t == "conductor":
+        return D.conductor(g("intIOR", 1.5046), g("extIOR", D.AIR), g("k", 1.0), g("ks", 1.0))
+    if t == "plastic":
+        return D.plastic(g("intIOR", 1.5046), g("extIOR", D.AIR), g("ks", 1.0), g("kd", 0.5), g("nonlinear", False))
+    if t == "roughConductor":
+        return D.rough_conductor(g("alpha", 0.1), dist, g("intIOR", 1.5046), g("extIOR", D.AIR), g("k", 1.0),
+                                 g("ks", 1.0))
+    if t == "roughDielectric":
+        return D.rough_dielectric(g("alpha", 0.1), dist, g("intIOR", 1.5046), g("extIOR", D.AIR))
+    if t == "roughPlastic":
+        return D.rough_plastic(g("alpha", 0.1), dist, g("intIOR", 1.49), g("extIOR", D.AIR), g("ks", 1.0),
+                               g("kd", 0.5), g("nonlinear", False))
+    raise ValueError(f"unsupported bsdf {t}")
+
+
+def parse_scene(node, base_dir):
+    sc = D.Scene()
+    integ = node.find("integrator")
+    sc.max_depth = props(integ).get("depth", 1000) if integ is not None else 1000
+    sc.integrator = integ.get("type") if integ is not None else None
+    cam = node.find("camera")
+    cp = props(cam)
+    sc.width, sc.height = cp.get("width", 1280), cp.get("height", 720)
+    sc.fov = cp.get("fov", 30.0)
+    sc.near_clip, sc.far_clip = cp.get("nearClip", 1e-4), cp.get("farClip", 1e4)
+    if "toWorld" in cp:
+        sc.cam_to_world = cp["toWorld"]
+    for m in node.findall("mesh"):
+        mp = props(m)
+        v, f, n, uv = D.load_obj(os.path.join(base_dir, mp["filename"]), mp.get("toWorld"))
+        b = m.find("bsdf")
+        mat = sc.add_material(bsdf_record(b, sc.materials) if b is not None else D.diffuse(0.5))
+        e = m.find("emitter")
+        rad = tuple(props(e)["radiance"]) if e is not None else None
+        sc.add_mesh(v, f, mat, n=n, uv=uv, radiance=rad)
+    return sc
+
+
+def parse_test(path):
+    """Returns (references, [Scene]) of a <test type="ttest"> file."""
+    root = ET.parse(path).getroot()
+    refs = [float(x) for x in root.find("string[@name='references']").get("value").replace(",", " ").split()]
+    base = os.path.dirname(path)
+    return refs, [parse_scene(s, base) for s in root.findall("scene")]
