@@ -85,7 +85,7 @@ def main():
         _, st = gs.render(a.spp, seed=1000 + i)
         print(f"# warmup {i}: {st['gpu_ms']:.1f} ms", file=sys.stderr, flush=True)
     lat, tot = [], {"paths": 0, "extension_rays": 0, "shadow_rays": 0, "trace_ms": 0.0, "trace_launches": 0,
-                    "gpu_ms": 0.0}
+                    "gpu_ms": 0.0, "node_visits": 0, "triangle_tests": 0}
     t_all = time.perf_counter()
     for i in range(a.steps):
         t0 = time.perf_counter()
@@ -123,6 +123,9 @@ def main():
                      "launch_ms": ext_avg_ms, "rays_per_launch": tot["extension_rays"] / max(1, tot["trace_launches"])},
         "render_roofline_frac": render_bytes / wall / (HBM_PEAK_GBS * 1e9),
     }
+    if tot["node_visits"]:  # builds with traversal counters only
+        out["traversal"] = {"node_visits_per_ray": tot["node_visits"] / max(1, rays),
+                            "triangle_tests_per_ray": tot["triangle_tests"] / max(1, rays)}
     if not a.no_cpu_baseline:
         t = time.time()
         out["cpu_baseline"] = cpu_baseline(scene, a.cpu_seconds)

@@ -13,7 +13,8 @@ from . import desc  # noqa: F401
 from .desc import SceneDesc
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhikari_amd.so")
+# HIKARI_AMD_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("HIKARI_AMD_LIB", os.path.join(HERE, "libhikari_amd.so"))
 
 HK_OK = 0
 ERRORS = {-1: "HK_ERR_INVALID", -2: "HK_ERR_HIP", -3: "HK_ERR_OOM", -4: "HK_ERR_UNSUPPORTED", -5: "HK_ERR_IO",
@@ -51,7 +52,7 @@ class Request(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("bounces_launched", C.c_uint64), ("gpu_ms", C.c_double), ("trace_ms", C.c_double),
-                ("trace_launches", C.c_uint64)]
+                ("trace_launches", C.c_uint64), ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -6,9 +6,11 @@
 //  * k_intersect: batched rayIntersect over hk_ray / hk_hit arrays.
 //  * k_bsdf_batch: batched BSDF::Sample / Eval / Pdf for parity tests.
 //  * wavefront pathMIS (PathMISIntegrator::Li, src/integrator/
-//    PathMISIntegrator.cpp:16-157): k_camera -> [k_extend -> k_shade ->
-//    k_shadow]* -> k_resolve.  Path state is path-indexed SoA; ray, hit and
-//    shadow queues are compacted with wave ballots (one atomic per wave).
+//    PathMISIntegrator.cpp:16-157): [k_regen -> k_extend -> k_shade ->
+//    k_shadow]* -> k_resolve.  Path state is slot-indexed SoA; ray, hit and
+//    shadow queues are compacted with wave ballots (one atomic per wave);
+//    k_regen refills the slots of finished paths with new samples, so every
+//    bounce launch stays full until the segment's samples run out.
 //    Film accumulation is owned per output pixel (gather in k_resolve), so
 //    results are deterministic and independent of scheduling.
 #include <hip/hip_runtime.h>
@@ -19,10 +21,13 @@
 namespace hk {
 
 constexpr int kBlock = 256;          // threads per block for every kernel here
-constexpr int kLdsStack = 16;        // stack entries per lane kept in LDS
-constexpr int kPrivStack = 80;       // further entries in private memory
+#ifndef HK_LDS_STACK
+#define HK_LDS_STACK 8
+#endif
+constexpr int kLdsStack = HK_LDS_STACK;  // stack entries per lane kept in LDS
+constexpr int kPrivStack = 96 - kLdsStack;  // further entries in private memory
 constexpr float kInfF = __builtin_huge_valf();
-
+constexpr int32_t kDone = kEmptyRef;
 
 // ------------------------------------------------------------------ traversal
 struct TraceStack {
@@ -36,16 +41,30 @@ struct TraceStack {
     else priv[sp - kLdsStack] = e;
     sp++;
   }
-  __device__ __forceinline__ int2 pop() {
-    sp--;
-    return sp < kLdsStack ? lds[sp * kBlock + tid] : priv[sp - kLdsStack];
+  // next entry whose box can still hold a hit at t <= tlim, or kDone
+  __device__ __forceinline__ int32_t pop(float tlim) {
+    while (sp > 0) {
+      sp--;
+      int2 e = sp < kLdsStack ? lds[sp * kBlock + tid] : priv[sp - kLdsStack];
+      if (__int_as_float(e.y) <= tlim) return e.x;
+    }
+    return kDone;
   }
 };
 
+__device__ __forceinline__ float ubyte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
+
+// Closest (ANY = false) or any (ANY = true) hit over the quantized BVH4.
+// Inner loop descends internal nodes until the lane holds a leaf (or is
+// done), so leaf tests run with the wave converged ("while-while").
+struct TraceCount {
+  uint32_t nodes = 0, tris = 0;
+};
+
 template <bool ANY>
-__device__ __forceinline__ bool trace_ray(const Bvh4Node *__restrict__ nodes, const TraceTri *__restrict__ tris,
+__device__ __forceinline__ bool trace_ray(const TraceNode *__restrict__ nodes, const TraceTri *__restrict__ tris,
                                           int32_t root, int2 *lds_stack, vec3 o, vec3 d, float tmin, float tmax,
-                                          float &bt, float &bu, float &bv, int32_t &bprim) {
+                                          float &bt, float &bu, float &bv, int32_t &bprim, TraceCount &cnt) {
   float ix = fabsf(d.x) > 1e-30f ? 1.0f / d.x : copysignf(1e30f, d.x);
   float iy = fabsf(d.y) > 1e-30f ? 1.0f / d.y : copysignf(1e30f, d.y);
   float iz = fabsf(d.z) > 1e-30f ? 1.0f / d.z : copysignf(1e30f, d.z);
@@ -56,34 +75,56 @@ __device__ __forceinline__ bool trace_ray(const Bvh4Node *__restrict__ nodes, co
   bt = tmax;
   bprim = -1;
   bu = bv = 0.0f;
-  // Slab tests are widened by 4 ulp-relative (tnear <= tfar * (1 + 2^-21)),
-  // which covers the rounding of (lo - o) * (1/d); triangle boxes are padded
-  // on the host for the inexactness of the triangle test itself.
+  // Slab tests are widened by a few ulp (tnear <= tfar * (1 + 2^-21)), which
+  // covers the rounding of (lo - o) * (1/d); the host pads triangle boxes for
+  // the inexactness of the triangle test and rounds quantized boxes outward.
   const float kWiden = 1.0000005f;
   int32_t ref = root;
   for (;;) {
-    if (ref >= 0) {
-      const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
-      float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-      int4 ch = reinterpret_cast<const int4 *>(np)[6];
-      float tn[4];
-      const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-      const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-      const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-      const int32_t cha[4] = {ch.x, ch.y, ch.z, ch.w};
+    while (ref >= 0) {
+#ifdef HK_TRAVERSAL_STATS
+      cnt.nodes++;
+#endif
       float tlim = bt * kWiden;
+      float tn[4];
+      int32_t r[4];
+#ifdef HK_NODE_FLOAT
+      const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
+      float4 lx4 = np[0], hx4 = np[1], ly4 = np[2], hy4 = np[3], lz4 = np[4], hz4 = np[5];
+      int4 ch = *reinterpret_cast<const int4 *>(np + 6);
+      float lxs[4] = {lx4.x, lx4.y, lx4.z, lx4.w}, hxs[4] = {hx4.x, hx4.y, hx4.z, hx4.w};
+      float lys[4] = {ly4.x, ly4.y, ly4.z, ly4.w}, hys[4] = {hy4.x, hy4.y, hy4.z, hy4.w};
+      float lzs[4] = {lz4.x, lz4.y, lz4.z, lz4.w}, hzs[4] = {hz4.x, hz4.y, hz4.z, hz4.w};
+#else
+      const Bvh4QNode *np = nodes + ref;
+      float4 h = *reinterpret_cast<const float4 *>(np);
+      int4 ch = *reinterpret_cast<const int4 *>(&np->child[0]);
+      uint4 qa = *reinterpret_cast<const uint4 *>(&np->q[0]);
+      uint2 qb = *reinterpret_cast<const uint2 *>(&np->q[16]);
+      uint32_t ex = __float_as_uint(h.w);
+      float sx = __uint_as_float((ex & 0xffu) << 23), sy = __uint_as_float(((ex >> 8) & 0xffu) << 23);
+      float sz = __uint_as_float(((ex >> 16) & 0xffu) << 23);
+      float lxs[4], hxs[4], lys[4], hys[4], lzs[4], hzs[4];
 #pragma unroll
       for (int c = 0; c < 4; c++) {
-        float t0x = (lxa[c] - o.x) * ix, t1x = (hxa[c] - o.x) * ix;
-        float t0y = (lya[c] - o.y) * iy, t1y = (hya[c] - o.y) * iy;
-        float t0z = (lza[c] - o.z) * iz, t1z = (hza[c] - o.z) * iz;
+        lxs[c] = fmaf(ubyte_f(qa.x, c), sx, h.x);
+        hxs[c] = fmaf(ubyte_f(qa.y, c), sx, h.x);
+        lys[c] = fmaf(ubyte_f(qa.z, c), sy, h.y);
+        hys[c] = fmaf(ubyte_f(qa.w, c), sy, h.y);
+        lzs[c] = fmaf(ubyte_f(qb.x, c), sz, h.z);
+        hzs[c] = fmaf(ubyte_f(qb.y, c), sz, h.z);
+      }
+#endif
+      r[0] = ch.x; r[1] = ch.y; r[2] = ch.z; r[3] = ch.w;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        float t0x = (lxs[c] - o.x) * ix, t1x = (hxs[c] - o.x) * ix;
+        float t0y = (lys[c] - o.y) * iy, t1y = (hys[c] - o.y) * iy;
+        float t0z = (lzs[c] - o.z) * iz, t1z = (hzs[c] - o.z) * iz;
         float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
         float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tlim));
-        bool hit = cha[c] != kEmptyRef && a <= b * kWiden;
-        tn[c] = hit ? a : kInfF;
+        tn[c] = (r[c] != kEmptyRef && a <= b * kWiden) ? a : kInfF;
       }
-      // order children near -> far (sorting network on (t, ref))
-      int32_t r[4] = {cha[0], cha[1], cha[2], cha[3]};
 #define HK_CSWAP(i, j)                                    \
   if (tn[j] < tn[i]) {                                    \
     float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;          \
@@ -91,51 +132,40 @@ __device__ __forceinline__ bool trace_ray(const Bvh4Node *__restrict__ nodes, co
   }
       HK_CSWAP(0, 1) HK_CSWAP(2, 3) HK_CSWAP(0, 2) HK_CSWAP(1, 3) HK_CSWAP(1, 2)
 #undef HK_CSWAP
-      // push the far ones, continue with the nearest
       if (tn[3] != kInfF) st.push(r[3], tn[3]);
       if (tn[2] != kInfF) st.push(r[2], tn[2]);
       if (tn[1] != kInfF) st.push(r[1], tn[1]);
-      if (tn[0] != kInfF) {
-        ref = r[0];
-        continue;
-      }
-    } else {
-      uint32_t code = uint32_t(~ref);
-      uint32_t first = code >> 3, cnt = (code & 7u) + 1u;
-      for (uint32_t k = 0; k < cnt; k++) {
-        const float4 *tp = reinterpret_cast<const float4 *>(tris + first + k);
-        float4 a = tp[0], b = tp[1], c = tp[2];
-        float u, v, t;
-        if (mt_test(o, d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(b.w, c.x, c.y), tmin, bt, u, v, t)) {
-          int32_t prim = __float_as_int(a.w);
-          if (ANY) {
-            bt = t; bu = u; bv = v; bprim = prim;
-            return true;
-          }
-          // closest hit; among equal t the largest prim (brute-force order)
-          if (t < bt || prim > bprim) {
-            bt = t; bu = u; bv = v; bprim = prim;
-          }
+      ref = tn[0] != kInfF ? r[0] : st.pop(bt * kWiden);
+    }
+    if (ref == kDone) break;
+    uint32_t code = uint32_t(~ref);
+    uint32_t first = code >> 3, ntri = (code & 7u) + 1u;
+#ifdef HK_TRAVERSAL_STATS
+    cnt.tris += ntri;
+#endif
+    for (uint32_t k = 0; k < ntri; k++) {
+      const float4 *tp = reinterpret_cast<const float4 *>(tris + first + k);
+      float4 a = tp[0], b = tp[1], c = tp[2];
+      float u, v, t;
+      if (mt_test(o, d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(b.w, c.x, c.y), tmin, bt, u, v, t)) {
+        int32_t prim = __float_as_int(a.w);
+        if (ANY) {
+          bt = t; bu = u; bv = v; bprim = prim;
+          return true;
+        }
+        // closest hit; among equal t the largest prim (brute-force order)
+        if (t < bt || prim > bprim) {
+          bt = t; bu = u; bv = v; bprim = prim;
         }
       }
     }
-    // pop the next entry that can still hold a hit at t <= bt
-    bool found = false;
-    while (st.sp > 0) {
-      int2 e = st.pop();
-      if (__int_as_float(e.y) <= bt * kWiden) {
-        ref = e.x;
-        found = true;
-        break;
-      }
-    }
-    if (!found) break;
+    ref = st.pop(bt * kWiden);
   }
   return bprim >= 0;
 }
 
 // ------------------------------------------------------------------ batched rayIntersect
-__global__ __launch_bounds__(kBlock) void k_intersect(const Bvh4Node *__restrict__ nodes,
+__global__ __launch_bounds__(kBlock) void k_intersect(const TraceNode *__restrict__ nodes,
                                                       const TraceTri *__restrict__ tris, int32_t root,
                                                       const float4 *__restrict__ rays, float4 *__restrict__ hits,
                                                       uint32_t n, int any) {
@@ -145,10 +175,11 @@ __global__ __launch_bounds__(kBlock) void k_intersect(const Bvh4Node *__restrict
     float4 a = rays[2 * i], b = rays[2 * i + 1];
     float t, u, v;
     int32_t prim;
+    TraceCount tc;
     bool h = any ? trace_ray<true>(nodes, tris, root, stack, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, t, u,
-                                   v, prim)
+                                   v, prim, tc)
                  : trace_ray<false>(nodes, tris, root, stack, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, t,
-                                    u, v, prim);
+                                    u, v, prim, tc);
     hits[i] = h ? make_float4(t, u, v, __int_as_float(prim)) : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
   }
 }
@@ -202,11 +233,25 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
   return pred ? base + (uint32_t)__popcll(below) : 0xffffffffu;
 }
 
+// Adds a lane's traversal counters to stats[0] (node visits), stats[1]
+// (triangle tests): one atomic per wave.
+__device__ __forceinline__ void count_flush(unsigned long long *stats, TraceCount tc) {
+  unsigned long long n = tc.nodes, t = tc.tris;
+  for (int off = 32; off > 0; off >>= 1) {
+    n += __shfl_down(n, off);
+    t += __shfl_down(t, off);
+  }
+  if (lane_id() == 0) {
+    atomicAdd(&stats[0], n);
+    atomicAdd(&stats[1], t);
+  }
+}
+
 __device__ __forceinline__ uint32_t find_item(const WorkItem *items, uint32_t n, uint32_t id) {
-  uint32_t lo = 0, hi = n;  // largest i with items[i].path_begin <= id
+  uint32_t lo = 0, hi = n;  // largest i with items[i].sample_begin <= id
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
-    if (items[mid].path_begin <= id) lo = mid;
+    if (items[mid].sample_begin <= id) lo = mid;
     else hi = mid;
   }
   return lo;
@@ -347,17 +392,28 @@ __device__ __forceinline__ void post_intersect(const DevScene &sc, int32_t prim,
 }
 
 constexpr uint32_t kFlagPrevDiscrete = 1u;
+constexpr uint32_t kShadowLast = 0x80000000u;  // shadow-queue mask bit: the path ended at this bounce
 
-// ------------------------------------------------------------------ k_camera
-// RenderBlock's per-sample prologue (src/Main.cpp:22-31) + PerspectiveCamera::
-// SampleRay (src/camera/PerspectiveCamera.cpp:77-102).
-__global__ __launch_bounds__(kBlock) void k_camera(RenderArgs A) {
+// ------------------------------------------------------------------ k_regen
+// Starts samples g0 .. g0 + n_new - 1 of the segment in the top n_new free
+// path slots and appends their camera rays to extension queue qi after the
+// q_base paths already there.  Per sample: RenderBlock's prologue
+// (src/Main.cpp:22-31) + PerspectiveCamera::SampleRay (src/camera/
+// PerspectiveCamera.cpp:77-102).  A sample's path depends only on (request,
+// pixel, sample index), never on the slot it runs in.
+__global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t q_base, uint32_t free_top,
+                                                  uint32_t n_new, uint32_t g0) {
   uint32_t stride = gridDim.x * blockDim.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[0] = A.n_paths;
-  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < A.n_paths; id += stride) {
-    const WorkItem it = A.items[find_item(A.items, A.n_items, id)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    A.counters[qi] = q_base + n_new;
+    A.counters[3] = free_top - n_new;
+  }
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_new; j += stride) {
+    uint32_t g = g0 + j;
+    uint32_t slot = A.free_slots[free_top - 1u - j];
+    const WorkItem it = A.items[find_item(A.items, A.n_items, g)];
     const DevRequest &rq = A.reqs[it.req];
-    uint32_t local = id - it.path_begin;
+    uint32_t local = g - it.sample_begin;
     uint32_t q = rq.pix0 + local % it.npix, s = it.s0 + local / it.npix;
     uint32_t x = q % (uint32_t)rq.width, y = q / (uint32_t)rq.width;
     Rng rng = rng_for_path(rq.seed, q, s);
@@ -375,14 +431,16 @@ __global__ __launch_bounds__(kBlock) void k_camera(RenderArgs A) {
     vec3 o = v3(c[3] / c[15], c[7] / c[15], c[11] / c[15]);
     vec3 d = v3(c[0] * dir.x + c[1] * dir.y + c[2] * dir.z, c[4] * dir.x + c[5] * dir.y + c[6] * dir.z,
                 c[8] * dir.x + c[9] * dir.y + c[10] * dir.z);
-    A.q_path[0][id] = id;
-    A.q_o[0][id] = make_float4(o.x, o.y, o.z, rq.near_clip * inv_z);
-    A.q_d[0][id] = make_float4(d.x, d.y, d.z, rq.far_clip * inv_z);
-    A.ps_beta[id] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    A.ps_L[id] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    A.ps_rng[id] = rng.state;
-    A.ps_pix[id] = q;
-    A.ps_meta[id] = 0u;
+    uint32_t pos = q_base + j;
+    A.q_path[qi][pos] = slot;
+    A.q_o[qi][pos] = make_float4(o.x, o.y, o.z, rq.near_clip * inv_z);
+    A.q_d[qi][pos] = make_float4(d.x, d.y, d.z, rq.far_clip * inv_z);
+    A.ps_beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    A.ps_L[slot] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    A.ps_rng[slot] = rng.state;
+    A.ps_pix[slot] = q;
+    A.ps_sid[slot] = g;
+    A.ps_meta[slot] = 0u;
   }
 }
 
@@ -398,14 +456,18 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   uint32_t stride = gridDim.x * blockDim.x;
   const float4 *qo = A.q_o[qi], *qd = A.q_d[qi];
   float4 *qh = A.q_hit[qi];
+  TraceCount tc;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
     float4 a = qo[i], b = qd[i];
     float t, u, v;
     int32_t prim;
     bool h = trace_ray<false>(A.sc.nodes, A.sc.tris, A.sc.root_ref, stack, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z),
-                              a.w, b.w, t, u, v, prim);
+                              a.w, b.w, t, u, v, prim, tc);
     qh[i] = h ? make_float4(t, u, v, __int_as_float(prim)) : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
   }
+#ifdef HK_TRAVERSAL_STATS
+  count_flush(A.stats + 2, tc);
+#endif
 }
 
 // ------------------------------------------------------------------ k_shade
@@ -420,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
     bool active = i < count;
     bool cont = false;
     uint32_t smask = 0, path = 0;
-    vec3 P = v3(0.0f), nd = v3(0.0f);
+    vec3 P = v3(0.0f), nd = v3(0.0f), Lout = v3(0.0f);
     if (active) {
       path = A.q_path[qi][i];
       float4 o4 = A.q_o[qi][i], d4 = A.q_d[qi][i], h4 = A.q_hit[qi][i];
@@ -514,6 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
       }
       A.ps_beta[path] = make_float4(beta.x, beta.y, beta.z, pdf_b);
       A.ps_L[path] = make_float4(L.x, L.y, L.z, w_mats);
+      Lout = L;
       A.ps_rng[path] = rng.state;
       A.ps_meta[path] = (depth & 0xffffu) | (flags << 16);
     }
@@ -522,6 +585,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
       A.q_path[qn][slot] = path;
       A.q_o[qn][slot] = make_float4(P.x, P.y, P.z, 1e-4f);
       A.q_d[qn][slot] = make_float4(nd.x, nd.y, nd.z, kInfF);
+    }
+    // A finished path frees its slot now (k_regen runs only after k_shadow);
+    // its radiance goes to the sample buffer here, or in k_shadow when
+    // emitter samples of this bounce are still pending.
+    bool done = active && !cont;
+    uint32_t fs = wave_append(&A.counters[3], done);
+    if (done) {
+      A.free_slots[fs] = path;
+      if (smask == 0u) {
+        A.sample_L[A.ps_sid[path]] = make_float4(Lout.x, Lout.y, Lout.z, 0.0f);
+      } else {
+        smask |= kShadowLast;
+      }
     }
     uint32_t ss = wave_append(&A.counters[2], smask != 0u);
     if (smask != 0u) {
@@ -539,8 +615,11 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A) {
   uint32_t count = A.counters[2];
   uint32_t stride = gridDim.x * blockDim.x;
   uint32_t traced = 0;
+  TraceCount tc;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
     uint32_t path = A.s_path[i], mask = A.s_mask[i];
+    bool last = (mask & kShadowLast) != 0u;
+    mask &= ~kShadowLast;
     float4 l4 = A.ps_L[path];
     vec3 L = v3(l4.x, l4.y, l4.z);
     while (mask) {
@@ -552,14 +631,18 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A) {
       int32_t prim;
       traced++;
       bool occ = trace_ray<true>(A.sc.nodes, A.sc.tris, A.sc.root_ref, stack, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z),
-                                 1e-4f, 1.0f - 1e-4f, t, u, v, prim);
+                                 1e-4f, 1.0f - 1e-4f, t, u, v, prim, tc);
       if (!occ) L = L + v3(c.x, c.y, c.z);
     }
-    A.ps_L[path] = make_float4(L.x, L.y, L.z, l4.w);
+    if (last) A.sample_L[A.ps_sid[path]] = make_float4(L.x, L.y, L.z, 0.0f);
+    else A.ps_L[path] = make_float4(L.x, L.y, L.z, l4.w);
   }
   // one atomic per wave for the statistics
   for (int off = 32; off > 0; off >>= 1) traced += __shfl_down(traced, off);
   if (lane_id() == 0 && traced) atomicAdd(&A.stats[1], (unsigned long long)traced);
+#ifdef HK_TRAVERSAL_STATS
+  count_flush(A.stats + 4, tc);
+#endif
 }
 
 // ------------------------------------------------------------------ k_resolve
@@ -586,8 +669,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(RenderArgs A, uint32_t item_
           float cx = ((float)qx + jx) - 0.5f, cy = ((float)qy + jy) - 0.5f;
           if (x < (int)ceilf(cx - R) || x > (int)floorf(cx + R) || y < (int)ceilf(cy - R) || y > (int)floorf(cy + R))
             continue;
-          uint32_t path = it.path_begin + s * it.npix + q;
-          float4 l4 = A.ps_L[path];
+          float4 l4 = A.sample_L[it.sample_begin + s * it.npix + q];
           vec3 L = v3(l4.x, l4.y, l4.z);
           if (!color_valid(L)) continue;  // invalid samples are dropped (Block.cpp:97-101)
           float wx = rq.filter[(int)(fabsf((float)x - cx) * rq.lookup)];
@@ -620,6 +702,11 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
   if (i < n) p[i] = v;
 }
 
+__global__ void k_iota_u32(uint32_t *p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
 // ------------------------------------------------------------------ host launchers
 static uint32_t grid_for(uint32_t n, uint32_t cap) {
   uint32_t g = (n + kBlock - 1) / kBlock;
@@ -627,7 +714,7 @@ static uint32_t grid_for(uint32_t n, uint32_t cap) {
   return g < 1 ? 1 : g;
 }
 
-hipError_t launch_intersect(const Bvh4Node *nodes, const TraceTri *tris, int32_t root, const void *rays, void *hits,
+hipError_t launch_intersect(const TraceNode *nodes, const TraceTri *tris, int32_t root, const void *rays, void *hits,
                             uint32_t n, int any, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_intersect, dim3(grid_for(n, 1u << 20)), dim3(kBlock), 0, st, nodes, tris, root,
@@ -645,8 +732,10 @@ hipError_t launch_bsdf_batch(const DevMaterial *mats, const float *rft, int idx,
   return hipGetLastError();
 }
 
-hipError_t launch_camera(const RenderArgs &A, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_camera, dim3(grid_for(A.n_paths, grid)), dim3(kBlock), 0, st, A);
+hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t free_top, uint32_t n_new,
+                        uint32_t g0, hipStream_t st) {
+  hipLaunchKernelGGL(k_regen, dim3(grid_for(n_new, 1u << 16)), dim3(kBlock), 0, st, A, qi, q_base, free_top, n_new,
+                     g0);
   return hipGetLastError();
 }
 hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st) {
@@ -672,6 +761,11 @@ hipError_t launch_finalize(const void *accum, float *rgb, uint32_t n, hipStream_
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_iota_u32(uint32_t *p, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
   return hipGetLastError();
 }
 

@@ -403,6 +403,9 @@ int hk_set_option(hk_ctx *ctx, const char *name, int64_t value) {
   if (n == "max_paths") {
     if (value < 0 || value > (int64_t(1) << 28)) throw HkError(HK_ERR_INVALID, "max_paths out of range");
     ctx->max_paths = value == 0 ? kDefaultMaxPaths : uint32_t(value);
+  } else if (n == "max_samples") {
+    if (value < 0 || value > (int64_t(1) << 31)) throw HkError(HK_ERR_INVALID, "max_samples out of range");
+    ctx->max_samples = value == 0 ? kDefaultMaxSamples : uint32_t(value);
   } else if (n == "profile") {
     ctx->profile = value != 0;
   } else if (n == "grid") {
@@ -446,7 +449,7 @@ int hk_accel_get_info(const hk_accel *a, hk_accel_info *info) {
   info->n_nodes = a->bvh.n_nodes;
   info->n_leaves = a->bvh.n_leaves;
   info->max_depth = a->bvh.max_depth;
-  info->node_bytes = uint64_t(a->bvh.n_nodes) * sizeof(Bvh4Node);
+  info->node_bytes = uint64_t(a->bvh.n_nodes) * sizeof(TraceNode);
   info->triangle_bytes = uint64_t(a->bvh.n_tris) * sizeof(TraceTri);
   info->build_ms = a->build_ms;
   return HK_OK;
@@ -456,7 +459,7 @@ int hk_accel_intersect_device(hk_accel *a, const hk_ray *d_rays, uint32_t n, hk_
   hk_ctx *ctx = a ? a->ctx : nullptr;
   HK_API_BEGIN
   if (!a || (n && (!d_rays || !d_hits))) throw HkError(HK_ERR_INVALID, "null argument");
-  hip_check(launch_intersect((const Bvh4Node *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, d_rays, d_hits,
+  hip_check(launch_intersect((const TraceNode *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, d_rays, d_hits,
                              n, any_hit, ctx->stream),
             "k_intersect launch");
   return HK_OK;
@@ -471,7 +474,7 @@ int hk_accel_intersect(hk_accel *a, const hk_ray *rays, uint32_t n, hk_hit *hits
   DBuf dr, dh;
   dr.upload(rays, sizeof(hk_ray) * size_t(n), ctx->stream);
   dh.alloc(sizeof(hk_hit) * size_t(n));
-  hip_check(launch_intersect((const Bvh4Node *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, dr.p, dh.p, n,
+  hip_check(launch_intersect((const TraceNode *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, dr.p, dh.p, n,
                              any_hit, ctx->stream),
             "k_intersect launch");
   hip_check(hipMemcpyAsync(hits, dh.p, sizeof(hk_hit) * size_t(n), hipMemcpyDeviceToHost, ctx->stream), "D2H");

@@ -39,18 +39,20 @@ struct DevBvh {
   uint32_t n_nodes = 0, n_tris = 0, n_leaves = 0, max_depth = 0;
 };
 
-constexpr uint32_t kDefaultMaxPaths = 1u << 22;
+constexpr uint32_t kDefaultMaxPaths = 1u << 22;     // path slots of the wavefront
+constexpr uint32_t kDefaultMaxSamples = 1u << 28;   // samples per segment (16 B each in HBM)
 constexpr int kMaxEmitters = 32;
 
 // Wavefront buffers, grown on demand and reused across renders of a context.
 struct Workspace {
-  uint32_t cap_paths = 0, cap_emitters = 0;
-  DBuf ps_beta, ps_L, ps_rng, ps_pix, ps_meta;
+  uint32_t cap_paths = 0, cap_emitters = 0, cap_samples = 0;
+  DBuf sample_L, free_slots;
+  DBuf ps_beta, ps_L, ps_rng, ps_pix, ps_sid, ps_meta;
   DBuf q_path[2], q_o[2], q_d[2], q_hit[2];
   DBuf s_path, s_mask, sh_o, sh_d, sh_c;
   DBuf counters, stats;
   DBuf reqs, items;
-  void ensure(uint32_t paths, uint32_t emitters);
+  void ensure(uint32_t paths, uint32_t emitters, uint32_t samples);
   void release();
 };
 
@@ -63,6 +65,7 @@ struct hk_ctx {
   int n_cu = 0;
   std::string arch;
   uint32_t max_paths = hk::kDefaultMaxPaths;
+  uint32_t max_samples = hk::kDefaultMaxSamples;
   uint32_t grid = 0;  // persistent grid for queue kernels (0: automatic)
   bool profile = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;

@@ -33,7 +33,11 @@ void upload_bvh(hk_ctx *ctx, const float *tri9, uint32_t nt, DevBvh &out) {
   // the traversal stack holds at most 3 entries per level below the root
   if (3 * r.max_depth + 1 > 96)
     throw HkError(HK_ERR_UNSUPPORTED, "BVH depth %u exceeds the traversal stack", r.max_depth);
+#ifdef HK_NODE_FLOAT
   out.nodes.upload(r.nodes.data(), r.nodes.size() * sizeof(Bvh4Node), ctx->stream);
+#else
+  out.nodes.upload(r.qnodes.data(), r.qnodes.size() * sizeof(Bvh4QNode), ctx->stream);
+#endif
   out.tris.upload(r.tris.data(), r.tris.size() * sizeof(TraceTri), ctx->stream);
   out.n_nodes = uint32_t(r.nodes.size());
   out.n_tris = nt;
@@ -42,13 +46,19 @@ void upload_bvh(hk_ctx *ctx, const float *tri9, uint32_t nt, DevBvh &out) {
   hip_check(hipStreamSynchronize(ctx->stream), "BVH upload");
 }
 
-void Workspace::ensure(uint32_t paths, uint32_t emitters) {
+void Workspace::ensure(uint32_t paths, uint32_t emitters, uint32_t samples) {
   emitters = std::max(1u, emitters);
+  if (samples > cap_samples) {
+    sample_L.alloc(size_t(samples) * 16);
+    cap_samples = samples;
+  }
   if (paths <= cap_paths && emitters <= cap_emitters) return;
   paths = std::max(paths, cap_paths);
   emitters = std::max(emitters, cap_emitters);
   size_t P = paths;
-  ps_beta.alloc(P * 16); ps_L.alloc(P * 16); ps_rng.alloc(P * 8); ps_pix.alloc(P * 4); ps_meta.alloc(P * 4);
+  free_slots.alloc(P * 4);
+  ps_beta.alloc(P * 16); ps_L.alloc(P * 16); ps_rng.alloc(P * 8); ps_pix.alloc(P * 4); ps_sid.alloc(P * 4);
+  ps_meta.alloc(P * 4);
   for (int i = 0; i < 2; i++) {
     q_path[i].alloc(P * 4); q_o[i].alloc(P * 16); q_d[i].alloc(P * 16); q_hit[i].alloc(P * 16);
   }
@@ -60,11 +70,11 @@ void Workspace::ensure(uint32_t paths, uint32_t emitters) {
   cap_emitters = emitters;
 }
 void Workspace::release() {
-  DBuf *all[] = {&ps_beta, &ps_L, &ps_rng, &ps_pix, &ps_meta, &q_path[0], &q_path[1], &q_o[0], &q_o[1], &q_d[0],
-                 &q_d[1], &q_hit[0], &q_hit[1], &s_path, &s_mask, &sh_o, &sh_d, &sh_c, &counters, &stats, &reqs,
-                 &items};
+  DBuf *all[] = {&sample_L, &free_slots, &ps_beta, &ps_L, &ps_rng, &ps_pix, &ps_sid, &ps_meta, &q_path[0],
+                 &q_path[1], &q_o[0], &q_o[1], &q_d[0], &q_d[1], &q_hit[0], &q_hit[1], &s_path, &s_mask, &sh_o,
+                 &sh_d, &sh_c, &counters, &stats, &reqs, &items};
   for (DBuf *b : all) b->release();
-  cap_paths = cap_emitters = 0;
+  cap_paths = cap_emitters = cap_samples = 0;
 }
 
 static float bbox_radius(const float *lo, const float *hi) {
@@ -83,6 +93,9 @@ static RenderArgs make_args(hk_scene *s) {
   A.ps_L = ws.ps_L.as<float4>();
   A.ps_rng = ws.ps_rng.as<uint64_t>();
   A.ps_pix = ws.ps_pix.as<uint32_t>();
+  A.ps_sid = ws.ps_sid.as<uint32_t>();
+  A.sample_L = ws.sample_L.as<float4>();
+  A.free_slots = ws.free_slots.as<uint32_t>();
   A.ps_meta = ws.ps_meta.as<uint32_t>();
   for (int i = 0; i < 2; i++) {
     A.q_path[i] = ws.q_path[i].as<uint32_t>();
@@ -112,22 +125,32 @@ static uint32_t kernel_grid(hk_ctx *ctx, uint32_t count) {
   return std::max(1u, g);
 }
 
-// Runs all passes of the given requests; paths of different requests share
-// each pass.  A request's samples are cut into fixed chunks (`chunk` samples
-// per pixel) that never depend on the other requests, so every request
+// Renders the given requests.  Their samples are cut into fixed chunks
+// (`chunk` samples per pixel, set by the sample capacity alone) and packed
+// into segments of at most cap_samples samples; each segment runs the
+// bounce loop with `pool` recycled path slots until all its samples have
+// finished, then resolves its chunks into the films in order.  Neither the
+// chunks nor a sample's path depend on the other requests, so every request
 // accumulates the same values in the same order alone or in a batch.
-static void run_passes(hk_scene *s, std::vector<ReqState> &rs, hk_render_stats *stats, uint32_t cap_paths,
-                       bool single_pass_no_resolve = false) {
+static void run_passes(hk_scene *s, std::vector<ReqState> &rs, hk_render_stats *stats, uint32_t pool_paths,
+                       uint32_t cap_samples, bool no_resolve = false) {
   hk_ctx *ctx = s->ctx;
   hipStream_t st = ctx->stream;
   Workspace &ws = ctx->ws;
-  ws.ensure(cap_paths, uint32_t(s->dev.n_emitters));
   std::vector<DevRequest> dreq(rs.size());
   for (size_t i = 0; i < rs.size(); i++) dreq[i] = rs[i].dev;
+  uint64_t total = 0;
+  for (const ReqState &q : rs) total += uint64_t(q.npix) * (q.spp - q.done);
+  uint32_t seg_cap = uint32_t(std::min<uint64_t>(cap_samples, std::max<uint64_t>(total, 1)));
+  uint32_t pool = std::max(1u, std::min(pool_paths, seg_cap));
+  ws.ensure(pool, uint32_t(s->dev.n_emitters), seg_cap);
   ws.reqs.upload(dreq.data(), dreq.size() * sizeof(DevRequest), st);
-  ws.items.alloc(sizeof(WorkItem) * (rs.size() + 1));
+  size_t max_items = 1;
+  for (const ReqState &q : rs) max_items += (q.spp - q.done + q.chunk - 1) / q.chunk;
+  ws.items.alloc(sizeof(WorkItem) * max_items);
   hip_check(hipMemsetAsync(ws.stats.p, 0, 64, st), "memset stats");
   RenderArgs A = make_args(s);
+  A.n_paths = pool;
   double trace_ms = 0;
   uint64_t launches = 0, paths_total = 0, trace_launches = 0;
   hip_check(hipEventRecord(ctx->ev2, st), "event");
@@ -140,11 +163,11 @@ static void run_passes(hk_scene *s, std::vector<ReqState> &rs, hk_render_stats *
       while (q.done < q.spp) {
         uint32_t k = std::min(q.chunk, q.spp - q.done);
         uint64_t need = uint64_t(q.npix) * k;
-        if (used + need > cap_paths) break;
+        if (used + need > seg_cap) break;
         WorkItem it;
         std::memset(&it, 0, sizeof it);
         it.req = r;
-        it.path_begin = used;
+        it.sample_begin = used;
         it.npix = q.npix;
         it.s0 = q.spp_offset + q.done;
         it.k = k;
@@ -152,18 +175,26 @@ static void run_passes(hk_scene *s, std::vector<ReqState> &rs, hk_render_stats *
         used += uint32_t(need);
         q.done += k;
       }
-      if (used >= cap_paths) break;
+      if (used >= seg_cap) break;
     }
     if (items.empty()) break;
     hip_check(hipMemcpyAsync(ws.items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st),
               "items upload");
     A.n_items = uint32_t(items.size());
-    A.n_paths = used;
+    A.n_samples = used;
     paths_total += used;
-    hip_check(launch_camera(A, 1u << 16, st), "k_camera");
-    uint32_t count = used;
+    hip_check(launch_iota_u32(A.free_slots, pool, st), "free slots");
+    uint32_t count = 0, free_top = pool, next = 0;
     int qi = 0;
-    while (count > 0) {
+    for (;;) {
+      uint32_t n_new = std::min(free_top, used - next);
+      if (n_new) {
+        hip_check(launch_regen(A, qi, count, free_top, n_new, next, st), "k_regen");
+        free_top -= n_new;
+        next += n_new;
+        count += n_new;
+      }
+      if (count == 0) break;
       uint32_t g = kernel_grid(ctx, count);
       if (ctx->profile) hip_check(hipEventRecord(ctx->ev0, st), "event");
       hip_check(launch_extend(A, qi, g, st), "k_extend");
@@ -182,17 +213,18 @@ static void run_passes(hk_scene *s, std::vector<ReqState> &rs, hk_render_stats *
       launches++;
       qi ^= 1;
       count = ctx->host_counters[qi];
+      free_top = ctx->host_counters[3];
     }
-    if (single_pass_no_resolve) {  // caller reads per-path results of this pass
-      hip_check(hipStreamSynchronize(st), "pass sync");
+    if (no_resolve) {  // caller reads the per-sample results of this segment
+      hip_check(hipStreamSynchronize(st), "segment sync");
       break;
     }
     for (uint32_t i = 0; i < items.size(); i++)
       hip_check(launch_resolve(A, i, items[i].npix, st), "k_resolve");
   }
   hip_check(hipEventRecord(ctx->ev3, st), "event");
-  unsigned long long hs[2] = {0, 0};
-  hip_check(hipMemcpyAsync(hs, ws.stats.p, 16, hipMemcpyDeviceToHost, st), "stats D2H");
+  unsigned long long hs[6] = {0, 0, 0, 0, 0, 0};
+  hip_check(hipMemcpyAsync(hs, ws.stats.p, sizeof hs, hipMemcpyDeviceToHost, st), "stats D2H");
   hip_check(hipStreamSynchronize(st), "render sync");
   if (stats) {
     float ms = 0;
@@ -204,10 +236,12 @@ static void run_passes(hk_scene *s, std::vector<ReqState> &rs, hk_render_stats *
     stats->gpu_ms += ms;
     stats->trace_ms += trace_ms;
     stats->trace_launches += trace_launches;
+    stats->node_visits += hs[2] + hs[4];
+    stats->triangle_tests += hs[3] + hs[5];
   }
 }
 
-static void request_setup(hk_scene *s, const hk_request *req, ReqState &q, uint32_t cap_paths) {
+static void request_setup(hk_scene *s, const hk_request *req, ReqState &q, uint32_t cap_samples) {
   const hk_scene_desc &d = s->desc;
   const float *c2w = req->use_scene_camera ? d.cam_to_world : req->cam_to_world;
   float fov = req->use_scene_camera ? d.fov : req->fov;
@@ -221,10 +255,11 @@ static void request_setup(hk_scene *s, const hk_request *req, ReqState &q, uint3
   filter_setup(d, q.dev);
   q.dev.seed = req->seed;
   q.npix = uint32_t(w) * uint32_t(h);
-  if (q.npix > cap_paths) throw HkError(HK_ERR_INVALID, "film of %u pixels exceeds the wavefront (%u paths)", q.npix, cap_paths);
+  if (q.npix > cap_samples)
+    throw HkError(HK_ERR_INVALID, "film of %u pixels exceeds the sample capacity (%u)", q.npix, cap_samples);
   q.spp = req->spp;
   q.spp_offset = req->spp_offset;
-  q.chunk = std::max(1u, std::min(q.spp == 0 ? 1u : q.spp, cap_paths / q.npix));
+  q.chunk = std::max(1u, std::min(q.spp == 0 ? 1u : q.spp, cap_samples / q.npix));
   q.done = 0;
   q.accum.alloc(size_t(q.npix) * 16);
   hip_check(hipMemsetAsync(q.accum.p, 0, size_t(q.npix) * 16, s->ctx->stream), "accum clear");
@@ -407,7 +442,7 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
   if (!emitters.empty()) s->emitters.upload(emitters.data(), emitters.size() * sizeof(DevEmitter), st);
   s->cdf.upload(cdf.data(), cdf.size() * 4, st);
 
-  ds.nodes = s->bvh.nodes.as<Bvh4Node>();
+  ds.nodes = s->bvh.nodes.as<TraceNode>();
   ds.tris = s->bvh.tris.as<TraceTri>();
   ds.V = s->V.as<float>();
   ds.N = s->N.as<float>();
@@ -446,8 +481,8 @@ int hk_render(hk_scene *s, const hk_request *req, float *out_rgb, hk_render_stat
   if (!s || !req) throw HkError(HK_ERR_INVALID, "null argument");
   if (stats) std::memset(stats, 0, sizeof *stats);
   std::vector<ReqState> rs(1);
-  request_setup(s, req, rs[0], ctx->max_paths);
-  run_passes(s, rs, stats, ctx->max_paths);
+  request_setup(s, req, rs[0], ctx->max_samples);
+  run_passes(s, rs, stats, ctx->max_paths, ctx->max_samples);
   finalize_to_host(s, rs[0], out_rgb);
   return HK_OK;
   HK_API_END(ctx)
@@ -459,10 +494,10 @@ int hk_render_batch(hk_scene *s, const hk_request *reqs, uint32_t n, float **out
   HK_API_BEGIN
   if (!s || (n && !reqs)) throw HkError(HK_ERR_INVALID, "null argument");
   if (stats) std::memset(stats, 0, sizeof *stats);
-  uint32_t cap = max_paths ? max_paths : ctx->max_paths;
+  uint32_t pool = max_paths ? max_paths : ctx->max_paths;
   std::vector<ReqState> rs(n);
-  for (uint32_t i = 0; i < n; i++) request_setup(s, &reqs[i], rs[i], cap);
-  run_passes(s, rs, stats, cap);
+  for (uint32_t i = 0; i < n; i++) request_setup(s, &reqs[i], rs[i], ctx->max_samples);
+  run_passes(s, rs, stats, pool, ctx->max_samples);
   for (uint32_t i = 0; i < n; i++) finalize_to_host(s, rs[i], out_rgb ? out_rgb[i] : nullptr);
   return HK_OK;
   HK_API_END(ctx)
@@ -474,7 +509,7 @@ int hk_render_pixel_samples(hk_scene *s, uint32_t seed, uint32_t x, uint32_t y, 
   if (!s || !out) throw HkError(HK_ERR_INVALID, "null argument");
   const hk_scene_desc &d = s->desc;
   if (x >= uint32_t(d.width) || y >= uint32_t(d.height)) throw HkError(HK_ERR_INVALID, "pixel out of range");
-  uint32_t cap = ctx->max_paths;
+  uint32_t cap = ctx->max_samples;
   uint32_t pix = y * uint32_t(d.width) + x;
   for (uint32_t done = 0; done < n;) {
     uint32_t k = std::min(cap, n - done);
@@ -491,9 +526,9 @@ int hk_render_pixel_samples(hk_scene *s, uint32_t seed, uint32_t x, uint32_t y, 
     rs[0].dev.pix0 = pix;
     rs[0].npix = 1;
     rs[0].chunk = k;
-    run_passes(s, rs, nullptr, std::max(k, 1u), true);
+    run_passes(s, rs, nullptr, ctx->max_paths, std::max(k, 1u), true);
     std::vector<float> L(size_t(k) * 4);
-    hip_check(hipMemcpy(L.data(), ctx->ws.ps_L.p, size_t(k) * 16, hipMemcpyDeviceToHost), "D2H");
+    hip_check(hipMemcpy(L.data(), ctx->ws.sample_L.p, size_t(k) * 16, hipMemcpyDeviceToHost), "D2H");
     for (uint32_t i = 0; i < k; i++) {
       Rng r = rng_for_path(seed, pix, s0 + done + i);
       float jx = r.next(), jy = r.next();

@@ -54,6 +54,27 @@ struct alignas(64) Bvh4Node {
 };
 static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node layout");
 
+// Quantized BVH4 node (64 B, one cache line): the node box's lower corner and
+// per-axis power-of-two steps; child boxes as 8-bit multiples of the step,
+// rounded outward on the host so that the decoded box, fmaf(q, step, lo),
+// contains the exact child box.  Same child encoding as Bvh4Node.
+struct alignas(64) Bvh4QNode {
+  float ox, oy, oz;
+  uint32_t exps;        // byte a: biased FP32 exponent of the step on axis a
+  int32_t child[4];
+  uint8_t q[24];        // lo_x[4] hi_x[4] lo_y[4] hi_y[4] lo_z[4] hi_z[4]
+  uint32_t pad[2];
+};
+static_assert(sizeof(Bvh4QNode) == 64, "Bvh4QNode layout");
+
+// Node format the kernels walk: quantized 64-B nodes unless the library is
+// built with HK_NODE_FLOAT (128-B FP32 nodes, for comparison).
+#ifdef HK_NODE_FLOAT
+using TraceNode = Bvh4Node;
+#else
+using TraceNode = Bvh4QNode;
+#endif
+
 // Leaf-ordered triangle for traversal (48 B): p0 + prim id, e1 = p1 - p0,
 // e2 = p2 - p0 (edges computed in FP32 on the host exactly as the reference
 // computes them per test, src/core/Mesh.cpp:349).
@@ -65,7 +86,7 @@ struct alignas(16) TraceTri {
 static_assert(sizeof(TraceTri) == 48, "TraceTri layout");
 
 struct DevScene {
-  const Bvh4Node *nodes;
+  const TraceNode *nodes;
   const TraceTri *tris;
   const float *V;          // nv x 3 (global vertex array)
   const float *N;          // nv x 3 or null
@@ -109,10 +130,11 @@ struct DevRequest {
   float *accum;            // width x height x 4 (rgb * w, w)
 };
 
-// A slice of one request processed in one pass: every pixel, samples
-// [s0, s0 + k); paths [path_begin, path_begin + npix * k), sample-major.
+// A slice of one request processed in one segment: every pixel, samples
+// [s0, s0 + k); segment sample ids [sample_begin, sample_begin + npix * k),
+// sample-major.
 struct WorkItem {
-  uint32_t req, path_begin, npix, s0, k, pad0, pad1, pad2;
+  uint32_t req, sample_begin, npix, s0, k, pad0, pad1, pad2;
 };
 
 }  // namespace hk

@@ -155,6 +155,7 @@ typedef struct hk_render_stats {
   double gpu_ms;          /* device time of the render (HIP events) */
   double trace_ms;        /* device time inside the closest-hit kernel */
   uint64_t trace_launches;
+  uint64_t node_visits, triangle_tests;  /* BVH work; non-zero only in builds with HK_TRAVERSAL_STATS */
 } hk_render_stats;
 
 /* Whole-image render with the gpuPathMIS integrator (the wavefront form of
@@ -171,12 +172,19 @@ int hk_render_pixel_samples(hk_scene *scene, uint32_t seed, uint32_t x, uint32_t
 /* Batched renders: n requests over one resident scene share one wavefront
  * (paths tagged by request).  out_rgb[i]: host buffer of request i
  * (height x width x 3).  Results equal the requests rendered alone, bit for
- * bit.  max_paths bounds the wavefront (0: automatic). */
+ * bit.  max_paths: path slots of the wavefront (0: the context's). */
 int hk_render_batch(hk_scene *scene, const hk_request *reqs, uint32_t n, float **out_rgb, uint32_t max_paths,
                     hk_render_stats *stats);
 
-/* Tuning knobs (0 keeps the default): wavefront capacity in paths and
- * samples per pixel per pass. */
+/* Tuning knobs (value 0 restores the default):
+ *   "max_paths"   path slots of the wavefront (default 4M); finished paths
+ *                 hand their slot to the next sample at the next bounce
+ *   "max_samples" samples per segment (default 256M, 16 B of HBM each): a
+ *                 request's samples are cut into chunks of
+ *                 floor(max_samples / pixels) samples per pixel, resolved
+ *                 into the film when their segment ends
+ *   "grid"        cap on blocks per queue-kernel launch (0: one lane per item)
+ *   "profile"     1: time the closest-hit kernel with HIP events */
 int hk_set_option(hk_ctx *ctx, const char *name, int64_t value);
 
 #ifdef __cplusplus

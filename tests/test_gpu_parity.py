@@ -180,3 +180,41 @@ def test_cbox_image_vs_oracle(ctx):
     print("relMSE vs oracle (same streams):", rm, "gpu stats", st, "oracle stats", ost)
     assert rm < 1e-3
     assert abs(lum(img).mean() - lum(ref).mean()) / lum(ref).mean() < 0.01
+
+
+def test_slot_recycling_and_chunking_do_not_change_bits():
+    """Path-slot recycling (tiny wavefront) and sample chunking (tiny sample
+    capacity, several segments) leave every film bit unchanged."""
+    sc = scenes.cbox(width=40, height=32, sphere_subdiv=2)
+    imgs = []
+    for max_paths, max_samples in [(0, 0), (777, 0), (0, 40 * 32 * 5), (301, 40 * 32 * 3 + 17)]:
+        c = hk.Context(0)
+        c.set_option("max_paths", max_paths)
+        c.set_option("max_samples", max_samples)
+        img, st = c.scene(sc).render(12, seed=5)
+        print(max_paths, max_samples, st)
+        assert st["paths"] == 40 * 32 * 12
+        imgs.append(img)
+        c.close()
+    for im in imgs[1:]:
+        assert np.array_equal(im.view(np.uint32), imgs[0].view(np.uint32))
+
+
+def test_batch_equals_alone(ctx):
+    """Requests rendered in one batch (shared wavefront) equal the same
+    requests rendered alone, bit for bit."""
+    sc = scenes.cbox(width=32, height=24, sphere_subdiv=2)
+    gs = ctx.scene(sc)
+    reqs = [hk.make_request(sc, spp=6, seed=1),
+            hk.make_request(cam_to_world=D.lookat((0.3, 0.1, 0.99), (0, 0, -1), (0, 1, 0)), fov=35.0, width=20,
+                            height=16, spp=9, seed=2),
+            hk.make_request(sc, spp=3, seed=3, spp_offset=7),
+            hk.make_request(cam_to_world=D.lookat((-0.5, 0.5, 0.9), (0, -0.2, -1), (0, 1, 0)), fov=50.0, width=13,
+                            height=11, spp=17, seed=4)]
+    alone = [gs.render(0, request=r)[0] for r in reqs]
+    for max_paths in (0, 500):
+        batch, st = gs.render_batch(reqs, max_paths=max_paths)
+        assert st["paths"] == sum(r.spp * (r.width * r.height) for r in reqs)
+        for a, b in zip(alone, batch):
+            assert a.shape == b.shape
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
