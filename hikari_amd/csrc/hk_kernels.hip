@@ -7,13 +7,16 @@
 //  * k_bsdf_batch: batched BSDF::Sample / Eval / Pdf for parity tests.
 //  * wavefront pathMIS (PathMISIntegrator::Li, src/integrator/
 //    PathMISIntegrator.cpp:16-157): [k_regen -> k_extend -> k_shade ->
-//    k_shadow]* -> k_resolve.  Path state is slot-indexed SoA; ray, hit and
-//    shadow queues are compacted with wave ballots (one atomic per wave);
-//    k_regen refills the slots of finished paths with new samples, so every
-//    bounce launch stays full until the segment's samples run out.
+//    k_shadow]* -> k_resolve.  Path state lives in the extension queues
+//    (SoA, compacted with wave ballots, one atomic per wave) so every kernel
+//    reads and writes it coalesced; k_regen tops the queue up with new
+//    samples, so every bounce launch stays full until the segment's samples
+//    run out.
 //    Film accumulation is owned per output pixel (gather in k_resolve), so
 //    results are deterministic and independent of scheduling.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "hk_bsdf.h"
 #include "hk_kernels.h"
@@ -54,114 +57,152 @@ struct TraceStack {
 
 __device__ __forceinline__ float ubyte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
 
-// Closest (ANY = false) or any (ANY = true) hit over the quantized BVH4.
-// Inner loop descends internal nodes until the lane holds a leaf (or is
-// done), so leaf tests run with the wave converged ("while-while").
 struct TraceCount {
   uint32_t nodes = 0, tris = 0;
 };
 
-template <bool ANY>
-__device__ __forceinline__ bool trace_ray(const TraceNode *__restrict__ nodes, const TraceTri *__restrict__ tris,
-                                          int32_t root, int2 *lds_stack, vec3 o, vec3 d, float tmin, float tmax,
-                                          float &bt, float &bu, float &bv, int32_t &bprim, TraceCount &cnt) {
-  float ix = fabsf(d.x) > 1e-30f ? 1.0f / d.x : copysignf(1e30f, d.x);
-  float iy = fabsf(d.y) > 1e-30f ? 1.0f / d.y : copysignf(1e30f, d.y);
-  float iz = fabsf(d.z) > 1e-30f ? 1.0f / d.z : copysignf(1e30f, d.z);
+// One lane's traversal state (closest or any hit over the BVH4).  Internal
+// nodes are processed by node_step, leaves by leaf_step; a lane is done when
+// ref == kDone.  Slab tests are widened by a few ulp (tnear <= tfar *
+// (1 + 2^-21)), which covers the rounding of (lo - o) * (1/d); the host pads
+// triangle boxes for the inexactness of the triangle test and rounds
+// quantized boxes outward.
+constexpr float kWiden = 1.0000005f;
+struct RayTrav {
+  vec3 o, d;
+  float ix, iy, iz, tmin;
+  float bt, bu, bv;
+  int32_t bprim;
+  int32_t ref;
   TraceStack st;
-  st.lds = lds_stack;
-  st.tid = threadIdx.x;
-  st.sp = 0;
-  bt = tmax;
-  bprim = -1;
-  bu = bv = 0.0f;
-  // Slab tests are widened by a few ulp (tnear <= tfar * (1 + 2^-21)), which
-  // covers the rounding of (lo - o) * (1/d); the host pads triangle boxes for
-  // the inexactness of the triangle test and rounds quantized boxes outward.
-  const float kWiden = 1.0000005f;
-  int32_t ref = root;
-  for (;;) {
-    while (ref >= 0) {
+};
+
+__device__ __forceinline__ void trav_begin(RayTrav &R, int2 *lds_stack, int32_t root, vec3 o, vec3 d, float tmin,
+                                           float tmax) {
+  R.o = o;
+  R.d = d;
+  R.ix = fabsf(d.x) > 1e-30f ? 1.0f / d.x : copysignf(1e30f, d.x);
+  R.iy = fabsf(d.y) > 1e-30f ? 1.0f / d.y : copysignf(1e30f, d.y);
+  R.iz = fabsf(d.z) > 1e-30f ? 1.0f / d.z : copysignf(1e30f, d.z);
+  R.tmin = tmin;
+  R.bt = tmax;
+  R.bu = R.bv = 0.0f;
+  R.bprim = -1;
+  R.ref = root;
+  R.st.lds = lds_stack;
+  R.st.tid = threadIdx.x;
+  R.st.sp = 0;
+}
+
+// Internal node R.ref: pushes the hit children far to near, continues with
+// the nearest one (or the next stack entry that can still hold a hit).
+__device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, RayTrav &R, TraceCount &cnt) {
 #ifdef HK_TRAVERSAL_STATS
-      cnt.nodes++;
+  cnt.nodes++;
 #endif
-      float tlim = bt * kWiden;
-      float tn[4];
-      int32_t r[4];
+  const vec3 o = R.o;
+  const float ix = R.ix, iy = R.iy, iz = R.iz;
+  float tlim = R.bt * kWiden;
+  float tn[4];
+  int32_t r[4];
 #ifdef HK_NODE_FLOAT
-      const float4 *np = reinterpret_cast<const float4 *>(nodes + ref);
-      float4 lx4 = np[0], hx4 = np[1], ly4 = np[2], hy4 = np[3], lz4 = np[4], hz4 = np[5];
-      int4 ch = *reinterpret_cast<const int4 *>(np + 6);
-      float lxs[4] = {lx4.x, lx4.y, lx4.z, lx4.w}, hxs[4] = {hx4.x, hx4.y, hx4.z, hx4.w};
-      float lys[4] = {ly4.x, ly4.y, ly4.z, ly4.w}, hys[4] = {hy4.x, hy4.y, hy4.z, hy4.w};
-      float lzs[4] = {lz4.x, lz4.y, lz4.z, lz4.w}, hzs[4] = {hz4.x, hz4.y, hz4.z, hz4.w};
+  const float4 *np = reinterpret_cast<const float4 *>(nodes + R.ref);
+  float4 lx4 = np[0], hx4 = np[1], ly4 = np[2], hy4 = np[3], lz4 = np[4], hz4 = np[5];
+  int4 ch = *reinterpret_cast<const int4 *>(np + 6);
+  float lxs[4] = {lx4.x, lx4.y, lx4.z, lx4.w}, hxs[4] = {hx4.x, hx4.y, hx4.z, hx4.w};
+  float lys[4] = {ly4.x, ly4.y, ly4.z, ly4.w}, hys[4] = {hy4.x, hy4.y, hy4.z, hy4.w};
+  float lzs[4] = {lz4.x, lz4.y, lz4.z, lz4.w}, hzs[4] = {hz4.x, hz4.y, hz4.z, hz4.w};
 #else
-      const Bvh4QNode *np = nodes + ref;
-      float4 h = *reinterpret_cast<const float4 *>(np);
-      int4 ch = *reinterpret_cast<const int4 *>(&np->child[0]);
-      uint4 qa = *reinterpret_cast<const uint4 *>(&np->q[0]);
-      uint2 qb = *reinterpret_cast<const uint2 *>(&np->q[16]);
-      uint32_t ex = __float_as_uint(h.w);
-      float sx = __uint_as_float((ex & 0xffu) << 23), sy = __uint_as_float(((ex >> 8) & 0xffu) << 23);
-      float sz = __uint_as_float(((ex >> 16) & 0xffu) << 23);
-      float lxs[4], hxs[4], lys[4], hys[4], lzs[4], hzs[4];
+  const Bvh4QNode *np = nodes + R.ref;
+  float4 h = *reinterpret_cast<const float4 *>(np);
+  int4 ch = *reinterpret_cast<const int4 *>(&np->child[0]);
+  uint4 qa = *reinterpret_cast<const uint4 *>(&np->q[0]);
+  uint2 qb = *reinterpret_cast<const uint2 *>(&np->q[16]);
+  uint32_t ex = __float_as_uint(h.w);
+  float sx = __uint_as_float((ex & 0xffu) << 23), sy = __uint_as_float(((ex >> 8) & 0xffu) << 23);
+  float sz = __uint_as_float(((ex >> 16) & 0xffu) << 23);
+  float lxs[4], hxs[4], lys[4], hys[4], lzs[4], hzs[4];
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        lxs[c] = fmaf(ubyte_f(qa.x, c), sx, h.x);
-        hxs[c] = fmaf(ubyte_f(qa.y, c), sx, h.x);
-        lys[c] = fmaf(ubyte_f(qa.z, c), sy, h.y);
-        hys[c] = fmaf(ubyte_f(qa.w, c), sy, h.y);
-        lzs[c] = fmaf(ubyte_f(qb.x, c), sz, h.z);
-        hzs[c] = fmaf(ubyte_f(qb.y, c), sz, h.z);
-      }
+  for (int c = 0; c < 4; c++) {
+    lxs[c] = fmaf(ubyte_f(qa.x, c), sx, h.x);
+    hxs[c] = fmaf(ubyte_f(qa.y, c), sx, h.x);
+    lys[c] = fmaf(ubyte_f(qa.z, c), sy, h.y);
+    hys[c] = fmaf(ubyte_f(qa.w, c), sy, h.y);
+    lzs[c] = fmaf(ubyte_f(qb.x, c), sz, h.z);
+    hzs[c] = fmaf(ubyte_f(qb.y, c), sz, h.z);
+  }
 #endif
-      r[0] = ch.x; r[1] = ch.y; r[2] = ch.z; r[3] = ch.w;
+  r[0] = ch.x; r[1] = ch.y; r[2] = ch.z; r[3] = ch.w;
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        float t0x = (lxs[c] - o.x) * ix, t1x = (hxs[c] - o.x) * ix;
-        float t0y = (lys[c] - o.y) * iy, t1y = (hys[c] - o.y) * iy;
-        float t0z = (lzs[c] - o.z) * iz, t1z = (hzs[c] - o.z) * iz;
-        float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-        float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tlim));
-        tn[c] = (r[c] != kEmptyRef && a <= b * kWiden) ? a : kInfF;
-      }
+  for (int c = 0; c < 4; c++) {
+    float t0x = (lxs[c] - o.x) * ix, t1x = (hxs[c] - o.x) * ix;
+    float t0y = (lys[c] - o.y) * iy, t1y = (hys[c] - o.y) * iy;
+    float t0z = (lzs[c] - o.z) * iz, t1z = (hzs[c] - o.z) * iz;
+    float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), R.tmin));
+    float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tlim));
+    tn[c] = (r[c] != kEmptyRef && a <= b * kWiden) ? a : kInfF;
+  }
 #define HK_CSWAP(i, j)                                    \
   if (tn[j] < tn[i]) {                                    \
     float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;          \
     int32_t rr = r[i]; r[i] = r[j]; r[j] = rr;            \
   }
-      HK_CSWAP(0, 1) HK_CSWAP(2, 3) HK_CSWAP(0, 2) HK_CSWAP(1, 3) HK_CSWAP(1, 2)
+  HK_CSWAP(0, 1) HK_CSWAP(2, 3) HK_CSWAP(0, 2) HK_CSWAP(1, 3) HK_CSWAP(1, 2)
 #undef HK_CSWAP
-      if (tn[3] != kInfF) st.push(r[3], tn[3]);
-      if (tn[2] != kInfF) st.push(r[2], tn[2]);
-      if (tn[1] != kInfF) st.push(r[1], tn[1]);
-      ref = tn[0] != kInfF ? r[0] : st.pop(bt * kWiden);
-    }
-    if (ref == kDone) break;
-    uint32_t code = uint32_t(~ref);
-    uint32_t first = code >> 3, ntri = (code & 7u) + 1u;
+  if (tn[3] != kInfF) R.st.push(r[3], tn[3]);
+  if (tn[2] != kInfF) R.st.push(r[2], tn[2]);
+  if (tn[1] != kInfF) R.st.push(r[1], tn[1]);
+  R.ref = tn[0] != kInfF ? r[0] : R.st.pop(R.bt * kWiden);
+}
+
+// Leaf R.ref: exact FP32 Moller-Trumbore on its triangles, then the next
+// stack entry.  Closest hit: among equal t the largest prim id wins (the
+// order of the reference's brute-force loop); any hit: done at the first.
+template <bool ANY>
+__device__ __forceinline__ void leaf_step(const TraceTri *__restrict__ tris, RayTrav &R, TraceCount &cnt) {
+  uint32_t code = uint32_t(~R.ref);
+  uint32_t first = code >> 3, ntri = (code & 7u) + 1u;
 #ifdef HK_TRAVERSAL_STATS
-    cnt.tris += ntri;
+  cnt.tris += ntri;
 #endif
-    for (uint32_t k = 0; k < ntri; k++) {
-      const float4 *tp = reinterpret_cast<const float4 *>(tris + first + k);
-      float4 a = tp[0], b = tp[1], c = tp[2];
-      float u, v, t;
-      if (mt_test(o, d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(b.w, c.x, c.y), tmin, bt, u, v, t)) {
-        int32_t prim = __float_as_int(a.w);
-        if (ANY) {
-          bt = t; bu = u; bv = v; bprim = prim;
-          return true;
-        }
-        // closest hit; among equal t the largest prim (brute-force order)
-        if (t < bt || prim > bprim) {
-          bt = t; bu = u; bv = v; bprim = prim;
-        }
+  for (uint32_t k = 0; k < ntri; k++) {
+    const float4 *tp = reinterpret_cast<const float4 *>(tris + first + k);
+    float4 a = tp[0], b = tp[1], c = tp[2];
+    float u, v, t;
+    if (mt_test(R.o, R.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(b.w, c.x, c.y), R.tmin, R.bt, u, v, t)) {
+      int32_t prim = __float_as_int(a.w);
+      if (ANY) {
+        R.bt = t; R.bu = u; R.bv = v; R.bprim = prim;
+        R.ref = kDone;
+        return;
+      }
+      if (t < R.bt || prim > R.bprim) {
+        R.bt = t; R.bu = u; R.bv = v; R.bprim = prim;
       }
     }
-    ref = st.pop(bt * kWiden);
   }
-  return bprim >= 0;
+  R.ref = R.st.pop(R.bt * kWiden);
+}
+
+// Closest (ANY = false) or any (ANY = true) hit of one ray.  The inner loop
+// descends internal nodes until the lane holds a leaf (or is done), so leaf
+// tests run with the wave converged ("while-while").
+template <bool ANY>
+__device__ __forceinline__ bool trace_ray(const TraceNode *__restrict__ nodes, const TraceTri *__restrict__ tris,
+                                          int32_t root, int2 *lds_stack, vec3 o, vec3 d, float tmin, float tmax,
+                                          float &bt, float &bu, float &bv, int32_t &bprim, TraceCount &cnt) {
+  RayTrav R;
+  trav_begin(R, lds_stack, root, o, d, tmin, tmax);
+  for (;;) {
+    while (R.ref >= 0) node_step(nodes, R, cnt);
+    if (R.ref == kDone) break;
+    leaf_step<ANY>(tris, R, cnt);
+  }
+  bt = R.bt;
+  bu = R.bu;
+  bv = R.bv;
+  bprim = R.bprim;
+  return R.bprim >= 0;
 }
 
 // ------------------------------------------------------------------ batched rayIntersect
@@ -395,22 +436,20 @@ constexpr uint32_t kFlagPrevDiscrete = 1u;
 constexpr uint32_t kShadowLast = 0x80000000u;  // shadow-queue mask bit: the path ended at this bounce
 
 // ------------------------------------------------------------------ k_regen
-// Starts samples g0 .. g0 + n_new - 1 of the segment in the top n_new free
-// path slots and appends their camera rays to extension queue qi after the
-// q_base paths already there.  Per sample: RenderBlock's prologue
-// (src/Main.cpp:22-31) + PerspectiveCamera::SampleRay (src/camera/
-// PerspectiveCamera.cpp:77-102).  A sample's path depends only on (request,
-// pixel, sample index), never on the slot it runs in.
-__global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t q_base, uint32_t free_top,
-                                                  uint32_t n_new, uint32_t g0) {
+// Starts samples g0 .. g0 + n_new - 1 of the segment: their camera paths go
+// to queue qi after the q_base continuing paths already there.  Per sample:
+// RenderBlock's prologue (src/Main.cpp:22-31) + PerspectiveCamera::SampleRay
+// (src/camera/PerspectiveCamera.cpp:77-102).  A sample's path depends only on
+// (request, pixel, sample index), never on its queue position.
+__global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t q_base, uint32_t n_new,
+                                                  uint32_t g0) {
   uint32_t stride = gridDim.x * blockDim.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     A.counters[qi] = q_base + n_new;
-    A.counters[3] = free_top - n_new;
+    A.counters[4] = 0u;  // k_extend's fetch counter
   }
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_new; j += stride) {
     uint32_t g = g0 + j;
-    uint32_t slot = A.free_slots[free_top - 1u - j];
     const WorkItem it = A.items[find_item(A.items, A.n_items, g)];
     const DevRequest &rq = A.reqs[it.req];
     uint32_t local = g - it.sample_begin;
@@ -432,30 +471,95 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t
     vec3 d = v3(c[0] * dir.x + c[1] * dir.y + c[2] * dir.z, c[4] * dir.x + c[5] * dir.y + c[6] * dir.z,
                 c[8] * dir.x + c[9] * dir.y + c[10] * dir.z);
     uint32_t pos = q_base + j;
-    A.q_path[qi][pos] = slot;
     A.q_o[qi][pos] = make_float4(o.x, o.y, o.z, rq.near_clip * inv_z);
     A.q_d[qi][pos] = make_float4(d.x, d.y, d.z, rq.far_clip * inv_z);
-    A.ps_beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    A.ps_L[slot] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    A.ps_rng[slot] = rng.state;
-    A.ps_pix[slot] = q;
-    A.ps_sid[slot] = g;
-    A.ps_meta[slot] = 0u;
+    A.q_beta[qi][pos] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    A.q_L[qi][pos] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    A.q_aux[qi][pos] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), q, g);
+    A.q_meta[qi][pos] = 0u;
   }
 }
 
 // ------------------------------------------------------------------ k_extend
+// Closest hit of every queued ray (Scene::RayIntersect, src/core/Scene.cpp:
+// 117-120), one lane per ray.
+// Persistent form (HK_PERSISTENT_TRACE builds): a wave keeps tracing and, whenever at least
+// kRefill of its lanes have finished their rays, fetches that many new queue
+// entries with one atomic (counters[4], cleared by the kernel before), so
+// a few long rays no longer hold a whole wave.
+#ifndef HK_REFILL
+#define HK_REFILL 16
+#endif
+constexpr int kRefill = HK_REFILL;
+static_assert(kRefill >= 1 && kRefill <= 64, "refill threshold must fit a wave");
+constexpr uint32_t kNoRay = 0xffffffffu;
+
+#ifdef HK_PERSISTENT_TRACE
 __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   __shared__ int2 stack[kLdsStack * kBlock];
   uint32_t count = A.counters[qi];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     A.counters[qi ^ 1] = 0u;
     A.counters[2] = 0u;
+    A.counters[5] = 0u;  // k_shadow's fetch counter
+    atomicAdd(&A.stats[0], (unsigned long long)count);
+  }
+  const float4 *qo = A.q_o[qi], *qd = A.q_d[qi];
+  float4 *qh = A.hit;
+  TraceCount tc;
+  RayTrav R;
+  R.ref = kDone;
+  uint32_t rid = kNoRay;
+  bool exhausted = false;
+  uint32_t lane = lane_id();
+  for (;;) {
+    unsigned long long idle = __ballot(rid == kNoRay);
+    int n_idle = __popcll(idle);
+    if (!exhausted && n_idle >= kRefill) {
+      uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&A.counters[4], (uint32_t)n_idle);
+      base = __shfl(base, (int)leader);
+      if (base + (uint32_t)n_idle >= count) exhausted = true;
+      if (rid == kNoRay) {
+        unsigned long long below = lane == 0 ? 0ull : (idle & ((1ull << lane) - 1ull));
+        uint32_t idx = base + (uint32_t)__popcll(below);
+        if (idx < count) {
+          rid = idx;
+          float4 a = qo[idx], b = qd[idx];
+          trav_begin(R, stack, A.sc.root_ref, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w);
+        }
+      }
+    }
+    if (__ballot(rid != kNoRay) == 0ull) {
+      if (exhausted) break;
+      continue;  // every lane idle: the next round fetches a full wave
+    }
+    while (R.ref >= 0) node_step(A.sc.nodes, R, tc);
+    if (R.ref != kDone) leaf_step<false>(A.sc.tris, R, tc);
+    if (rid != kNoRay && R.ref == kDone) {
+      qh[rid] = R.bprim >= 0 ? make_float4(R.bt, R.bu, R.bv, __int_as_float(R.bprim))
+                             : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+      rid = kNoRay;
+    }
+  }
+#ifdef HK_TRAVERSAL_STATS
+  count_flush(A.stats + 2, tc);
+#endif
+}
+#else
+__global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
+  __shared__ int2 stack[kLdsStack * kBlock];
+  uint32_t count = A.counters[qi];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    A.counters[qi ^ 1] = 0u;
+    A.counters[2] = 0u;
+    A.counters[5] = 0u;  // k_shadow's fetch counter (persistent build)
     atomicAdd(&A.stats[0], (unsigned long long)count);
   }
   uint32_t stride = gridDim.x * blockDim.x;
   const float4 *qo = A.q_o[qi], *qd = A.q_d[qi];
-  float4 *qh = A.q_hit[qi];
+  float4 *qh = A.hit;
   TraceCount tc;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
     float4 a = qo[i], b = qd[i];
@@ -469,31 +573,38 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   count_flush(A.stats + 2, tc);
 #endif
 }
+#endif
 
 // ------------------------------------------------------------------ k_shade
 // One bounce of PathMISIntegrator::Li for every queued path.
 __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
   const DevScene &sc = A.sc;
   uint32_t count = A.counters[qi];
+  if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[4] = 0u;  // k_extend's fetch counter
   int qn = qi ^ 1;
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool active = i < count;
     bool cont = false;
-    uint32_t smask = 0, path = 0;
-    vec3 P = v3(0.0f), nd = v3(0.0f), Lout = v3(0.0f);
+    uint32_t smask = 0, pix = 0, sid = 0, meta_out = 0;
+    vec3 P = v3(0.0f), nd = v3(0.0f), L = v3(0.0f), beta = v3(0.0f);
+    float pdf_b = 0.0f, w_mats = 0.0f;
+    Rng rng;
     if (active) {
-      path = A.q_path[qi][i];
-      float4 o4 = A.q_o[qi][i], d4 = A.q_d[qi][i], h4 = A.q_hit[qi][i];
-      float4 b4 = A.ps_beta[path], l4 = A.ps_L[path];
-      uint32_t meta = A.ps_meta[path];
-      Rng rng;
-      rng.state = A.ps_rng[path];
-      rng.inc = rng_inc(A.ps_pix[path]);
+      float4 o4 = A.q_o[qi][i], d4 = A.q_d[qi][i], h4 = A.hit[i];
+      float4 b4 = A.q_beta[qi][i], l4 = A.q_L[qi][i];
+      uint4 aux = A.q_aux[qi][i];
+      uint32_t meta = A.q_meta[qi][i];
+      pix = aux.z;
+      sid = aux.w;
+      rng.state = (uint64_t)aux.x | ((uint64_t)aux.y << 32);
+      rng.inc = rng_inc(pix);
       uint32_t depth = meta & 0xffffu, flags = meta >> 16;
-      vec3 beta = v3(b4.x, b4.y, b4.z), L = v3(l4.x, l4.y, l4.z);
-      float pdf_b = b4.w, w_mats = l4.w;
+      beta = v3(b4.x, b4.y, b4.z);
+      L = v3(l4.x, l4.y, l4.z);
+      pdf_b = b4.w;
+      w_mats = l4.w;
       vec3 ro = v3(o4.x, o4.y, o4.z), rd = v3(d4.x, d4.y, d4.z);
       int32_t prim = __float_as_int(h4.w);
       if (prim < 0) {  // escaped (PathMISIntegrator.cpp:33-66)
@@ -540,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
             float w_ems = (er.pdf + pb != 0.0f) ? er.pdf / (er.pdf + pb) : 1.0f;
             vec3 c = beta * bsdf_eval(sc.mats, sc.rft, mat, br) * fabsf(br.wo.z) * ld * w_ems;
             if (!(c.x == 0.0f && c.y == 0.0f && c.z == 0.0f)) {
-              size_t slot = size_t(e) * A.n_paths + path;
+              size_t slot = size_t(e) * A.n_paths + i;
               vec3 sd = er.p - P;
               A.sh_o[slot] = make_float4(P.x, P.y, P.z, 0.0f);
               A.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, 0.0f);
@@ -574,58 +685,125 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
           }
         }
       }
-      A.ps_beta[path] = make_float4(beta.x, beta.y, beta.z, pdf_b);
-      A.ps_L[path] = make_float4(L.x, L.y, L.z, w_mats);
-      Lout = L;
-      A.ps_rng[path] = rng.state;
-      A.ps_meta[path] = (depth & 0xffffu) | (flags << 16);
+      meta_out = (depth & 0xffffu) | (flags << 16);
     }
     uint32_t slot = wave_append(&A.counters[qn], cont);
-    if (cont) {
-      A.q_path[qn][slot] = path;
+    if (cont) {  // the path moves to the compacted next queue
       A.q_o[qn][slot] = make_float4(P.x, P.y, P.z, 1e-4f);
       A.q_d[qn][slot] = make_float4(nd.x, nd.y, nd.z, kInfF);
+      A.q_beta[qn][slot] = make_float4(beta.x, beta.y, beta.z, pdf_b);
+      A.q_L[qn][slot] = make_float4(L.x, L.y, L.z, w_mats);
+      A.q_aux[qn][slot] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), pix, sid);
+      A.q_meta[qn][slot] = meta_out;
+    } else if (active) {  // finished: radiance so far to the sample buffer
+      A.sample_L[sid] = make_float4(L.x, L.y, L.z, 0.0f);
     }
-    // A finished path frees its slot now (k_regen runs only after k_shadow);
-    // its radiance goes to the sample buffer here, or in k_shadow when
-    // emitter samples of this bounce are still pending.
-    bool done = active && !cont;
-    uint32_t fs = wave_append(&A.counters[3], done);
-    if (done) {
-      A.free_slots[fs] = path;
-      if (smask == 0u) {
-        A.sample_L[A.ps_sid[path]] = make_float4(Lout.x, Lout.y, Lout.z, 0.0f);
-      } else {
-        smask |= kShadowLast;
-      }
-    }
+    // pending emitter samples of this bounce: k_shadow adds the unoccluded
+    // ones to the path's radiance where it now lives
     uint32_t ss = wave_append(&A.counters[2], smask != 0u);
-    if (smask != 0u) {
-      A.s_path[ss] = path;
-      A.s_mask[ss] = smask;
-    }
+    if (smask != 0u) A.s_ent[ss] = make_uint4(i, cont ? slot : (sid | kShadowLast), smask, 0u);
   }
 }
 
 // ------------------------------------------------------------------ k_shadow
 // Scene::ShadowRayIntersect (src/core/Scene.cpp:122-126) for the queued
-// emitter samples; one lane owns a path, so its radiance needs no atomics.
-__global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A) {
+// emitter samples.  One lane owns a shadow entry (one path): it traces the
+// entry's emitter rays in emitter order and adds the unoccluded
+// contributions to the path's radiance, so no atomics are needed and the
+// additions happen in the reference's order.  HK_PERSISTENT_TRACE builds
+// refill lanes as k_extend does (fetch counter counters[5], cleared by
+// k_extend).
+#ifdef HK_PERSISTENT_TRACE
+__global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
+  __shared__ int2 stack[kLdsStack * kBlock];
+  uint32_t count = A.counters[2];
+  uint32_t traced = 0;
+  TraceCount tc;
+  RayTrav R;
+  R.ref = kDone;
+  uint32_t ent = kNoRay, mask = 0, src = 0;
+  float4 *lp = nullptr;
+  vec3 L = v3(0.0f), c = v3(0.0f);
+  float lw = 0.0f;
+  bool exhausted = false, start = false;
+  uint32_t lane = lane_id();
+  for (;;) {
+    unsigned long long idle = __ballot(ent == kNoRay);
+    int n_idle = __popcll(idle);
+    if (!exhausted && n_idle >= kRefill) {
+      uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&A.counters[5], (uint32_t)n_idle);
+      base = __shfl(base, (int)leader);
+      if (base + (uint32_t)n_idle >= count) exhausted = true;
+      if (ent == kNoRay) {
+        unsigned long long below = lane == 0 ? 0ull : (idle & ((1ull << lane) - 1ull));
+        uint32_t idx = base + (uint32_t)__popcll(below);
+        if (idx < count) {
+          ent = idx;
+          uint4 e4 = A.s_ent[idx];
+          src = e4.x;
+          mask = e4.z;
+          uint32_t dst = e4.y & ~kShadowLast;
+          lp = (e4.y & kShadowLast) ? &A.sample_L[dst] : &A.q_L[qn][dst];
+          float4 l4 = *lp;
+          L = v3(l4.x, l4.y, l4.z);
+          lw = l4.w;
+          start = true;
+        }
+      }
+    }
+    if (__ballot(ent != kNoRay) == 0ull) {
+      if (exhausted) break;
+      continue;
+    }
+    if (start) {  // the entry's next emitter ray
+      start = false;
+      int e = __ffs((int)mask) - 1;
+      mask &= mask - 1u;
+      size_t slot = size_t(e) * A.n_paths + src;
+      float4 o = A.sh_o[slot], d = A.sh_d[slot], c4 = A.sh_c[slot];
+      c = v3(c4.x, c4.y, c4.z);
+      trav_begin(R, stack, A.sc.root_ref, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), 1e-4f, 1.0f - 1e-4f);
+      traced++;
+    }
+    while (R.ref >= 0) node_step(A.sc.nodes, R, tc);
+    if (R.ref != kDone) leaf_step<true>(A.sc.tris, R, tc);
+    if (ent != kNoRay && R.ref == kDone) {
+      if (R.bprim < 0) L = L + c;
+      if (mask) {
+        start = true;
+      } else {
+        *lp = make_float4(L.x, L.y, L.z, lw);
+        ent = kNoRay;
+      }
+    }
+  }
+  // one atomic per wave for the statistics
+  for (int off = 32; off > 0; off >>= 1) traced += __shfl_down(traced, off);
+  if (lane == 0 && traced) atomicAdd(&A.stats[1], (unsigned long long)traced);
+#ifdef HK_TRAVERSAL_STATS
+  count_flush(A.stats + 4, tc);
+#endif
+}
+#else
+__global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   __shared__ int2 stack[kLdsStack * kBlock];
   uint32_t count = A.counters[2];
   uint32_t stride = gridDim.x * blockDim.x;
   uint32_t traced = 0;
   TraceCount tc;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-    uint32_t path = A.s_path[i], mask = A.s_mask[i];
-    bool last = (mask & kShadowLast) != 0u;
-    mask &= ~kShadowLast;
-    float4 l4 = A.ps_L[path];
+    uint4 ent = A.s_ent[i];
+    uint32_t src = ent.x, mask = ent.z;
+    uint32_t dst = ent.y & ~kShadowLast;
+    float4 *lp = (ent.y & kShadowLast) ? &A.sample_L[dst] : &A.q_L[qn][dst];
+    float4 l4 = *lp;
     vec3 L = v3(l4.x, l4.y, l4.z);
     while (mask) {
       int e = __ffs((int)mask) - 1;
       mask &= mask - 1u;
-      size_t slot = size_t(e) * A.n_paths + path;
+      size_t slot = size_t(e) * A.n_paths + src;
       float4 o = A.sh_o[slot], d = A.sh_d[slot], c = A.sh_c[slot];
       float t, u, v;
       int32_t prim;
@@ -634,8 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A) {
                                  1e-4f, 1.0f - 1e-4f, t, u, v, prim, tc);
       if (!occ) L = L + v3(c.x, c.y, c.z);
     }
-    if (last) A.sample_L[A.ps_sid[path]] = make_float4(L.x, L.y, L.z, 0.0f);
-    else A.ps_L[path] = make_float4(L.x, L.y, L.z, l4.w);
+    *lp = make_float4(L.x, L.y, L.z, l4.w);
   }
   // one atomic per wave for the statistics
   for (int off = 32; off > 0; off >>= 1) traced += __shfl_down(traced, off);
@@ -644,6 +821,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A) {
   count_flush(A.stats + 4, tc);
 #endif
 }
+#endif
 
 // ------------------------------------------------------------------ k_resolve
 // ImageBlock::Put (src/core/Block.cpp:95-134) as a gather: each output pixel
@@ -702,10 +880,6 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_iota_u32(uint32_t *p, uint32_t n) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = i;
-}
 
 // ------------------------------------------------------------------ host launchers
 static uint32_t grid_for(uint32_t n, uint32_t cap) {
@@ -732,22 +906,32 @@ hipError_t launch_bsdf_batch(const DevMaterial *mats, const float *rft, int idx,
   return hipGetLastError();
 }
 
-hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t free_top, uint32_t n_new,
-                        uint32_t g0, hipStream_t st) {
-  hipLaunchKernelGGL(k_regen, dim3(grid_for(n_new, 1u << 16)), dim3(kBlock), 0, st, A, qi, q_base, free_top, n_new,
-                     g0);
+hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t n_new, uint32_t g0, hipStream_t st) {
+  hipLaunchKernelGGL(k_regen, dim3(grid_for(n_new, 1u << 16)), dim3(kBlock), 0, st, A, qi, q_base, n_new, g0);
   return hipGetLastError();
 }
+// Blocks of a kernel that are resident at once on the current device (grid
+// cap of the persistent kernels; plain launches are never larger anyway).
+static uint32_t resident_blocks(const void *kernel) {
+  int dev = 0, n_cu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1u << 16;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1u << 16;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess) return 1u << 16;
+  return uint32_t(std::max(1, n_cu * per_cu));
+}
+
 hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_extend, dim3(grid), dim3(kBlock), 0, st, A, qi);
+  static uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_extend));
+  hipLaunchKernelGGL(k_extend, dim3(std::min(grid, cap)), dim3(kBlock), 0, st, A, qi);
   return hipGetLastError();
 }
 hipError_t launch_shade(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st) {
   hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, st, A, qi);
   return hipGetLastError();
 }
-hipError_t launch_shadow(const RenderArgs &A, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(kBlock), 0, st, A);
+hipError_t launch_shadow(const RenderArgs &A, int qn, uint32_t grid, hipStream_t st) {
+  static uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_shadow));
+  hipLaunchKernelGGL(k_shadow, dim3(std::min(grid, cap)), dim3(kBlock), 0, st, A, qn);
   return hipGetLastError();
 }
 hipError_t launch_resolve(const RenderArgs &A, uint32_t item, uint32_t npix, hipStream_t st) {
@@ -761,11 +945,6 @@ hipError_t launch_finalize(const void *accum, float *rgb, uint32_t n, hipStream_
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
-  return hipGetLastError();
-}
-hipError_t launch_iota_u32(uint32_t *p, uint32_t n, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_iota_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
   return hipGetLastError();
 }
 

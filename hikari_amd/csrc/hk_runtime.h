@@ -46,10 +46,9 @@ constexpr int kMaxEmitters = 32;
 // Wavefront buffers, grown on demand and reused across renders of a context.
 struct Workspace {
   uint32_t cap_paths = 0, cap_emitters = 0, cap_samples = 0;
-  DBuf sample_L, free_slots;
-  DBuf ps_beta, ps_L, ps_rng, ps_pix, ps_sid, ps_meta;
-  DBuf q_path[2], q_o[2], q_d[2], q_hit[2];
-  DBuf s_path, s_mask, sh_o, sh_d, sh_c;
+  DBuf sample_L;
+  DBuf q_o[2], q_d[2], q_beta[2], q_L[2], q_aux[2], q_meta[2], hit;
+  DBuf s_ent, sh_o, sh_d, sh_c;
   DBuf counters, stats;
   DBuf reqs, items;
   void ensure(uint32_t paths, uint32_t emitters, uint32_t samples);

@@ -56,13 +56,12 @@ void Workspace::ensure(uint32_t paths, uint32_t emitters, uint32_t samples) {
   paths = std::max(paths, cap_paths);
   emitters = std::max(emitters, cap_emitters);
   size_t P = paths;
-  free_slots.alloc(P * 4);
-  ps_beta.alloc(P * 16); ps_L.alloc(P * 16); ps_rng.alloc(P * 8); ps_pix.alloc(P * 4); ps_sid.alloc(P * 4);
-  ps_meta.alloc(P * 4);
   for (int i = 0; i < 2; i++) {
-    q_path[i].alloc(P * 4); q_o[i].alloc(P * 16); q_d[i].alloc(P * 16); q_hit[i].alloc(P * 16);
+    q_o[i].alloc(P * 16); q_d[i].alloc(P * 16); q_beta[i].alloc(P * 16); q_L[i].alloc(P * 16);
+    q_aux[i].alloc(P * 16); q_meta[i].alloc(P * 4);
   }
-  s_path.alloc(P * 4); s_mask.alloc(P * 4);
+  hit.alloc(P * 16);
+  s_ent.alloc(P * 16);
   sh_o.alloc(P * emitters * 16); sh_d.alloc(P * emitters * 16); sh_c.alloc(P * emitters * 16);
   counters.alloc(64);
   stats.alloc(64);
@@ -70,9 +69,9 @@ void Workspace::ensure(uint32_t paths, uint32_t emitters, uint32_t samples) {
   cap_emitters = emitters;
 }
 void Workspace::release() {
-  DBuf *all[] = {&sample_L, &free_slots, &ps_beta, &ps_L, &ps_rng, &ps_pix, &ps_sid, &ps_meta, &q_path[0],
-                 &q_path[1], &q_o[0], &q_o[1], &q_d[0], &q_d[1], &q_hit[0], &q_hit[1], &s_path, &s_mask, &sh_o,
-                 &sh_d, &sh_c, &counters, &stats, &reqs, &items};
+  DBuf *all[] = {&sample_L, &q_o[0], &q_o[1], &q_d[0], &q_d[1], &q_beta[0], &q_beta[1], &q_L[0], &q_L[1],
+                 &q_aux[0], &q_aux[1], &q_meta[0], &q_meta[1], &hit, &s_ent, &sh_o, &sh_d, &sh_c, &counters,
+                 &stats, &reqs, &items};
   for (DBuf *b : all) b->release();
   cap_paths = cap_emitters = cap_samples = 0;
 }
@@ -89,22 +88,17 @@ static RenderArgs make_args(hk_scene *s) {
   A.sc = s->dev;
   A.reqs = ws.reqs.as<DevRequest>();
   A.items = ws.items.as<WorkItem>();
-  A.ps_beta = ws.ps_beta.as<float4>();
-  A.ps_L = ws.ps_L.as<float4>();
-  A.ps_rng = ws.ps_rng.as<uint64_t>();
-  A.ps_pix = ws.ps_pix.as<uint32_t>();
-  A.ps_sid = ws.ps_sid.as<uint32_t>();
   A.sample_L = ws.sample_L.as<float4>();
-  A.free_slots = ws.free_slots.as<uint32_t>();
-  A.ps_meta = ws.ps_meta.as<uint32_t>();
   for (int i = 0; i < 2; i++) {
-    A.q_path[i] = ws.q_path[i].as<uint32_t>();
     A.q_o[i] = ws.q_o[i].as<float4>();
     A.q_d[i] = ws.q_d[i].as<float4>();
-    A.q_hit[i] = ws.q_hit[i].as<float4>();
+    A.q_beta[i] = ws.q_beta[i].as<float4>();
+    A.q_L[i] = ws.q_L[i].as<float4>();
+    A.q_aux[i] = ws.q_aux[i].as<uint4>();
+    A.q_meta[i] = ws.q_meta[i].as<uint32_t>();
   }
-  A.s_path = ws.s_path.as<uint32_t>();
-  A.s_mask = ws.s_mask.as<uint32_t>();
+  A.hit = ws.hit.as<float4>();
+  A.s_ent = ws.s_ent.as<uint4>();
   A.sh_o = ws.sh_o.as<float4>();
   A.sh_d = ws.sh_d.as<float4>();
   A.sh_c = ws.sh_c.as<float4>();
@@ -128,8 +122,9 @@ static uint32_t kernel_grid(hk_ctx *ctx, uint32_t count) {
 // Renders the given requests.  Their samples are cut into fixed chunks
 // (`chunk` samples per pixel, set by the sample capacity alone) and packed
 // into segments of at most cap_samples samples; each segment runs the
-// bounce loop with `pool` recycled path slots until all its samples have
-// finished, then resolves its chunks into the films in order.  Neither the
+// bounce loop with up to `pool` paths in flight (finished paths are replaced
+// by new samples at every bounce) until all its samples have finished, then
+// resolves its chunks into the films in order.  Neither the
 // chunks nor a sample's path depend on the other requests, so every request
 // accumulates the same values in the same order alone or in a batch.
 static void run_passes(hk_scene *s, std::vector<ReqState> &rs, hk_render_stats *stats, uint32_t pool_paths,
@@ -183,14 +178,12 @@ static void run_passes(hk_scene *s, std::vector<ReqState> &rs, hk_render_stats *
     A.n_items = uint32_t(items.size());
     A.n_samples = used;
     paths_total += used;
-    hip_check(launch_iota_u32(A.free_slots, pool, st), "free slots");
-    uint32_t count = 0, free_top = pool, next = 0;
+    uint32_t count = 0, next = 0;
     int qi = 0;
     for (;;) {
-      uint32_t n_new = std::min(free_top, used - next);
+      uint32_t n_new = std::min(pool - count, used - next);
       if (n_new) {
-        hip_check(launch_regen(A, qi, count, free_top, n_new, next, st), "k_regen");
-        free_top -= n_new;
+        hip_check(launch_regen(A, qi, count, n_new, next, st), "k_regen");
         next += n_new;
         count += n_new;
       }
@@ -209,11 +202,10 @@ static void run_passes(hk_scene *s, std::vector<ReqState> &rs, hk_render_stats *
         trace_launches++;
       }
       uint32_t nshadow = ctx->host_counters[2];
-      if (nshadow) hip_check(launch_shadow(A, kernel_grid(ctx, nshadow), st), "k_shadow");
+      if (nshadow) hip_check(launch_shadow(A, qi ^ 1, kernel_grid(ctx, nshadow), st), "k_shadow");
       launches++;
       qi ^= 1;
       count = ctx->host_counters[qi];
-      free_top = ctx->host_counters[3];
     }
     if (no_resolve) {  // caller reads the per-sample results of this segment
       hip_check(hipStreamSynchronize(st), "segment sync");

@@ -172,13 +172,13 @@ int hk_render_pixel_samples(hk_scene *scene, uint32_t seed, uint32_t x, uint32_t
 /* Batched renders: n requests over one resident scene share one wavefront
  * (paths tagged by request).  out_rgb[i]: host buffer of request i
  * (height x width x 3).  Results equal the requests rendered alone, bit for
- * bit.  max_paths: path slots of the wavefront (0: the context's). */
+ * bit.  max_paths: paths in flight (0: the context's). */
 int hk_render_batch(hk_scene *scene, const hk_request *reqs, uint32_t n, float **out_rgb, uint32_t max_paths,
                     hk_render_stats *stats);
 
 /* Tuning knobs (value 0 restores the default):
- *   "max_paths"   path slots of the wavefront (default 4M); finished paths
- *                 hand their slot to the next sample at the next bounce
+ *   "max_paths"   paths in flight (default 4M); at every bounce finished
+ *                 paths are replaced by the next samples
  *   "max_samples" samples per segment (default 256M, 16 B of HBM each): a
  *                 request's samples are cut into chunks of
  *                 floor(max_samples / pixels) samples per pixel, resolved
