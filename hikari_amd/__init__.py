@@ -23,7 +23,8 @@ ERRORS = {-1: "HK_ERR_INVALID", -2: "HK_ERR_HIP", -3: "HK_ERR_OOM", -4: "HK_ERR_
 EXPORTS = ["hk_ctx_create", "hk_ctx_destroy", "hk_last_error", "hk_version", "hk_ctx_sync", "hk_accel_build",
            "hk_accel_destroy", "hk_accel_get_info", "hk_accel_intersect", "hk_accel_intersect_device",
            "hk_bsdf_batch", "hk_scene_create", "hk_scene_destroy", "hk_scene_device_bytes", "hk_render",
-           "hk_render_pixel_samples", "hk_render_batch", "hk_set_option"]
+           "hk_render_pixel_samples", "hk_render_batch", "hk_set_option", "hk_server_create", "hk_server_add_scene",
+           "hk_server_submit", "hk_server_wait", "hk_server_get_stats", "hk_server_destroy"]
 
 
 class HkError(RuntimeError):
@@ -56,6 +57,15 @@ class RenderStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("queue_ms", C.c_double), ("latency_ms", C.c_double), ("paths", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("bounces", C.c_uint64), ("paths", C.c_uint64),
+                ("fixed_bytes", C.c_uint64), ("film_bytes", C.c_uint64)]
 
 
 _lib = None
@@ -91,6 +101,12 @@ def lib():
         L.hk_render_pixel_samples.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P]
         L.hk_render_batch.argtypes = [P, C.POINTER(Request), C.c_uint32, C.POINTER(C.POINTER(C.c_float)),
                                       C.c_uint32, C.POINTER(RenderStats)]
+        L.hk_server_create.argtypes = [P, C.c_uint64, C.POINTER(P)]
+        L.hk_server_add_scene.argtypes = [P, P, C.POINTER(C.c_int32)]
+        L.hk_server_submit.argtypes = [P, C.c_int32, C.POINTER(Request), P, C.POINTER(C.c_uint64)]
+        L.hk_server_wait.argtypes = [P, C.c_uint64, C.POINTER(ServerResult)]
+        L.hk_server_get_stats.argtypes = [P, C.POINTER(ServerStats)]
+        L.hk_server_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -244,5 +260,55 @@ class GpuScene:
         try:
             if self.h:
                 lib().hk_scene_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop over resident scenes (hk_server)."""
+
+    def __init__(self, ctx, hbm_budget=0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        self._scenes = []
+        self._pending = {}
+        ctx.check(lib().hk_server_create(ctx.h, int(hbm_budget), C.byref(self.h)))
+
+    def add_scene(self, gpu_scene):
+        sid = C.c_int32()
+        self.ctx.check(lib().hk_server_add_scene(self.h, gpu_scene.h, C.byref(sid)))
+        self._scenes.append(gpu_scene)
+        return sid.value
+
+    def submit(self, scene_id, request):
+        gs = self._scenes[scene_id]
+        W, H = (gs.scene.width, gs.scene.height) if request.use_scene_camera else (request.width, request.height)
+        img = np.zeros((H, W, 3), np.float32)
+        t = C.c_uint64()
+        self.ctx.check(lib().hk_server_submit(self.h, scene_id, C.byref(request), _p(img), C.byref(t)))
+        self._pending[t.value] = img
+        return t.value
+
+    def wait(self, ticket):
+        """-> (image, {"status", "queue_ms", "latency_ms", "paths"})"""
+        res = ServerResult()
+        rc = lib().hk_server_wait(self.h, ticket, C.byref(res))
+        img = self._pending.pop(ticket, None)
+        self.ctx.check(rc)
+        return img, {k: getattr(res, k) for k, _ in ServerResult._fields_}
+
+    def stats(self):
+        st = ServerStats()
+        self.ctx.check(lib().hk_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().hk_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

@@ -6,43 +6,45 @@
 
 namespace hk {
 
-// Everything one wavefront segment touches (passed by value to every kernel).
-// A segment is a list of work items (fixed per-request sample chunks) whose
-// samples are numbered 0..n_samples-1.  Path state lives in the extension
-// queues (SoA, ping-pong): k_shade moves a continuing path from queue qi to
-// the compacted queue qi^1, a finished path writes its radiance to
-// sample_L[sample id], and k_regen appends new samples' camera paths after
-// the continuing ones, so every bounce launch stays full (up to n_paths)
-// until the segment's samples run out.
+// Everything the bounce kernels touch (passed by value to every kernel).
+// Work arrives as chunks (WorkItem, one slot each); each bounce k_regen
+// starts the samples listed in `plan` behind the continuing paths of the
+// extension queue.  Path state lives in the extension queues (SoA,
+// ping-pong): k_shade moves a continuing path from queue qi to the compacted
+// queue qi^1; a finished path writes its radiance to the sample buffer and
+// counts itself in item_done[slot], so the host knows when a chunk is
+// complete and can be resolved into its film.
 struct RenderArgs {
   DevScene sc;
-  const DevRequest *reqs;
-  const WorkItem *items;
-  uint32_t n_items;
-  uint32_t n_paths;    // queue capacity (paths in flight)
-  uint32_t n_samples;  // samples of the segment
-  float4 *sample_L;    // final radiance per sample (rgb, unused)
+  const DevRequest *reqs;    // request table (by request slot)
+  const WorkItem *items;     // item table (by item slot)
+  const RegenEntry *plan;    // this bounce's new samples
+  uint32_t n_plan;
+  uint32_t n_paths;          // queue capacity (paths in flight)
+  uint32_t *item_done;       // finished samples per item slot
+  float4 *sample_L;          // final radiance per sample (rgb, unused)
   // per queue entry
   float4 *q_o[2];      // ray origin, tmin
   float4 *q_d[2];      // ray direction, tmax
   float4 *q_beta[2];   // throughput rgb, pdf of the last BSDF sample
   float4 *q_L[2];      // radiance rgb, MIS weight of BSDF sampling (W_MATS)
-  uint4 *q_aux[2];     // pcg32 state (lo, hi), pixel index, sample id
-  uint32_t *q_meta[2]; // depth | flags << 16
+  uint4 *q_aux[2];     // pcg32 state (lo, hi), pixel index, local sample index
+  uint32_t *q_meta[2]; // depth | flags << 16 | item slot << 20
   float4 *hit;         // closest hit of the current queue: t, u, v, prim bits
-  // shadow queue: {source queue index, target (next-queue index, or sample id
-  // | kShadowLast), emitter mask}; per (emitter, source index) rays
+  // shadow queue: {source queue index, target (next-queue index, or sample
+  // buffer index), emitter mask, item slot | kShadowLast for a finished path};
+  // per (emitter, source index) rays
   uint4 *s_ent;
   float4 *sh_o, *sh_d, *sh_c;
-  uint32_t *counters;            // [0],[1] extension queues, [2] shadow queue
+  uint32_t *counters;            // [0],[1] extension queues, [2] shadow queue, [4],[5] fetch counters
   unsigned long long *stats;     // [0] extension rays, [1] shadow rays, [2..5] traversal counters
 };
 
-hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t n_new, uint32_t g0, hipStream_t st);
+hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t n_new, hipStream_t st);
 hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st);
 hipError_t launch_shade(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st);
 hipError_t launch_shadow(const RenderArgs &A, int qn, uint32_t grid, hipStream_t st);
-hipError_t launch_resolve(const RenderArgs &A, uint32_t item, uint32_t npix, hipStream_t st);
+hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, uint32_t npix, hipStream_t st);
 hipError_t launch_finalize(const void *accum, float *rgb, uint32_t n, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st);
 

@@ -288,15 +288,31 @@ __device__ __forceinline__ void count_flush(unsigned long long *stats, TraceCoun
   }
 }
 
-__device__ __forceinline__ uint32_t find_item(const WorkItem *items, uint32_t n, uint32_t id) {
-  uint32_t lo = 0, hi = n;  // largest i with items[i].sample_begin <= id
+// Plan entry holding launch index j: the last entry with prefix <= j.
+__device__ __forceinline__ uint32_t find_plan(const RegenEntry *plan, uint32_t n, uint32_t j) {
+  uint32_t lo = 0, hi = n;
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
-    if (items[mid].sample_begin <= id) lo = mid;
+    if (plan[mid].prefix <= j) lo = mid;
     else hi = mid;
   }
   return lo;
 }
+
+// Adds one to item_done[slot] for every lane with pred set: one atomic per
+// distinct slot in the wave.  Every lane of the wave must call it.
+__device__ __forceinline__ void count_done(uint32_t *item_done, bool pred, uint32_t slot) {
+  unsigned long long m = __ballot(pred);
+  uint32_t lane = lane_id();
+  while (m) {
+    int leader = __ffsll((long long)m) - 1;
+    uint32_t ls = __shfl(slot, leader);
+    unsigned long long same = __ballot(pred && slot == ls);
+    if ((int)lane == leader) atomicAdd(&item_done[ls], (uint32_t)__popcll(same));
+    m &= ~same;
+  }
+}
+
 
 __device__ __forceinline__ vec3 mat3_mul(const float *m, vec3 v) {  // row-major 3x3
   return v3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
@@ -433,26 +449,27 @@ __device__ __forceinline__ void post_intersect(const DevScene &sc, int32_t prim,
 }
 
 constexpr uint32_t kFlagPrevDiscrete = 1u;
-constexpr uint32_t kShadowLast = 0x80000000u;  // shadow-queue mask bit: the path ended at this bounce
+constexpr uint32_t kShadowLast = 0x80000000u;  // shadow entry flag: the path ended at this bounce
 
 // ------------------------------------------------------------------ k_regen
-// Starts samples g0 .. g0 + n_new - 1 of the segment: their camera paths go
-// to queue qi after the q_base continuing paths already there.  Per sample:
+// Starts the n_new samples of this bounce's plan: their camera paths go to
+// queue qi after the q_base continuing paths already there.  Per sample:
 // RenderBlock's prologue (src/Main.cpp:22-31) + PerspectiveCamera::SampleRay
 // (src/camera/PerspectiveCamera.cpp:77-102).  A sample's path depends only on
 // (request, pixel, sample index), never on its queue position.
-__global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t q_base, uint32_t n_new,
-                                                  uint32_t g0) {
+constexpr uint32_t kSlotShift = 20;  // q_meta: depth (16 bits) | flags (4) | item slot (12)
+
+__global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t q_base, uint32_t n_new) {
   uint32_t stride = gridDim.x * blockDim.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     A.counters[qi] = q_base + n_new;
-    A.counters[4] = 0u;  // k_extend's fetch counter
+    A.counters[4] = 0u;  // k_extend's fetch counter (persistent build)
   }
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_new; j += stride) {
-    uint32_t g = g0 + j;
-    const WorkItem it = A.items[find_item(A.items, A.n_items, g)];
+    const RegenEntry e = A.plan[find_plan(A.plan, A.n_plan, j)];
+    const WorkItem it = A.items[e.slot];
     const DevRequest &rq = A.reqs[it.req];
-    uint32_t local = g - it.sample_begin;
+    uint32_t local = e.local0 + (j - e.prefix);
     uint32_t q = rq.pix0 + local % it.npix, s = it.s0 + local / it.npix;
     uint32_t x = q % (uint32_t)rq.width, y = q / (uint32_t)rq.width;
     Rng rng = rng_for_path(rq.seed, q, s);
@@ -475,8 +492,8 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t
     A.q_d[qi][pos] = make_float4(d.x, d.y, d.z, rq.far_clip * inv_z);
     A.q_beta[qi][pos] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     A.q_L[qi][pos] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    A.q_aux[qi][pos] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), q, g);
-    A.q_meta[qi][pos] = 0u;
+    A.q_aux[qi][pos] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), q, local);
+    A.q_meta[qi][pos] = e.slot << kSlotShift;
   }
 }
 
@@ -587,7 +604,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
     uint32_t i = base + threadIdx.x;
     bool active = i < count;
     bool cont = false;
-    uint32_t smask = 0, pix = 0, sid = 0, meta_out = 0;
+    uint32_t smask = 0, pix = 0, local = 0, islot = 0, meta_out = 0;
     vec3 P = v3(0.0f), nd = v3(0.0f), L = v3(0.0f), beta = v3(0.0f);
     float pdf_b = 0.0f, w_mats = 0.0f;
     Rng rng;
@@ -597,10 +614,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
       uint4 aux = A.q_aux[qi][i];
       uint32_t meta = A.q_meta[qi][i];
       pix = aux.z;
-      sid = aux.w;
+      local = aux.w;
+      islot = meta >> kSlotShift;
       rng.state = (uint64_t)aux.x | ((uint64_t)aux.y << 32);
       rng.inc = rng_inc(pix);
-      uint32_t depth = meta & 0xffffu, flags = meta >> 16;
+      uint32_t depth = meta & 0xffffu, flags = (meta >> 16) & 0xfu;
       beta = v3(b4.x, b4.y, b4.z);
       L = v3(l4.x, l4.y, l4.z);
       pdf_b = b4.w;
@@ -685,7 +703,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
           }
         }
       }
-      meta_out = (depth & 0xffffu) | (flags << 16);
+      meta_out = (depth & 0xffffu) | (flags << 16) | (islot << kSlotShift);
     }
     uint32_t slot = wave_append(&A.counters[qn], cont);
     if (cont) {  // the path moves to the compacted next queue
@@ -693,15 +711,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
       A.q_d[qn][slot] = make_float4(nd.x, nd.y, nd.z, kInfF);
       A.q_beta[qn][slot] = make_float4(beta.x, beta.y, beta.z, pdf_b);
       A.q_L[qn][slot] = make_float4(L.x, L.y, L.z, w_mats);
-      A.q_aux[qn][slot] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), pix, sid);
+      A.q_aux[qn][slot] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), pix, local);
       A.q_meta[qn][slot] = meta_out;
-    } else if (active) {  // finished: radiance so far to the sample buffer
-      A.sample_L[sid] = make_float4(L.x, L.y, L.z, 0.0f);
     }
-    // pending emitter samples of this bounce: k_shadow adds the unoccluded
-    // ones to the path's radiance where it now lives
+    // A finished path writes its radiance to the sample buffer: here, or in
+    // k_shadow when emitter samples of this bounce are still pending (it is
+    // counted as done where that final write happens).
+    bool fin = active && !cont;
+    uint32_t sbuf = A.items[islot].buf_base + local;
+    if (fin) A.sample_L[sbuf] = make_float4(L.x, L.y, L.z, 0.0f);
+    count_done(A.item_done, fin && smask == 0u, islot);
     uint32_t ss = wave_append(&A.counters[2], smask != 0u);
-    if (smask != 0u) A.s_ent[ss] = make_uint4(i, cont ? slot : (sid | kShadowLast), smask, 0u);
+    if (smask != 0u)
+      A.s_ent[ss] = cont ? make_uint4(i, slot, smask, 0u) : make_uint4(i, sbuf, smask, islot | kShadowLast);
   }
 }
 
@@ -725,7 +747,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   float4 *lp = nullptr;
   vec3 L = v3(0.0f), c = v3(0.0f);
   float lw = 0.0f;
-  bool exhausted = false, start = false;
+  bool exhausted = false, start = false, last = false;
+  uint32_t islot = 0;
   uint32_t lane = lane_id();
   for (;;) {
     unsigned long long idle = __ballot(ent == kNoRay);
@@ -744,8 +767,9 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
           uint4 e4 = A.s_ent[idx];
           src = e4.x;
           mask = e4.z;
-          uint32_t dst = e4.y & ~kShadowLast;
-          lp = (e4.y & kShadowLast) ? &A.sample_L[dst] : &A.q_L[qn][dst];
+          last = (e4.w & kShadowLast) != 0u;
+          islot = e4.w & ~kShadowLast;
+          lp = last ? &A.sample_L[e4.y] : &A.q_L[qn][e4.y];
           float4 l4 = *lp;
           L = v3(l4.x, l4.y, l4.z);
           lw = l4.w;
@@ -769,6 +793,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
     }
     while (R.ref >= 0) node_step(A.sc.nodes, R, tc);
     if (R.ref != kDone) leaf_step<true>(A.sc.tris, R, tc);
+    bool fin = false;
     if (ent != kNoRay && R.ref == kDone) {
       if (R.bprim < 0) L = L + c;
       if (mask) {
@@ -776,8 +801,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
       } else {
         *lp = make_float4(L.x, L.y, L.z, lw);
         ent = kNoRay;
+        fin = last;
       }
     }
+    count_done(A.item_done, fin, islot);
   }
   // one atomic per wave for the statistics
   for (int off = 32; off > 0; off >>= 1) traced += __shfl_down(traced, off);
@@ -793,26 +820,33 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   uint32_t stride = gridDim.x * blockDim.x;
   uint32_t traced = 0;
   TraceCount tc;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-    uint4 ent = A.s_ent[i];
-    uint32_t src = ent.x, mask = ent.z;
-    uint32_t dst = ent.y & ~kShadowLast;
-    float4 *lp = (ent.y & kShadowLast) ? &A.sample_L[dst] : &A.q_L[qn][dst];
-    float4 l4 = *lp;
-    vec3 L = v3(l4.x, l4.y, l4.z);
-    while (mask) {
-      int e = __ffs((int)mask) - 1;
-      mask &= mask - 1u;
-      size_t slot = size_t(e) * A.n_paths + src;
-      float4 o = A.sh_o[slot], d = A.sh_d[slot], c = A.sh_c[slot];
-      float t, u, v;
-      int32_t prim;
-      traced++;
-      bool occ = trace_ray<true>(A.sc.nodes, A.sc.tris, A.sc.root_ref, stack, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z),
-                                 1e-4f, 1.0f - 1e-4f, t, u, v, prim, tc);
-      if (!occ) L = L + v3(c.x, c.y, c.z);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    bool last = false;
+    uint32_t islot = 0;
+    if (i < count) {
+      uint4 ent = A.s_ent[i];
+      uint32_t src = ent.x, mask = ent.z;
+      last = (ent.w & kShadowLast) != 0u;
+      islot = ent.w & ~kShadowLast;
+      float4 *lp = last ? &A.sample_L[ent.y] : &A.q_L[qn][ent.y];
+      float4 l4 = *lp;
+      vec3 L = v3(l4.x, l4.y, l4.z);
+      while (mask) {
+        int e = __ffs((int)mask) - 1;
+        mask &= mask - 1u;
+        size_t slot = size_t(e) * A.n_paths + src;
+        float4 o = A.sh_o[slot], d = A.sh_d[slot], c = A.sh_c[slot];
+        float t, u, v;
+        int32_t prim;
+        traced++;
+        bool occ = trace_ray<true>(A.sc.nodes, A.sc.tris, A.sc.root_ref, stack, v3(o.x, o.y, o.z),
+                                   v3(d.x, d.y, d.z), 1e-4f, 1.0f - 1e-4f, t, u, v, prim, tc);
+        if (!occ) L = L + v3(c.x, c.y, c.z);
+      }
+      *lp = make_float4(L.x, L.y, L.z, l4.w);
     }
-    *lp = make_float4(L.x, L.y, L.z, l4.w);
+    count_done(A.item_done, last, islot);
   }
   // one atomic per wave for the statistics
   for (int off = 32; off > 0; off >>= 1) traced += __shfl_down(traced, off);
@@ -826,8 +860,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
 // ------------------------------------------------------------------ k_resolve
 // ImageBlock::Put (src/core/Block.cpp:95-134) as a gather: each output pixel
 // owns its accumulator and sums the filtered samples of its neighbours.
-__global__ __launch_bounds__(kBlock) void k_resolve(RenderArgs A, uint32_t item_idx) {
-  const WorkItem it = A.items[item_idx];
+__global__ __launch_bounds__(kBlock) void k_resolve(RenderArgs A, uint32_t slot) {
+  const WorkItem it = A.items[slot];
   const DevRequest &rq = A.reqs[it.req];
   int W = rq.width, H = rq.height;
   float R = rq.filter_radius;
@@ -847,7 +881,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(RenderArgs A, uint32_t item_
           float cx = ((float)qx + jx) - 0.5f, cy = ((float)qy + jy) - 0.5f;
           if (x < (int)ceilf(cx - R) || x > (int)floorf(cx + R) || y < (int)ceilf(cy - R) || y > (int)floorf(cy + R))
             continue;
-          float4 l4 = A.sample_L[it.sample_begin + s * it.npix + q];
+          float4 l4 = A.sample_L[it.buf_base + s * it.npix + q];
           vec3 L = v3(l4.x, l4.y, l4.z);
           if (!color_valid(L)) continue;  // invalid samples are dropped (Block.cpp:97-101)
           float wx = rq.filter[(int)(fabsf((float)x - cx) * rq.lookup)];
@@ -906,8 +940,8 @@ hipError_t launch_bsdf_batch(const DevMaterial *mats, const float *rft, int idx,
   return hipGetLastError();
 }
 
-hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t n_new, uint32_t g0, hipStream_t st) {
-  hipLaunchKernelGGL(k_regen, dim3(grid_for(n_new, 1u << 16)), dim3(kBlock), 0, st, A, qi, q_base, n_new, g0);
+hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t n_new, hipStream_t st) {
+  hipLaunchKernelGGL(k_regen, dim3(grid_for(n_new, 1u << 16)), dim3(kBlock), 0, st, A, qi, q_base, n_new);
   return hipGetLastError();
 }
 // Blocks of a kernel that are resident at once on the current device (grid
@@ -934,8 +968,8 @@ hipError_t launch_shadow(const RenderArgs &A, int qn, uint32_t grid, hipStream_t
   hipLaunchKernelGGL(k_shadow, dim3(std::min(grid, cap)), dim3(kBlock), 0, st, A, qn);
   return hipGetLastError();
 }
-hipError_t launch_resolve(const RenderArgs &A, uint32_t item, uint32_t npix, hipStream_t st) {
-  hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, 1u << 16)), dim3(kBlock), 0, st, A, item);
+hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, uint32_t npix, hipStream_t st) {
+  hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, 1u << 16)), dim3(kBlock), 0, st, A, slot);
   return hipGetLastError();
 }
 hipError_t launch_finalize(const void *accum, float *rgb, uint32_t n, hipStream_t st) {

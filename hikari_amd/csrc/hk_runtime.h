@@ -43,15 +43,31 @@ constexpr uint32_t kDefaultMaxPaths = 1u << 22;     // path slots of the wavefro
 constexpr uint32_t kDefaultMaxSamples = 1u << 28;   // samples per segment (16 B each in HBM)
 constexpr int kMaxEmitters = 32;
 
-// Wavefront buffers, grown on demand and reused across renders of a context.
+// Owned pinned host allocation (staging for small per-bounce copies).
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() {}
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  ~PinnedBuf() { release(); }
+  void alloc(size_t n);
+  void release();
+  template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+// Wavefront buffers of one engine, grown on demand and reused across renders.
 struct Workspace {
   uint32_t cap_paths = 0, cap_emitters = 0, cap_samples = 0;
+  bool tables = false;
   DBuf sample_L;
   DBuf q_o[2], q_d[2], q_beta[2], q_L[2], q_aux[2], q_meta[2], hit;
   DBuf s_ent, sh_o, sh_d, sh_c;
   DBuf counters, stats;
-  DBuf reqs, items;
+  DBuf reqs, items, plan, item_done;               // request / item tables, regen plan, per-item counts
+  PinnedBuf h_reqs, h_items, h_plan, h_done, h_counters, h_zero;  // host mirrors / staging
   void ensure(uint32_t paths, uint32_t emitters, uint32_t samples);
+  uint64_t device_bytes() const;
   void release();
 };
 

@@ -13,6 +13,7 @@
 
 #include "../../include/hikari_amd.h"
 #include "hk_build.h"
+#include "hk_engine.h"
 #include "hk_kernels.h"
 #include "hk_math.h"
 #include "hk_runtime.h"
@@ -46,225 +47,9 @@ void upload_bvh(hk_ctx *ctx, const float *tri9, uint32_t nt, DevBvh &out) {
   hip_check(hipStreamSynchronize(ctx->stream), "BVH upload");
 }
 
-void Workspace::ensure(uint32_t paths, uint32_t emitters, uint32_t samples) {
-  emitters = std::max(1u, emitters);
-  if (samples > cap_samples) {
-    sample_L.alloc(size_t(samples) * 16);
-    cap_samples = samples;
-  }
-  if (paths <= cap_paths && emitters <= cap_emitters) return;
-  paths = std::max(paths, cap_paths);
-  emitters = std::max(emitters, cap_emitters);
-  size_t P = paths;
-  for (int i = 0; i < 2; i++) {
-    q_o[i].alloc(P * 16); q_d[i].alloc(P * 16); q_beta[i].alloc(P * 16); q_L[i].alloc(P * 16);
-    q_aux[i].alloc(P * 16); q_meta[i].alloc(P * 4);
-  }
-  hit.alloc(P * 16);
-  s_ent.alloc(P * 16);
-  sh_o.alloc(P * emitters * 16); sh_d.alloc(P * emitters * 16); sh_c.alloc(P * emitters * 16);
-  counters.alloc(64);
-  stats.alloc(64);
-  cap_paths = paths;
-  cap_emitters = emitters;
-}
-void Workspace::release() {
-  DBuf *all[] = {&sample_L, &q_o[0], &q_o[1], &q_d[0], &q_d[1], &q_beta[0], &q_beta[1], &q_L[0], &q_L[1],
-                 &q_aux[0], &q_aux[1], &q_meta[0], &q_meta[1], &hit, &s_ent, &sh_o, &sh_d, &sh_c, &counters,
-                 &stats, &reqs, &items};
-  for (DBuf *b : all) b->release();
-  cap_paths = cap_emitters = cap_samples = 0;
-}
-
 static float bbox_radius(const float *lo, const float *hi) {
   float e[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
   return std::sqrt(e[0] * e[0] + (e[1] * e[1] + e[2] * e[2])) * 0.5f;  // BoundingBox::GetRadius
-}
-
-static RenderArgs make_args(hk_scene *s) {
-  Workspace &ws = s->ctx->ws;
-  RenderArgs A;
-  std::memset(&A, 0, sizeof A);
-  A.sc = s->dev;
-  A.reqs = ws.reqs.as<DevRequest>();
-  A.items = ws.items.as<WorkItem>();
-  A.sample_L = ws.sample_L.as<float4>();
-  for (int i = 0; i < 2; i++) {
-    A.q_o[i] = ws.q_o[i].as<float4>();
-    A.q_d[i] = ws.q_d[i].as<float4>();
-    A.q_beta[i] = ws.q_beta[i].as<float4>();
-    A.q_L[i] = ws.q_L[i].as<float4>();
-    A.q_aux[i] = ws.q_aux[i].as<uint4>();
-    A.q_meta[i] = ws.q_meta[i].as<uint32_t>();
-  }
-  A.hit = ws.hit.as<float4>();
-  A.s_ent = ws.s_ent.as<uint4>();
-  A.sh_o = ws.sh_o.as<float4>();
-  A.sh_d = ws.sh_d.as<float4>();
-  A.sh_c = ws.sh_c.as<float4>();
-  A.counters = ws.counters.as<uint32_t>();
-  A.stats = ws.stats.as<unsigned long long>();
-  return A;
-}
-
-struct ReqState {
-  DevRequest dev;
-  uint32_t npix, spp, spp_offset, chunk, done;
-  DBuf accum;
-};
-
-static uint32_t kernel_grid(hk_ctx *ctx, uint32_t count) {
-  uint32_t g = (count + 255) / 256;
-  if (ctx->grid) g = std::min(g, ctx->grid);
-  return std::max(1u, g);
-}
-
-// Renders the given requests.  Their samples are cut into fixed chunks
-// (`chunk` samples per pixel, set by the sample capacity alone) and packed
-// into segments of at most cap_samples samples; each segment runs the
-// bounce loop with up to `pool` paths in flight (finished paths are replaced
-// by new samples at every bounce) until all its samples have finished, then
-// resolves its chunks into the films in order.  Neither the
-// chunks nor a sample's path depend on the other requests, so every request
-// accumulates the same values in the same order alone or in a batch.
-static void run_passes(hk_scene *s, std::vector<ReqState> &rs, hk_render_stats *stats, uint32_t pool_paths,
-                       uint32_t cap_samples, bool no_resolve = false) {
-  hk_ctx *ctx = s->ctx;
-  hipStream_t st = ctx->stream;
-  Workspace &ws = ctx->ws;
-  std::vector<DevRequest> dreq(rs.size());
-  for (size_t i = 0; i < rs.size(); i++) dreq[i] = rs[i].dev;
-  uint64_t total = 0;
-  for (const ReqState &q : rs) total += uint64_t(q.npix) * (q.spp - q.done);
-  uint32_t seg_cap = uint32_t(std::min<uint64_t>(cap_samples, std::max<uint64_t>(total, 1)));
-  uint32_t pool = std::max(1u, std::min(pool_paths, seg_cap));
-  ws.ensure(pool, uint32_t(s->dev.n_emitters), seg_cap);
-  ws.reqs.upload(dreq.data(), dreq.size() * sizeof(DevRequest), st);
-  size_t max_items = 1;
-  for (const ReqState &q : rs) max_items += (q.spp - q.done + q.chunk - 1) / q.chunk;
-  ws.items.alloc(sizeof(WorkItem) * max_items);
-  hip_check(hipMemsetAsync(ws.stats.p, 0, 64, st), "memset stats");
-  RenderArgs A = make_args(s);
-  A.n_paths = pool;
-  double trace_ms = 0;
-  uint64_t launches = 0, paths_total = 0, trace_launches = 0;
-  hip_check(hipEventRecord(ctx->ev2, st), "event");
-  std::vector<WorkItem> items;
-  for (;;) {
-    items.clear();
-    uint32_t used = 0;
-    for (uint32_t r = 0; r < rs.size(); r++) {
-      ReqState &q = rs[r];
-      while (q.done < q.spp) {
-        uint32_t k = std::min(q.chunk, q.spp - q.done);
-        uint64_t need = uint64_t(q.npix) * k;
-        if (used + need > seg_cap) break;
-        WorkItem it;
-        std::memset(&it, 0, sizeof it);
-        it.req = r;
-        it.sample_begin = used;
-        it.npix = q.npix;
-        it.s0 = q.spp_offset + q.done;
-        it.k = k;
-        items.push_back(it);
-        used += uint32_t(need);
-        q.done += k;
-      }
-      if (used >= seg_cap) break;
-    }
-    if (items.empty()) break;
-    hip_check(hipMemcpyAsync(ws.items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st),
-              "items upload");
-    A.n_items = uint32_t(items.size());
-    A.n_samples = used;
-    paths_total += used;
-    uint32_t count = 0, next = 0;
-    int qi = 0;
-    for (;;) {
-      uint32_t n_new = std::min(pool - count, used - next);
-      if (n_new) {
-        hip_check(launch_regen(A, qi, count, n_new, next, st), "k_regen");
-        next += n_new;
-        count += n_new;
-      }
-      if (count == 0) break;
-      uint32_t g = kernel_grid(ctx, count);
-      if (ctx->profile) hip_check(hipEventRecord(ctx->ev0, st), "event");
-      hip_check(launch_extend(A, qi, g, st), "k_extend");
-      if (ctx->profile) hip_check(hipEventRecord(ctx->ev1, st), "event");
-      hip_check(launch_shade(A, qi, g, st), "k_shade");
-      hip_check(hipMemcpyAsync(ctx->host_counters, A.counters, 16, hipMemcpyDeviceToHost, st), "counters D2H");
-      hip_check(hipStreamSynchronize(st), "bounce sync");
-      if (ctx->profile) {
-        float ms = 0;
-        hip_check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
-        trace_ms += ms;
-        trace_launches++;
-      }
-      uint32_t nshadow = ctx->host_counters[2];
-      if (nshadow) hip_check(launch_shadow(A, qi ^ 1, kernel_grid(ctx, nshadow), st), "k_shadow");
-      launches++;
-      qi ^= 1;
-      count = ctx->host_counters[qi];
-    }
-    if (no_resolve) {  // caller reads the per-sample results of this segment
-      hip_check(hipStreamSynchronize(st), "segment sync");
-      break;
-    }
-    for (uint32_t i = 0; i < items.size(); i++)
-      hip_check(launch_resolve(A, i, items[i].npix, st), "k_resolve");
-  }
-  hip_check(hipEventRecord(ctx->ev3, st), "event");
-  unsigned long long hs[6] = {0, 0, 0, 0, 0, 0};
-  hip_check(hipMemcpyAsync(hs, ws.stats.p, sizeof hs, hipMemcpyDeviceToHost, st), "stats D2H");
-  hip_check(hipStreamSynchronize(st), "render sync");
-  if (stats) {
-    float ms = 0;
-    hip_check(hipEventElapsedTime(&ms, ctx->ev2, ctx->ev3), "event time");
-    stats->paths += paths_total;
-    stats->extension_rays += hs[0];
-    stats->shadow_rays += hs[1];
-    stats->bounces_launched += launches;
-    stats->gpu_ms += ms;
-    stats->trace_ms += trace_ms;
-    stats->trace_launches += trace_launches;
-    stats->node_visits += hs[2] + hs[4];
-    stats->triangle_tests += hs[3] + hs[5];
-  }
-}
-
-static void request_setup(hk_scene *s, const hk_request *req, ReqState &q, uint32_t cap_samples) {
-  const hk_scene_desc &d = s->desc;
-  const float *c2w = req->use_scene_camera ? d.cam_to_world : req->cam_to_world;
-  float fov = req->use_scene_camera ? d.fov : req->fov;
-  int w = req->use_scene_camera ? d.width : req->width;
-  int h = req->use_scene_camera ? d.height : req->height;
-  if (w <= 0 || h <= 0 || uint64_t(w) * uint64_t(h) > (1u << 26))
-    throw HkError(HK_ERR_INVALID, "bad film size %dx%d", w, h);
-  if (!(fov > 0.0f && fov < 180.0f)) throw HkError(HK_ERR_INVALID, "bad field of view %f", fov);
-  std::memset(&q.dev, 0, sizeof q.dev);
-  camera_setup(c2w, fov, w, h, d.near_clip, d.far_clip, q.dev);
-  filter_setup(d, q.dev);
-  q.dev.seed = req->seed;
-  q.npix = uint32_t(w) * uint32_t(h);
-  if (q.npix > cap_samples)
-    throw HkError(HK_ERR_INVALID, "film of %u pixels exceeds the sample capacity (%u)", q.npix, cap_samples);
-  q.spp = req->spp;
-  q.spp_offset = req->spp_offset;
-  q.chunk = std::max(1u, std::min(q.spp == 0 ? 1u : q.spp, cap_samples / q.npix));
-  q.done = 0;
-  q.accum.alloc(size_t(q.npix) * 16);
-  hip_check(hipMemsetAsync(q.accum.p, 0, size_t(q.npix) * 16, s->ctx->stream), "accum clear");
-  q.dev.accum = q.accum.as<float>();
-}
-
-static void finalize_to_host(hk_scene *s, ReqState &q, float *out_rgb) {
-  if (!out_rgb) return;
-  DBuf rgb;
-  rgb.alloc(size_t(q.npix) * 12);
-  hip_check(launch_finalize(q.accum.p, rgb.as<float>(), q.npix, s->ctx->stream), "k_finalize");
-  hip_check(hipMemcpyAsync(out_rgb, rgb.p, size_t(q.npix) * 12, hipMemcpyDeviceToHost, s->ctx->stream), "D2H");
-  hip_check(hipStreamSynchronize(s->ctx->stream), "finalize");
 }
 
 }  // namespace hk
@@ -471,12 +256,7 @@ int hk_render(hk_scene *s, const hk_request *req, float *out_rgb, hk_render_stat
   hk_ctx *ctx = s ? s->ctx : nullptr;
   HK_API_BEGIN
   if (!s || !req) throw HkError(HK_ERR_INVALID, "null argument");
-  if (stats) std::memset(stats, 0, sizeof *stats);
-  std::vector<ReqState> rs(1);
-  request_setup(s, req, rs[0], ctx->max_samples);
-  run_passes(s, rs, stats, ctx->max_paths, ctx->max_samples);
-  finalize_to_host(s, rs[0], out_rgb);
-  return HK_OK;
+  return hk_render_batch(s, req, 1, &out_rgb, 0, stats);
   HK_API_END(ctx)
 }
 
@@ -487,10 +267,19 @@ int hk_render_batch(hk_scene *s, const hk_request *reqs, uint32_t n, float **out
   if (!s || (n && !reqs)) throw HkError(HK_ERR_INVALID, "null argument");
   if (stats) std::memset(stats, 0, sizeof *stats);
   uint32_t pool = max_paths ? max_paths : ctx->max_paths;
-  std::vector<ReqState> rs(n);
-  for (uint32_t i = 0; i < n; i++) request_setup(s, &reqs[i], rs[i], ctx->max_samples);
-  run_passes(s, rs, stats, pool, ctx->max_samples);
-  for (uint32_t i = 0; i < n; i++) finalize_to_host(s, rs[i], out_rgb ? out_rgb[i] : nullptr);
+  // validate every request before any work starts
+  std::vector<std::unique_ptr<RequestRec>> rs;
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    rs.push_back(prepare_request(s, reqs[i], out_rgb ? out_rgb[i] : nullptr, ctx->max_samples));
+    total += uint64_t(rs.back()->npix) * rs.back()->spp;
+  }
+  uint32_t buf = uint32_t(std::min<uint64_t>(ctx->max_samples, std::max<uint64_t>(total, 1)));
+  Engine eng(s, ctx->ws, ctx->stream, uint32_t(std::min<uint64_t>(pool, std::max<uint64_t>(total, 1))), buf,
+             ctx->max_samples, ctx->profile, ctx->ev0, ctx->ev1);
+  for (auto &r : rs) eng.submit(r.get());
+  eng.run();
+  eng.take_stats(stats);
   return HK_OK;
   HK_API_END(ctx)
 }
@@ -501,37 +290,36 @@ int hk_render_pixel_samples(hk_scene *s, uint32_t seed, uint32_t x, uint32_t y, 
   if (!s || !out) throw HkError(HK_ERR_INVALID, "null argument");
   const hk_scene_desc &d = s->desc;
   if (x >= uint32_t(d.width) || y >= uint32_t(d.height)) throw HkError(HK_ERR_INVALID, "pixel out of range");
-  uint32_t cap = ctx->max_samples;
   uint32_t pix = y * uint32_t(d.width) + x;
-  for (uint32_t done = 0; done < n;) {
-    uint32_t k = std::min(cap, n - done);
-    // a one-pixel "request": pixel index offset handled through the film of
-    // width 1 and a pixel-index shift in the RNG key
-    std::vector<ReqState> rs(1);
-    hk_request rq;
-    std::memset(&rq, 0, sizeof rq);
-    rq.use_scene_camera = 1;
-    rq.seed = seed;
-    rq.spp = k;
-    rq.spp_offset = s0 + done;
-    request_setup(s, &rq, rs[0], std::max(cap, uint32_t(d.width) * uint32_t(d.height)));
-    rs[0].dev.pix0 = pix;
-    rs[0].npix = 1;
-    rs[0].chunk = k;
-    run_passes(s, rs, nullptr, ctx->max_paths, std::max(k, 1u), true);
-    std::vector<float> L(size_t(k) * 4);
-    hip_check(hipMemcpy(L.data(), ctx->ws.sample_L.p, size_t(k) * 16, hipMemcpyDeviceToHost), "D2H");
-    for (uint32_t i = 0; i < k; i++) {
-      Rng r = rng_for_path(seed, pix, s0 + done + i);
-      float jx = r.next(), jy = r.next();
-      float *o = out + 5 * size_t(done + i);
-      o[0] = L[4 * i];
-      o[1] = L[4 * i + 1];
-      o[2] = L[4 * i + 2];
-      o[3] = float(x) + jx;
-      o[4] = float(y) + jy;
-    }
-    done += k;
+  // a one-pixel request: the scene's film with the pixel range narrowed to
+  // `pix` (pix0 shifts the pixel index of every sample)
+  hk_request rq;
+  std::memset(&rq, 0, sizeof rq);
+  rq.use_scene_camera = 1;
+  rq.seed = seed;
+  rq.spp = n;
+  rq.spp_offset = s0;
+  uint32_t cap = std::max(ctx->max_samples, uint32_t(d.width) * uint32_t(d.height));
+  std::unique_ptr<RequestRec> r = prepare_request(s, rq, nullptr, cap);
+  r->dev.pix0 = pix;
+  r->npix = 1;
+  r->chunk = std::max(1u, std::min(n, ctx->max_samples));
+  r->n_chunks = (n + r->chunk - 1) / r->chunk;
+  r->chunk_item.assign(r->n_chunks, -1);
+  r->keep_samples = true;
+  Engine eng(s, ctx->ws, ctx->stream, std::min(ctx->max_paths, std::max(1u, r->chunk)), std::max(1u, r->chunk),
+             r->chunk, false, ctx->ev0, ctx->ev1);
+  eng.submit(r.get());
+  eng.run();
+  for (uint32_t i = 0; i < n; i++) {
+    Rng rg = rng_for_path(seed, pix, s0 + i);
+    float jx = rg.next(), jy = rg.next();
+    float *o = out + 5 * size_t(i);
+    o[0] = r->samples[4 * size_t(i)];
+    o[1] = r->samples[4 * size_t(i) + 1];
+    o[2] = r->samples[4 * size_t(i) + 2];
+    o[3] = float(x) + jx;
+    o[4] = float(y) + jy;
   }
   return HK_OK;
   HK_API_END(ctx)

@@ -130,11 +130,22 @@ struct DevRequest {
   float *accum;            // width x height x 4 (rgb * w, w)
 };
 
-// A slice of one request processed in one segment: every pixel, samples
-// [s0, s0 + k); segment sample ids [sample_begin, sample_begin + npix * k),
-// sample-major.
+// A chunk of one request: every pixel of its film slice, samples
+// [s0, s0 + k).  It occupies a slot of the device item table while it runs;
+// the radiance of its sample `local` (= sample offset * npix + pixel) goes to
+// sample_L[buf_base + local].
 struct WorkItem {
-  uint32_t req, sample_begin, npix, s0, k, pad0, pad1, pad2;
+  uint32_t req, npix, s0, k;
+  uint32_t buf_base, pad0, pad1, pad2;
+};
+constexpr uint32_t kMaxItems = 4096;     // item slots (12 bits of the path state)
+constexpr uint32_t kMaxRequests = 4096;  // request slots
+
+// One entry of k_regen's plan: start `count` samples of the item in `slot`,
+// local indices local0 .. local0 + count - 1; entries are consecutive in the
+// launch (prefix = number of samples planned before this entry).
+struct RegenEntry {
+  uint32_t slot, local0, count, prefix;
 };
 
 }  // namespace hk

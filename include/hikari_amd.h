@@ -176,6 +176,40 @@ int hk_render_pixel_samples(hk_scene *scene, uint32_t seed, uint32_t x, uint32_t
 int hk_render_batch(hk_scene *scene, const hk_request *reqs, uint32_t n, float **out_rgb, uint32_t max_paths,
                     hk_render_stats *stats);
 
+/* ---------------------------------------------------------------- serving
+ * A request loop with continuous batching (no counterpart in the reference,
+ * whose Render() handles one scene file per process).  A worker thread keeps
+ * one wavefront per resident scene, each on its own stream; a request that
+ * arrives while others run is admitted, first come first served, once its
+ * film fits the HBM budget, and its paths join the next bounce of the
+ * running ones.  hbm_budget bounds the server's path state and sample
+ * buffers (fixed per scene, max_paths / max_samples of the context) plus
+ * the films of admitted requests; 0 = unbounded.  Each result equals
+ * hk_render of the same request on a context with the same options, bit for
+ * bit.  Scenes must stay alive until hk_server_destroy. */
+typedef struct hk_server hk_server;
+typedef struct hk_server_result {
+  int32_t status;          /* HK_OK or the request's error */
+  double queue_ms;         /* submit -> first samples admitted */
+  double latency_ms;       /* submit -> film in host memory */
+  uint64_t paths;          /* path samples of the request */
+} hk_server_result;
+typedef struct hk_server_stats {
+  uint64_t submitted, completed, bounces, paths;
+  uint64_t fixed_bytes, film_bytes;   /* HBM held: workspaces, admitted films */
+} hk_server_stats;
+
+int hk_server_create(hk_ctx *ctx, uint64_t hbm_budget, hk_server **out);
+/* before the first submit; scene_id receives the scene's index */
+int hk_server_add_scene(hk_server *server, hk_scene *scene, int32_t *scene_id);
+/* out_rgb (height x width x 3) must stay valid until hk_server_wait returns */
+int hk_server_submit(hk_server *server, int32_t scene_id, const hk_request *req, float *out_rgb, uint64_t *ticket);
+/* blocks until the request is done; its status is also the return value */
+int hk_server_wait(hk_server *server, uint64_t ticket, hk_server_result *result);
+int hk_server_get_stats(hk_server *server, hk_server_stats *stats);
+/* finishes every submitted request, then frees the server */
+int hk_server_destroy(hk_server *server);
+
 /* Tuning knobs (value 0 restores the default):
  *   "max_paths"   paths in flight (default 4M); at every bounce finished
  *                 paths are replaced by the next samples

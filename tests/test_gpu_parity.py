@@ -218,3 +218,76 @@ def test_batch_equals_alone(ctx):
         for a, b in zip(alone, batch):
             assert a.shape == b.shape
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _serve_requests(box, field):
+    return [(0, hk.make_request(box, spp=20, seed=11)),
+            (1, hk.make_request(field, spp=6, seed=12)),
+            (0, hk.make_request(cam_to_world=D.lookat((0.2, -0.3, 0.95), (0, 0, -1), (0, 1, 0)), fov=40.0, width=24,
+                                height=18, spp=9, seed=13)),
+            (1, hk.make_request(cam_to_world=D.lookat((1.0, 3.0, 6.0), (0, 0, 0), (0, 1, 0)), fov=45.0, width=17,
+                                height=13, spp=31, seed=14, spp_offset=5)),
+            (0, hk.make_request(box, spp=1, seed=15)),
+            (1, hk.make_request(field, spp=0, seed=16))]
+
+
+def test_server_results_equal_alone():
+    """Continuous batching over two resident scenes: requests submitted while
+    others run (and split into chunks by a small sample capacity) return the
+    same films as hk_render of each request alone, bit for bit."""
+    import time
+    box = scenes.cbox(width=32, height=24, sphere_subdiv=2)
+    field = scenes.heightfield(width=28, height=20, n=24)
+    c = hk.Context(0)
+    c.set_option("max_paths", 2500)
+    c.set_option("max_samples", 32 * 24 * 7)
+    g_box, g_field = c.scene(box), c.scene(field)
+    reqs = _serve_requests(box, field)
+    alone = [(g_box if sid == 0 else g_field).render(0, request=r)[0] for sid, r in reqs]
+    srv = hk.Server(c)
+    ids = [srv.add_scene(g_box), srv.add_scene(g_field)]
+    assert ids == [0, 1]
+    tickets = []
+    for i, (sid, r) in enumerate(reqs):
+        tickets.append(srv.submit(sid, r))
+        if i == 0:
+            time.sleep(0.02)  # the rest arrive while the first is running
+    for t, a, (sid, r) in zip(tickets, alone, reqs):
+        img, res = srv.wait(t)
+        assert res["status"] == 0 and res["latency_ms"] >= res["queue_ms"] >= 0
+        assert res["paths"] == r.spp * r.width * r.height
+        assert np.array_equal(img.view(np.uint32), a.view(np.uint32)), (sid, r.spp, r.seed)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["film_bytes"] == 0
+    srv.close()
+    c.close()
+
+
+def test_server_hbm_budget_admission():
+    """A budget that holds the workspaces plus one film serves requests one at
+    a time; a film larger than the whole budget is refused with HK_ERR_BUDGET."""
+    box = scenes.cbox(width=32, height=24, sphere_subdiv=2)
+    c = hk.Context(0)
+    c.set_option("max_paths", 4000)
+    c.set_option("max_samples", 32 * 24 * 4)
+    g = c.scene(box)
+    probe = hk.Server(c)
+    probe.add_scene(g)
+    fixed = probe.stats()["fixed_bytes"]
+    probe.close()
+    film = 32 * 24 * 16
+    srv = hk.Server(c, hbm_budget=fixed + film + 100)
+    srv.add_scene(g)
+    reqs = [hk.make_request(box, spp=3 + i, seed=20 + i) for i in range(4)]
+    tickets = [srv.submit(0, r) for r in reqs]
+    big = hk.make_request(cam_to_world=D.lookat((0, 0, 0.99), (0, 0, -1), (0, 1, 0)), fov=45.0, width=64, height=48,
+                          spp=1, seed=1)
+    tb = srv.submit(0, big)
+    for t, r in zip(tickets, reqs):
+        img, res = srv.wait(t)
+        assert np.array_equal(img.view(np.uint32), g.render(0, request=r)[0].view(np.uint32))
+    with pytest.raises(hk.HkError) as e:
+        srv.wait(tb)
+    assert e.value.code == -6
+    srv.close()
+    c.close()
