@@ -10,6 +10,15 @@ Prints one JSON line: path samples/s (whole job), p50/p99 step latency,
 rays/s, the HBM roofline of the dominant kernel (k_extend, timed with HIP
 events on the context stream) and the oracle's CPU throughput on a bounded
 sample of the same workload.
+
+Other workloads (one JSON line each, not part of the default run):
+  --workload batch  64 requests over the same scene in one hk_render_batch,
+                    against the same requests rendered one at a time
+                    (hk_render); images are compared bit for bit.
+  --workload serve  Poisson arrivals over three resident scenes (box,
+                    displaced sphere, heightfield) through hk_server, against
+                    a first-come-first-served queue of plain hk_render calls
+                    on the same trace.
 """
 import argparse
 import json
@@ -40,6 +49,10 @@ def parse():
     ap.add_argument("--max-paths", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", choices=["c1", "batch", "serve"], default="c1")
+    ap.add_argument("--requests", type=int, default=0, help="batch/serve: number of requests (default 64 / 40)")
+    ap.add_argument("--qps", type=float, default=1.0, help="serve: Poisson arrival rate")
+    ap.add_argument("--budget-gb", type=float, default=96.0, help="serve: HBM budget of the server")
     return ap.parse_args()
 
 
@@ -61,12 +74,129 @@ def cpu_baseline(scene, target_s):
                       f"({paths} paths, {secs:.1f} s render time, BVH build excluded)"}
 
 
+def _pct(xs, q):
+    return float(np.percentile(np.asarray(xs, np.float64), q))
+
+
+def bench_batch(a):
+    """64 requests over the displaced-sphere scene: one batch vs one at a time."""
+    import hikari_amd as hk
+    from hikari_amd import scenes, workloads
+    scene = scenes.displaced_sphere(alpha=a.alpha)
+    ctx = hk.Context(0)
+    gs = ctx.scene(scene)
+    reqs = workloads.batch_requests(n=a.requests or 64)
+    total = sum(r.width * r.height * r.spp for r in reqs)
+    print(f"# {len(reqs)} requests, {total / 1e9:.2f} G path samples", file=sys.stderr, flush=True)
+    gs.render(4)  # warm-up (kernels loaded, workspace allocated)
+    t0 = time.perf_counter()
+    imgs_b, st = gs.render_batch(reqs)
+    t_batch = time.perf_counter() - t0
+    print(f"# batched: {t_batch:.2f} s", file=sys.stderr, flush=True)
+    lat_plain, same = [], 0
+    t0 = time.perf_counter()
+    for r, ib in zip(reqs, imgs_b):
+        img, _ = gs.render(0, request=r)
+        lat_plain.append(time.perf_counter() - t0)
+        same += int(np.array_equal(img.view(np.uint32), ib.view(np.uint32)))
+    t_plain = time.perf_counter() - t0
+    print(f"# one at a time: {t_plain:.2f} s, identical images {same}/{len(reqs)}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "path samples/sec at matched relMSE vs plain GPU path; p50/p99 render latency",
+        "value": total / t_batch, "unit": "path samples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded procedural scene and request mix)",
+        "config": {"workload": f"{len(reqs)} batched requests over the displaced icosphere-8 scene, res "
+                               "{64..1024}^2, spp {16..1024}, camera on a radius-4 sphere",
+                   "requests": len(reqs), "path_samples": total},
+        "plain_value": total / t_plain, "batch_over_plain": t_plain / t_batch,
+        "identical_images": same,
+        "latency_ms": {"batched_p50": t_batch * 1e3, "batched_p99": t_batch * 1e3,
+                       "plain_p50": _pct(lat_plain, 50) * 1e3, "plain_p99": _pct(lat_plain, 99) * 1e3},
+        "rays_per_s": (st["extension_rays"] + st["shadow_rays"]) / t_batch,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(a):
+    """Poisson arrivals over three resident scenes: hk_server vs FCFS plain renders."""
+    import threading
+    import hikari_amd as hk
+    from hikari_amd import scenes, workloads
+    box, sphere, field = scenes.cbox(), scenes.displaced_sphere(alpha=a.alpha), scenes.heightfield()
+    ctx = hk.Context(0)
+    gss = [ctx.scene(box), ctx.scene(sphere), ctx.scene(field)]
+    trace = workloads.serve_trace([box, None, field], n=a.requests or 40, qps=a.qps)
+    total = sum(r.width * r.height * r.spp for _, _, r in trace)
+    print(f"# {len(trace)} requests over {trace[-1][0]:.1f} s, {total / 1e9:.2f} G path samples",
+          file=sys.stderr, flush=True)
+    for g in gss:
+        g.render(1)
+    # plain: each request alone; first come first served on one queue
+    service = []
+    for _, sid, r in trace:
+        t0 = time.perf_counter()
+        gss[sid].render(0, request=r)
+        service.append(time.perf_counter() - t0)
+    t_free, lat_plain = 0.0, []
+    for (arr, _, _), sv in zip(trace, service):
+        start = max(arr, t_free)
+        t_free = start + sv
+        lat_plain.append(t_free - arr)
+    span_plain = t_free - trace[0][0]
+    print(f"# plain FCFS: busy {sum(service):.1f} s, span {span_plain:.1f} s", file=sys.stderr, flush=True)
+    srv = hk.Server(ctx, hbm_budget=int(a.budget_gb * 2**30))
+    for g in gss:
+        srv.add_scene(g)
+    tickets = [None] * len(trace)
+    t_start = time.perf_counter()
+
+    def feeder():
+        for i, (arr, sid, r) in enumerate(trace):
+            d = t_start + arr - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(sid, r)
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    lat, t_last = [], 0.0
+    for i in range(len(trace)):
+        while tickets[i] is None:
+            time.sleep(0.001)
+        _, res = srv.wait(tickets[i])
+        lat.append(res["latency_ms"] * 1e-3)
+        t_last = max(t_last, time.perf_counter() - t_start)
+    th.join()
+    span = t_last - trace[0][0]
+    stats = srv.stats()
+    srv.close()
+    out = {
+        "metric": "path samples/sec at matched relMSE vs plain GPU path; p50/p99 render latency",
+        "value": total / span, "unit": "path samples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": span * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded procedural scenes, Poisson request trace)",
+        "config": {"workload": f"serving {len(trace)} requests at {a.qps} req/s over 3 resident scenes "
+                               "(box 5k tris, displaced sphere 1.3M, heightfield 2.1M); res {64..1024}^2, "
+                               "spp {16..1024}", "hbm_budget_gb": a.budget_gb, "path_samples": total},
+        "latency_ms": {"p50": _pct(lat, 50) * 1e3, "p99": _pct(lat, 99) * 1e3},
+        "plain_fcfs": {"value": total / span_plain, "latency_p50_ms": _pct(lat_plain, 50) * 1e3,
+                       "latency_p99_ms": _pct(lat_plain, 99) * 1e3, "busy_s": sum(service)},
+        "server": {"bounces": stats["bounces"], "fixed_gb": stats["fixed_bytes"] / 2**30},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
               "multi-GPU sharding is described in DESIGN.md and not run", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "batch":
+        return bench_batch(a)
+    if a.workload == "serve":
+        return bench_serve(a)
     import hikari_amd as hk
     from hikari_amd import scenes
 
@@ -102,11 +232,15 @@ def main():
     ext_bytes_per_launch = EXTEND_B * tot["extension_rays"] / max(1, tot["trace_launches"])
     achieved = ext_bytes_per_launch / (ext_avg_ms * 1e-3) / 1e9 if ext_avg_ms > 0 else 0.0
     render_bytes = SEG_B * tot["extension_rays"] + SHADOW_B * tot["shadow_rays"] + PATH_B * tot["paths"]
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # measurement (bytes per ray, scaled by a plain copy's counter readings)
     traffic = None
     tp = os.path.join(ROOT, "profiles", "k_extend_traffic.json")
     if os.path.exists(tp):
         with open(tp) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            per_ray = json.load(fh).get("hbm_bytes_per_ray")
+        if per_ray:
+            traffic = per_ray * tot["extension_rays"] / max(1, tot["trace_launches"])
     out = {
         "metric": "path samples/sec at matched relMSE vs plain GPU path; p50/p99 render latency",
         "value": value, "unit": "path samples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,

@@ -33,23 +33,37 @@ constexpr float kInfF = __builtin_huge_valf();
 constexpr int32_t kDone = kEmptyRef;
 
 // ------------------------------------------------------------------ traversal
+// Traversal stack of one lane: the top kLdsStack entries in LDS, the rest in
+// a private array.  The array is declared by the caller, apart from this
+// struct, so that only the array lives in scratch memory and sp, the
+// pointers and the ray state stay in registers.
+// entry = (tnear bits << 32) | ref; the LDS pointer keeps its address space
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
 struct TraceStack {
-  int2 *lds;  // kLdsStack x kBlock, entry (ref, tnear bits)
+  lds_u64 *lds;     // kLdsStack x kBlock entries
+  uint64_t *priv;   // kPrivStack entries
   int tid;
   int sp;
-  int2 priv[kPrivStack];
   __device__ __forceinline__ void push(int32_t ref, float t) {
-    int2 e = make_int2(ref, __float_as_int(t));
+    uint64_t e = ((uint64_t)__float_as_uint(t) << 32) | (uint32_t)ref;
     if (sp < kLdsStack) lds[sp * kBlock + tid] = e;
     else priv[sp - kLdsStack] = e;
     sp++;
   }
-  // next entry whose box can still hold a hit at t <= tlim, or kDone
+  // The empty asm statement on the overflow branch keeps the two loads in
+  // separate branches (a ds_read and a scratch load); without it the compiler
+  // merges them into one generic-address load that is slow for LDS.
   __device__ __forceinline__ int32_t pop(float tlim) {
     while (sp > 0) {
       sp--;
-      int2 e = sp < kLdsStack ? lds[sp * kBlock + tid] : priv[sp - kLdsStack];
-      if (__int_as_float(e.y) <= tlim) return e.x;
+      uint64_t e;
+      if (sp < kLdsStack) {
+        e = lds[sp * kBlock + tid];
+      } else {
+        e = priv[sp - kLdsStack];
+        __asm__ volatile("" : "+v"(e));
+      }
+      if (__uint_as_float((uint32_t)(e >> 32)) <= tlim) return (int32_t)(uint32_t)e;
     }
     return kDone;
   }
@@ -77,8 +91,8 @@ struct RayTrav {
   TraceStack st;
 };
 
-__device__ __forceinline__ void trav_begin(RayTrav &R, int2 *lds_stack, int32_t root, vec3 o, vec3 d, float tmin,
-                                           float tmax) {
+__device__ __forceinline__ void trav_begin(RayTrav &R, uint64_t *lds_stack, uint64_t *priv_stack, int32_t root, vec3 o,
+                                           vec3 d, float tmin, float tmax) {
   R.o = o;
   R.d = d;
   R.ix = fabsf(d.x) > 1e-30f ? 1.0f / d.x : copysignf(1e30f, d.x);
@@ -89,7 +103,8 @@ __device__ __forceinline__ void trav_begin(RayTrav &R, int2 *lds_stack, int32_t 
   R.bu = R.bv = 0.0f;
   R.bprim = -1;
   R.ref = root;
-  R.st.lds = lds_stack;
+  R.st.lds = (lds_u64 *)lds_stack;
+  R.st.priv = priv_stack;
   R.st.tid = threadIdx.x;
   R.st.sp = 0;
 }
@@ -189,10 +204,11 @@ __device__ __forceinline__ void leaf_step(const TraceTri *__restrict__ tris, Ray
 // tests run with the wave converged ("while-while").
 template <bool ANY>
 __device__ __forceinline__ bool trace_ray(const TraceNode *__restrict__ nodes, const TraceTri *__restrict__ tris,
-                                          int32_t root, int2 *lds_stack, vec3 o, vec3 d, float tmin, float tmax,
+                                          int32_t root, uint64_t *lds_stack, vec3 o, vec3 d, float tmin, float tmax,
                                           float &bt, float &bu, float &bv, int32_t &bprim, TraceCount &cnt) {
   RayTrav R;
-  trav_begin(R, lds_stack, root, o, d, tmin, tmax);
+  uint64_t priv[kPrivStack];
+  trav_begin(R, lds_stack, priv, root, o, d, tmin, tmax);
   for (;;) {
     while (R.ref >= 0) node_step(nodes, R, cnt);
     if (R.ref == kDone) break;
@@ -210,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_intersect(const TraceNode *__restric
                                                       const TraceTri *__restrict__ tris, int32_t root,
                                                       const float4 *__restrict__ rays, float4 *__restrict__ hits,
                                                       uint32_t n, int any) {
-  __shared__ int2 stack[kLdsStack * kBlock];
+  __shared__ uint64_t stack[kLdsStack * kBlock];
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     float4 a = rays[2 * i], b = rays[2 * i + 1];
@@ -513,7 +529,7 @@ constexpr uint32_t kNoRay = 0xffffffffu;
 
 #ifdef HK_PERSISTENT_TRACE
 __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
-  __shared__ int2 stack[kLdsStack * kBlock];
+  __shared__ uint64_t stack[kLdsStack * kBlock];
   uint32_t count = A.counters[qi];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     A.counters[qi ^ 1] = 0u;
@@ -525,6 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   float4 *qh = A.hit;
   TraceCount tc;
   RayTrav R;
+  uint64_t priv[kPrivStack];
   R.ref = kDone;
   uint32_t rid = kNoRay;
   bool exhausted = false;
@@ -544,7 +561,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
         if (idx < count) {
           rid = idx;
           float4 a = qo[idx], b = qd[idx];
-          trav_begin(R, stack, A.sc.root_ref, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w);
+          trav_begin(R, stack, priv, A.sc.root_ref, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w);
         }
       }
     }
@@ -566,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
 }
 #else
 __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
-  __shared__ int2 stack[kLdsStack * kBlock];
+  __shared__ uint64_t stack[kLdsStack * kBlock];
   uint32_t count = A.counters[qi];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     A.counters[qi ^ 1] = 0u;
@@ -737,11 +754,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
 // k_extend).
 #ifdef HK_PERSISTENT_TRACE
 __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
-  __shared__ int2 stack[kLdsStack * kBlock];
+  __shared__ uint64_t stack[kLdsStack * kBlock];
   uint32_t count = A.counters[2];
   uint32_t traced = 0;
   TraceCount tc;
   RayTrav R;
+  uint64_t priv[kPrivStack];
   R.ref = kDone;
   uint32_t ent = kNoRay, mask = 0, src = 0;
   float4 *lp = nullptr;
@@ -788,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
       size_t slot = size_t(e) * A.n_paths + src;
       float4 o = A.sh_o[slot], d = A.sh_d[slot], c4 = A.sh_c[slot];
       c = v3(c4.x, c4.y, c4.z);
-      trav_begin(R, stack, A.sc.root_ref, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), 1e-4f, 1.0f - 1e-4f);
+      trav_begin(R, stack, priv, A.sc.root_ref, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), 1e-4f, 1.0f - 1e-4f);
       traced++;
     }
     while (R.ref >= 0) node_step(A.sc.nodes, R, tc);
@@ -815,7 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
 }
 #else
 __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
-  __shared__ int2 stack[kLdsStack * kBlock];
+  __shared__ uint64_t stack[kLdsStack * kBlock];
   uint32_t count = A.counters[2];
   uint32_t stride = gridDim.x * blockDim.x;
   uint32_t traced = 0;
