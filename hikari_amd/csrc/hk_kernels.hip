@@ -120,7 +120,7 @@ __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, R
   float tlim = R.bt * kWiden;
   float tn[4];
   int32_t r[4];
-#ifdef HK_NODE_FLOAT
+#ifndef HK_NODE_QUANTIZED
   const float4 *np = reinterpret_cast<const float4 *>(nodes + R.ref);
   float4 lx4 = np[0], hx4 = np[1], ly4 = np[2], hy4 = np[3], lz4 = np[4], hz4 = np[5];
   int4 ch = *reinterpret_cast<const int4 *>(np + 6);
@@ -197,6 +197,64 @@ __device__ __forceinline__ void leaf_step(const TraceTri *__restrict__ tris, Ray
     }
   }
   R.ref = R.st.pop(R.bt * kWiden);
+}
+
+// The triangles of leaf `leaf` (no stack pop; any hit: ref = kDone at the
+// first hit).
+template <bool ANY>
+__device__ __forceinline__ void leaf_tris(const TraceTri *__restrict__ tris, RayTrav &R, int32_t leaf,
+                                          TraceCount &cnt) {
+  uint32_t code = uint32_t(~leaf);
+  uint32_t first = code >> 3, ntri = (code & 7u) + 1u;
+#ifdef HK_TRAVERSAL_STATS
+  cnt.tris += ntri;
+#endif
+  for (uint32_t k = 0; k < ntri; k++) {
+    const float4 *tp = reinterpret_cast<const float4 *>(tris + first + k);
+    float4 a = tp[0], b = tp[1], c = tp[2];
+    float u, v, t;
+    if (mt_test(R.o, R.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(b.w, c.x, c.y), R.tmin, R.bt, u, v, t)) {
+      int32_t prim = __float_as_int(a.w);
+      if (ANY) {
+        R.bt = t; R.bu = u; R.bv = v; R.bprim = prim;
+        R.ref = kDone;
+        return;
+      }
+      if (t < R.bt || prim > R.bprim) {
+        R.bt = t; R.bu = u; R.bv = v; R.bprim = prim;
+      }
+    }
+  }
+}
+
+// One round of speculative while-while traversal for a wave (every lane
+// must call it).  Node phase: lanes descend; a lane that reaches a leaf
+// parks it in `pend` and keeps descending while any lane of the wave is
+// still looking for its first leaf.  Leaf phase: every parked leaf is
+// tested.  The lane's ray is finished when ref == kDone and pend == kDone.
+// Visiting nodes before a parked leaf has lowered bt only costs extra
+// visits: the result is the minimum t (largest prim among equal t) in any
+// order.
+template <bool ANY>
+__device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, const TraceTri *__restrict__ tris,
+                                           RayTrav &R, int32_t &pend, TraceCount &cnt) {
+#ifdef HK_NO_SPECULATION
+  while (R.ref >= 0) node_step(nodes, R, cnt);
+  if (R.ref != kDone) leaf_step<ANY>(tris, R, cnt);
+#else
+  for (;;) {
+    if (R.ref < 0 && R.ref != kDone && pend == kDone) {
+      pend = R.ref;
+      R.ref = R.st.pop(R.bt * kWiden);
+    }
+    if (__ballot(R.ref >= 0 && pend == kDone) == 0ull) break;
+    if (R.ref >= 0) node_step(nodes, R, cnt);
+  }
+  if (pend != kDone) {
+    leaf_tris<ANY>(tris, R, pend, cnt);
+    pend = kDone;
+  }
+#endif
 }
 
 // Closest (ANY = false) or any (ANY = true) hit of one ray.  The inner loop
@@ -515,11 +573,11 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t
 
 // ------------------------------------------------------------------ k_extend
 // Closest hit of every queued ray (Scene::RayIntersect, src/core/Scene.cpp:
-// 117-120), one lane per ray.
-// Persistent form (HK_PERSISTENT_TRACE builds): a wave keeps tracing and, whenever at least
-// kRefill of its lanes have finished their rays, fetches that many new queue
-// entries with one atomic (counters[4], cleared by the kernel before), so
-// a few long rays no longer hold a whole wave.
+// 117-120).  Persistent lanes: a wave keeps tracing and, whenever at least kRefill of
+// its lanes have finished their rays, fetches that many new queue entries
+// with one atomic (counters[4], cleared by k_regen / k_shade before), so a
+// few long rays no longer hold a whole wave.  HK_PLAIN_TRACE builds trace
+// one ray per lane instead (~6% slower on the 1.3M-triangle sphere).
 #ifndef HK_REFILL
 #define HK_REFILL 16
 #endif
@@ -527,7 +585,7 @@ constexpr int kRefill = HK_REFILL;
 static_assert(kRefill >= 1 && kRefill <= 64, "refill threshold must fit a wave");
 constexpr uint32_t kNoRay = 0xffffffffu;
 
-#ifdef HK_PERSISTENT_TRACE
+#ifndef HK_PLAIN_TRACE
 __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
   uint32_t count = A.counters[qi];
@@ -543,6 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   RayTrav R;
   uint64_t priv[kPrivStack];
   R.ref = kDone;
+  int32_t pend = kDone;  // parked leaf
   uint32_t rid = kNoRay;
   bool exhausted = false;
   uint32_t lane = lane_id();
@@ -562,6 +621,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
           rid = idx;
           float4 a = qo[idx], b = qd[idx];
           trav_begin(R, stack, priv, A.sc.root_ref, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w);
+          pend = kDone;
         }
       }
     }
@@ -569,9 +629,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
       if (exhausted) break;
       continue;  // every lane idle: the next round fetches a full wave
     }
-    while (R.ref >= 0) node_step(A.sc.nodes, R, tc);
-    if (R.ref != kDone) leaf_step<false>(A.sc.tris, R, tc);
-    if (rid != kNoRay && R.ref == kDone) {
+    trav_round<false>(A.sc.nodes, A.sc.tris, R, pend, tc);
+    if (rid != kNoRay && R.ref == kDone && pend == kDone) {
       qh[rid] = R.bprim >= 0 ? make_float4(R.bt, R.bu, R.bv, __int_as_float(R.bprim))
                              : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
       rid = kNoRay;
@@ -749,10 +808,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
 // emitter samples.  One lane owns a shadow entry (one path): it traces the
 // entry's emitter rays in emitter order and adds the unoccluded
 // contributions to the path's radiance, so no atomics are needed and the
-// additions happen in the reference's order.  HK_PERSISTENT_TRACE builds
-// refill lanes as k_extend does (fetch counter counters[5], cleared by
-// k_extend).
-#ifdef HK_PERSISTENT_TRACE
+// additions happen in the reference's order.  Lanes refill as in k_extend
+// (fetch counter counters[5], cleared by k_extend); HK_PLAIN_TRACE builds
+// give each lane one entry.
+#ifndef HK_PLAIN_TRACE
 __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
   uint32_t count = A.counters[2];
@@ -761,6 +820,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   RayTrav R;
   uint64_t priv[kPrivStack];
   R.ref = kDone;
+  int32_t pend = kDone;  // parked leaf
   uint32_t ent = kNoRay, mask = 0, src = 0;
   float4 *lp = nullptr;
   vec3 L = v3(0.0f), c = v3(0.0f);
@@ -807,12 +867,12 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
       float4 o = A.sh_o[slot], d = A.sh_d[slot], c4 = A.sh_c[slot];
       c = v3(c4.x, c4.y, c4.z);
       trav_begin(R, stack, priv, A.sc.root_ref, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), 1e-4f, 1.0f - 1e-4f);
+      pend = kDone;
       traced++;
     }
-    while (R.ref >= 0) node_step(A.sc.nodes, R, tc);
-    if (R.ref != kDone) leaf_step<true>(A.sc.tris, R, tc);
+    trav_round<true>(A.sc.nodes, A.sc.tris, R, pend, tc);
     bool fin = false;
-    if (ent != kNoRay && R.ref == kDone) {
+    if (ent != kNoRay && R.ref == kDone && pend == kDone) {
       if (R.bprim < 0) L = L + c;
       if (mask) {
         start = true;
@@ -962,8 +1022,8 @@ hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t n
   hipLaunchKernelGGL(k_regen, dim3(grid_for(n_new, 1u << 16)), dim3(kBlock), 0, st, A, qi, q_base, n_new);
   return hipGetLastError();
 }
-// Blocks of a kernel that are resident at once on the current device (grid
-// cap of the persistent kernels; plain launches are never larger anyway).
+// Blocks of a kernel that are resident at once on the current device (the
+// grid of the persistent kernels).
 static uint32_t resident_blocks(const void *kernel) {
   int dev = 0, n_cu = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 1u << 16;

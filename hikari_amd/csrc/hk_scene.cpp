@@ -34,10 +34,10 @@ void upload_bvh(hk_ctx *ctx, const float *tri9, uint32_t nt, DevBvh &out) {
   // the traversal stack holds at most 3 entries per level below the root
   if (3 * r.max_depth + 1 > 96)
     throw HkError(HK_ERR_UNSUPPORTED, "BVH depth %u exceeds the traversal stack", r.max_depth);
-#ifdef HK_NODE_FLOAT
-  out.nodes.upload(r.nodes.data(), r.nodes.size() * sizeof(Bvh4Node), ctx->stream);
-#else
+#ifdef HK_NODE_QUANTIZED
   out.nodes.upload(r.qnodes.data(), r.qnodes.size() * sizeof(Bvh4QNode), ctx->stream);
+#else
+  out.nodes.upload(r.nodes.data(), r.nodes.size() * sizeof(Bvh4Node), ctx->stream);
 #endif
   out.tris.upload(r.tris.data(), r.tris.size() * sizeof(TraceTri), ctx->stream);
   out.n_nodes = uint32_t(r.nodes.size());
