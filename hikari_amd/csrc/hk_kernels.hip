@@ -522,7 +522,13 @@ __device__ __forceinline__ void post_intersect(const DevScene &sc, int32_t prim,
   }
 }
 
+// q_meta: depth (12 bits) | flags (8 bits) << 12 | item slot (12 bits) << 20.
+// Flags: bit 0 the last BSDF sample was discrete (pathMIS); bit 1 the last
+// surface was not diffuse, bits 2..7 its emitter + 1 (pathEMS).
+constexpr uint32_t kDepthMask = 0xfffu;
+constexpr uint32_t kFlagShift = 12;
 constexpr uint32_t kFlagPrevDiscrete = 1u;
+constexpr uint32_t kFlagLastSpecular = 2u;
 constexpr uint32_t kShadowLast = 0x80000000u;  // shadow entry flag: the path ended at this bounce
 
 // ------------------------------------------------------------------ k_regen
@@ -531,7 +537,7 @@ constexpr uint32_t kShadowLast = 0x80000000u;  // shadow entry flag: the path en
 // RenderBlock's prologue (src/Main.cpp:22-31) + PerspectiveCamera::SampleRay
 // (src/camera/PerspectiveCamera.cpp:77-102).  A sample's path depends only on
 // (request, pixel, sample index), never on its queue position.
-constexpr uint32_t kSlotShift = 20;  // q_meta: depth (16 bits) | flags (4) | item slot (12)
+constexpr uint32_t kSlotShift = 20;
 
 __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t q_base, uint32_t n_new) {
   uint32_t stride = gridDim.x * blockDim.x;
@@ -694,7 +700,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
       islot = meta >> kSlotShift;
       rng.state = (uint64_t)aux.x | ((uint64_t)aux.y << 32);
       rng.inc = rng_inc(pix);
-      uint32_t depth = meta & 0xffffu, flags = (meta >> 16) & 0xfu;
+      uint32_t depth = meta & kDepthMask, flags = (meta >> kFlagShift) & 0xffu;
+      const int mode = sc.integrator;  // 0 pathMIS, 1 pathEMS, 2 pathMATS
       beta = v3(b4.x, b4.y, b4.z);
       L = v3(l4.x, l4.y, l4.z);
       pdf_b = b4.w;
@@ -702,7 +709,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
       vec3 ro = v3(o4.x, o4.y, o4.z), rd = v3(d4.x, d4.y, d4.z);
       int32_t prim = __float_as_int(h4.w);
       if (prim < 0) {  // escaped (PathMISIntegrator.cpp:33-66)
-        if (depth == 0) {
+        if (mode != 0) {  // PathEMSIntegrator.cpp:30-42, PathMATSIntegrator.cpp:28-40
+          if (sc.has_env && !sc.force_background) L = L + beta * env_eval(sc, rd) / 1.0f;
+          else if (depth == 0) L = ld3(sc.background);
+        } else if (depth == 0) {
           if (sc.has_env && !sc.force_background) L = L + beta * env_eval(sc, rd) / 1.0f;
           else L = ld3(sc.background);
         } else if (sc.has_env) {
@@ -714,7 +724,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
         post_intersect(sc, prim, h4.y, h4.z, P, fr, mesh_id);
         const DevMesh &mesh = sc.meshes[mesh_id];
         int mat = (int)mesh.material;
-        if (mesh.emitter >= 0) {  // :73-81 (+ the MATS weight of :122-140)
+        bool diffuse_mat = (sc.mats[mat].flags & kMatFlagDiffuse) != 0;
+        if (mesh.emitter >= 0 && mode != 0) {
+          // PathMATSIntegrator.cpp:42-50 (always); PathEMSIntegrator.cpp:44-60
+          // (camera rays, after a non-diffuse surface, or the same emitter again)
+          bool count = mode == 2 || depth == 0 || (flags & kFlagLastSpecular) ||
+                       (flags >> 2) == (uint32_t)(mesh.emitter + 1);
+          if (count && dot(fr.n, rd) < 0.0f) L = L + beta * ld3(mesh.radiance);
+        } else if (mesh.emitter >= 0) {  // :73-81 (+ the MATS weight of :122-140)
           vec3 wi = P - ro;
           float dist = norm(wi);
           wi = wi / dist;
@@ -728,8 +745,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
           L = L + beta * w_mats * le;
         }
         vec3 wi_loc = to_local(fr, -1.0f * rd);
-        // emitter sampling over every emitter (:83-109)
-        for (int e = 0; e < sc.n_emitters; e++) {
+        // emitter sampling over every emitter (:83-109); pathEMS: only at
+        // diffuse surfaces, skipping the surface's own emitter, no MIS weight
+        // (PathEMSIntegrator.cpp:61-86); pathMATS: none
+        bool ems = mode == 0 || (mode == 1 && diffuse_mat);
+        for (int e = 0; ems && e < sc.n_emitters; e++) {
+          if (mode == 1 && e == mesh.emitter) continue;
           float sx = rng.next(), sy = rng.next(), s1 = rng.next();
           DevEmitter em = sc.emitters[e];
           ERec er;
@@ -741,9 +762,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
             br.wo = to_local(fr, er.wi);
             br.measure = kSolidAngle;
             br.eta = 1.0f;
-            float pb = bsdf_pdf(sc.mats, sc.rft, mat, br);
-            float w_ems = (er.pdf + pb != 0.0f) ? er.pdf / (er.pdf + pb) : 1.0f;
-            vec3 c = beta * bsdf_eval(sc.mats, sc.rft, mat, br) * fabsf(br.wo.z) * ld * w_ems;
+            vec3 c;
+            if (mode == 0) {
+              float pb = bsdf_pdf(sc.mats, sc.rft, mat, br);
+              float w_ems = (er.pdf + pb != 0.0f) ? er.pdf / (er.pdf + pb) : 1.0f;
+              c = beta * bsdf_eval(sc.mats, sc.rft, mat, br) * fabsf(br.wo.z) * ld * w_ems;
+            } else {
+              c = beta * bsdf_eval(sc.mats, sc.rft, mat, br) * fabsf(br.wo.z) * ld;
+            }
             if (!(c.x == 0.0f && c.y == 0.0f && c.z == 0.0f)) {
               size_t slot = size_t(e) * A.n_paths + i;
               vec3 sd = er.p - P;
@@ -777,9 +803,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
             depth++;
             cont = depth < (uint32_t)sc.max_depth;
           }
+          if (mode == 1)
+            flags = (diffuse_mat ? 0u : kFlagLastSpecular) | ((uint32_t)(mesh.emitter + 1) << 2);
         }
       }
-      meta_out = (depth & 0xffffu) | (flags << 16) | (islot << kSlotShift);
+      meta_out = (depth & kDepthMask) | (flags << kFlagShift) | (islot << kSlotShift);
     }
     uint32_t slot = wave_append(&A.counters[qn], cont);
     if (cont) {  // the path moves to the compacted next queue

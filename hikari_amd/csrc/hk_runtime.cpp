@@ -172,7 +172,7 @@ void build_materials(const float *recs, int n, const char *rft_dir, std::vector<
     m.type = int(r[0]);
     m.dist = int(r[1]);
     m.nested = int(r[2]);
-    m.flags = int(r[3]);
+    m.flags = int(r[3]) & 3;
     m.rft_off = -1;
     if (m.type < 0 || m.type >= kMatTypeCount) throw HkError(HK_ERR_INVALID, "material %d: unknown type %d", i, m.type);
     if (m.dist != kBeckmann && m.dist != kGGX) throw HkError(HK_ERR_INVALID, "material %d: unknown distribution", i);
@@ -244,6 +244,16 @@ void build_materials(const float *recs, int n, const char *rft_dir, std::vector<
         break;
     }
     out[i] = m;
+  }
+  // BSDF::IsDiffuse (DiffuseBSDF.cpp:71, PlasticBSDF.cpp:165,
+  // RoughPlasticBSDF.cpp:198 true; coatings forward to the nested BSDF,
+  // CoatingBSDF.cpp:194, RoughCoatingBSDF.cpp:223; false otherwise)
+  auto diffuse_type = [](int t) { return t == kDiffuse || t == kPlastic || t == kRoughPlastic; };
+  for (int i = 0; i < n; i++) {
+    DevMaterial &m = out[i];
+    bool d = diffuse_type(m.type) ||
+             ((m.type == kCoating || m.type == kRoughCoating) && diffuse_type(out[m.nested].type));
+    if (d) m.flags |= kMatFlagDiffuse;
   }
   if (rft_pool.empty()) rft_pool.push_back(0.0f);
 }

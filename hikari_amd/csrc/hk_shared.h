@@ -16,8 +16,10 @@ constexpr int kRftThetaN = 100;  // reduced rough-transmittance table length (th
 
 // Device material record (128 B).  Derived quantities are computed on the
 // host exactly as the reference's constructors / Activate() do.
+constexpr int32_t kMatFlagDiffuse = 4;  // BSDF::IsDiffuse (pathEMS samples emitters only there)
+
 struct alignas(16) DevMaterial {
-  int32_t type, dist, nested, flags;    // flags: 1 nonlinear, 2 anisotropic
+  int32_t type, dist, nested, flags;    // flags: 1 nonlinear, 2 anisotropic, 4 diffuse
   float a[4];                           // colour A (see include/hikari_amd.h)
   float b[4];                           // colour B
   float eta, inv_eta, inv_eta2, spec_weight;
@@ -115,6 +117,7 @@ struct DevScene {
   float background[3];
   int32_t force_background;
   int32_t root_ref;
+  int32_t integrator;       // 0 pathMIS, 1 pathEMS, 2 pathMATS
 };
 
 // One render request as the kernels see it.

@@ -128,9 +128,15 @@ typedef struct hk_scene_desc {
   float filter_radius, filter_stddev, filter_b, filter_c;
   int32_t max_depth; float background[3]; int32_t force_background;
   const char *rft_dir;                          /* nullable */
+  int32_t integrator;                           /* 0 pathMIS, 1 pathEMS, 2 pathMATS */
 } hk_scene_desc;
 
-/* Replaces Scene::Activate (src/core/Scene.cpp:128-165) plus the Activate()
+/* Integrators (estimators of the bounce kernel): pathMIS
+ * (src/integrator/PathMISIntegrator.cpp:16-157), pathEMS (emitter sampling
+ * only, PathEMSIntegrator.cpp:16-119) and pathMATS (BSDF sampling only,
+ * PathMATSIntegrator.cpp:16-87).  max_depth is clamped to 4095.
+ *
+ * Replaces Scene::Activate (src/core/Scene.cpp:128-165) plus the Activate()
  * of every mesh / BSDF / emitter / camera: uploads geometry, BVH, material
  * table (with reduced rough-transmittance tables), emitter CDFs and the
  * environment map to HBM.  Emitters are ordered area lights in mesh order,

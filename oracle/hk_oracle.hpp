@@ -571,6 +571,15 @@ struct Material {
     int_ior = r[20]; ext_ior = r[21]; alpha_u = r[22]; alpha_v = r[23]; thickness = r[24];
   }
   R clamp_alpha(R v) const { return clampr(v, R(5e-4f), R(1)); }
+  // BSDF::IsDiffuse: true for DiffuseBSDF.cpp:71, PlasticBSDF.cpp:165,
+  // RoughPlasticBSDF.cpp:198; the coatings forward to their nested BSDF
+  // (CoatingBSDF.cpp:194, RoughCoatingBSDF.cpp:223); false otherwise
+  // (the BSDF base class).
+  bool is_diffuse() const {
+    if (type == kDiffuse || type == kPlastic || type == kRoughPlastic) return true;
+    if ((type == kCoating || type == kRoughCoating) && nested) return nested->is_diffuse();
+    return false;
+  }
   // Constructor/Activate() derived quantities.
   void activate(const RftFile *beck, const RftFile *ggx, std::vector<Rft<R> *> &pool) {
     eta = int_ior / ext_ior;

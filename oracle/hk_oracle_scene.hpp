@@ -342,6 +342,7 @@ struct SceneIn {
   float background[3] = {0, 0, 0};
   int force_background = 0;
   const char *rft_dir = nullptr;
+  int integrator = 0;  // 0 pathMIS, 1 pathEMS, 2 pathMATS
 };
 
 template <typename R>
@@ -380,6 +381,7 @@ struct Scene {
   float filter_table[33];
   float filter_radius = 0.5f, lookup = 64;
   int max_depth = 1000;
+  int integrator = 0;  // 0 pathMIS, 1 pathEMS, 2 pathMATS
   V3<R> background;
   bool force_background = false;
   double bbox_lo[3], bbox_hi[3];
@@ -511,6 +513,7 @@ struct Scene {
     filter_table[32] = 0.0f;
     lookup = 32 / filter_radius;
     max_depth = in.max_depth;
+    integrator = in.integrator;
     background = V3<R>(in.background[0], in.background[1], in.background[2]);
     force_background = in.force_background != 0;
   }
@@ -769,6 +772,111 @@ V3<R> path_mis_li(const Scene<R> &sc, Pcg32 &rng, V3<R> ro, V3<R> rd, R rtmin, R
   return li;
 }
 
+// src/integrator/PathMATSIntegrator.cpp:16-87: BSDF sampling only; every
+// emitter hit adds its radiance at full weight.
+template <typename R>
+V3<R> path_mats_li(const Scene<R> &sc, Pcg32 &rng, V3<R> ro, V3<R> rd, R rtmin, R rtmax, PathStats *st) {
+  using Isect = typename Scene<R>::Isect;
+  using ERec = typename Scene<R>::ERec;
+  PcgU1 u1(&rng);
+  V3<R> li(R(0)), beta(R(1));
+  int depth = 0;
+  V3<R> to = ro, td = rd;
+  R ttmin = rtmin, ttmax = rtmax;
+  while (depth < sc.max_depth) {
+    Isect is;
+    if (st) st->ext++;
+    if (!sc.intersect(Scene<R>::to_rayf(to, td, ttmin, ttmax), is)) {
+      if (sc.has_env && !sc.force_background) li = li + beta * sc.env_eval(td) / R(1);
+      else if (depth == 0) return sc.background;
+      break;
+    }
+    const auto &mesh = sc.meshes[is.mesh];
+    const Material<R> &bsdf = sc.mats[mesh.material];
+    if (mesh.emitter >= 0) {
+      ERec er;
+      er.ref = to; er.p = is.p; er.n = is.sh.n; er.wi = td;
+      li = li + sc.area_eval(mesh, er) * beta;
+    }
+    BRec<R> br;
+    br.wi = is.sh.to_local(R(-1) * td);
+    R sx = rng.next_float(), sy = rng.next_float();
+    beta = beta * bsdf.sample(br, sx, sy, &u1);
+    if (is_zero_fuzzy(beta)) break;
+    if (rng.next_float() < 0.95f) beta = beta * R(1.0f / 0.95f);
+    else break;
+    to = is.p; td = is.sh.to_world(br.wo); ttmin = R(1e-4f); ttmax = std::numeric_limits<R>::infinity();
+    depth++;
+  }
+  return li;
+}
+
+// src/integrator/PathEMSIntegrator.cpp:16-119: emitter sampling at diffuse
+// surfaces (skipping the surface's own emitter), emission counted only on
+// camera rays, after specular bounces and on consecutive hits of one emitter.
+template <typename R>
+V3<R> path_ems_li(const Scene<R> &sc, Pcg32 &rng, V3<R> ro, V3<R> rd, R rtmin, R rtmax, PathStats *st) {
+  using Isect = typename Scene<R>::Isect;
+  using ERec = typename Scene<R>::ERec;
+  PcgU1 u1(&rng);
+  V3<R> li(R(0)), beta(R(1));
+  int depth = 0;
+  bool last_specular = false;
+  int last_emitter = -1;
+  V3<R> to = ro, td = rd;
+  R ttmin = rtmin, ttmax = rtmax;
+  while (depth < sc.max_depth) {
+    Isect is;
+    if (st) st->ext++;
+    if (!sc.intersect(Scene<R>::to_rayf(to, td, ttmin, ttmax), is)) {
+      if (sc.has_env && !sc.force_background) li = li + beta * sc.env_eval(td) / R(1);
+      else if (depth == 0) return sc.background;
+      break;
+    }
+    const auto &mesh = sc.meshes[is.mesh];
+    const Material<R> &bsdf = sc.mats[mesh.material];
+    if (mesh.emitter >= 0 && (depth == 0 || last_specular || last_emitter == mesh.emitter)) {
+      ERec er;
+      er.ref = to; er.p = is.p; er.n = is.sh.n; er.wi = td;
+      li = li + beta * sc.area_eval(mesh, er);
+    }
+    if (bsdf.is_diffuse()) {
+      last_specular = false;
+      for (int e = 0; e < int(sc.emitters.size()); e++) {
+        const auto &em = sc.emitters[e];
+        if (em.kind == 0 && em.mesh == int(is.mesh)) continue;  // the surface's own emitter
+        ERec er;
+        er.ref = is.p;
+        if (em.kind == 1) er.dist = sc.bbox_radius;
+        R sx = rng.next_float(), sy = rng.next_float(), s1 = rng.next_float();
+        V3<R> ld = em.kind == 0 ? sc.area_sample(sc.meshes[em.mesh], er, sx, sy, s1) : sc.env_sample(er, sx, sy);
+        if (!is_zero_fuzzy(ld)) {
+          if (st) st->shadow++;
+          RayF sr = Scene<R>::to_rayf(is.p, er.p - is.p, R(1e-4f), R(1.0f - 1e-4f));
+          if (!sc.occluded(sr)) {
+            BRec<R> br;
+            br.wi = is.sh.to_local(R(-1) * td); br.wo = is.sh.to_local(er.wi); br.measure = kSolidAngle;
+            li = li + beta * bsdf.eval(br) * std::abs(cos_theta(br.wo)) * ld;
+          }
+        }
+      }
+    } else {
+      last_specular = true;
+    }
+    BRec<R> br;
+    br.wi = is.sh.to_local(R(-1) * td);
+    R sx = rng.next_float(), sy = rng.next_float();
+    beta = beta * bsdf.sample(br, sx, sy, &u1);
+    if (is_zero_fuzzy(beta)) break;
+    if (rng.next_float() < 0.95f) beta = beta * R(1.0f / 0.95f);
+    else break;
+    last_emitter = mesh.emitter;
+    to = is.p; td = is.sh.to_world(br.wo); ttmin = R(1e-4f); ttmax = std::numeric_limits<R>::infinity();
+    depth++;
+  }
+  return li;
+}
+
 // ---------------------------------------------------------------- film + driver
 // ImageBlock::Put (src/core/Block.cpp:95-134) on one image-sized block:
 // pixel centres at +0.5, tabulated filter, invalid samples dropped.
@@ -813,6 +921,8 @@ V3<R> render_sample(const Scene<R> &sc, uint32_t req_seed, uint32_t x, uint32_t 
   R tmin, tmax;
   sc.camera_ray(R(px), R(py), o, d, tmin, tmax);
   if (st) st->paths++;
+  if (sc.integrator == 1) return path_ems_li(sc, rng, o, d, tmin, tmax, st);
+  if (sc.integrator == 2) return path_mats_li(sc, rng, o, d, tmin, tmax, st);
   return path_mis_li(sc, rng, o, d, tmin, tmax, st);
 }
 

@@ -26,6 +26,7 @@ struct ho_scene {
   int32_t filter; float filter_radius, filter_stddev, filter_b, filter_c;
   int32_t max_depth; float background[3]; int32_t force_background;
   const char *rft_dir;
+  int32_t integrator;
 };
 
 static char g_err[512];
@@ -244,6 +245,7 @@ static void to_in(const ho_scene *s, SceneIn &in) {
   for (int c = 0; c < 3; c++) in.background[c] = s->background[c];
   in.force_background = s->force_background;
   in.rft_dir = s->rft_dir;
+  in.integrator = s->integrator;
 }
 
 // Render samples [s0, s1) of every pixel.  out_rgb: W*H*3 resolved image

@@ -1,0 +1,70 @@
+"""GPU checks through the C++ host layer and the pathEMS / pathMATS
+estimators: the reference's own t-test files run by hikari_render, image
+parity with the reference's renders of scene/pa5/odyssey, and per-path
+agreement of the GPU estimators with the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import hikari_amd as hk
+from oracle import pyoracle as O
+from tests.scenes_testlib import lum
+from tests.test_host_cpu import read_pfm
+from tests.xmlscene import parse_test
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+CLI = os.path.join(ROOT, "hikari_amd", "hikari_render")
+
+
+def run(*args):
+    r = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    return r
+
+
+@pytest.mark.parametrize("name,total", [("test-direct.xml", 15), ("test-furnace.xml", 6)])
+def test_reference_ttest_files(name, total):
+    """scene/pa5/tests/*.xml through the C++ harness (src/core/StudentTTest.cpp
+    semantics): every scene, pathMIS, pathEMS and pathMATS, passes."""
+    r = run(os.path.join(GOLDEN, name))
+    assert r.returncode == 0 and f"Passed {total}/{total} tests." in r.stdout
+
+
+@pytest.mark.parametrize("kind", ["mis", "ems", "mats"])
+def test_odyssey_matches_reference_render(tmp_path, kind):
+    """scene/pa5/odyssey/odyssey-<kind>.xml rendered by hikari_render against
+    the reference's own EXR of the same scene (32 spp, its own random
+    stream): mean luminance within 1%, 24x24 block means within 3% (median)."""
+    out = tmp_path / f"{kind}.pfm"
+    r = run(os.path.join(GOLDEN, "odyssey", f"odyssey-{kind}.xml"), "--spp", "128", "-o", str(out))
+    assert r.returncode == 0
+    ref_pfm = tmp_path / f"ref_{kind}.pfm"
+    assert run("--convert", os.path.join(GOLDEN, "odyssey", f"odyssey-{kind}.exr"), str(ref_pfm)).returncode == 0
+    img, ref = read_pfm(out), read_pfm(ref_pfm)
+    assert img.shape == ref.shape == (432, 768, 3)
+    li, lr = lum(img), lum(ref)
+    print(kind, "mean", li.mean(), lr.mean())
+    assert abs(li.mean() / lr.mean() - 1) < 0.01
+    b = lambda a: a.reshape(18, 24, 32, 24).mean(axis=(1, 3))  # noqa: E731
+    rel = np.abs(b(li) - b(lr)) / (b(lr) + 0.05)
+    assert np.median(rel) < 0.03, float(np.median(rel))
+
+
+@pytest.mark.parametrize("integrator", ["pathEMS", "pathMATS"])
+def test_estimator_paths_track_oracle(ctx, integrator):
+    """test-direct.xml scenes with pathEMS / pathMATS: the same (pixel, sample)
+    keys give the same paths on the GPU and in the oracle."""
+    refs, scs = parse_test(os.path.join(GOLDEN, "test-direct.xml"))
+    for sc in [s for s in scs if s.integrator == integrator]:
+        gs = ctx.scene(sc)
+        g = gs.pixel_samples(0, 0, 0, 3000)
+        o = O.pixel_samples(sc, 0, 0, 0, 3000)
+        assert np.array_equal(g[:, 3:], o[:, 3:])
+        close = np.all(np.abs(g[:, :3] - o[:, :3]) <= 1e-4 * np.maximum(1.0, np.abs(o[:, :3])), axis=1)
+        print(integrator, "identical paths:", close.mean())
+        assert close.mean() > 0.99

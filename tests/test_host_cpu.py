@@ -1,0 +1,181 @@
+"""CPU checks of the C++ host layer (hikari_amd/host): the XML scene loader
+(checked against the independent test-side reader tests/xmlscene.py), its
+error behaviour, and the EXR / PFM codecs (checked against numpy's float16
+conversion and against the reference's own EXR renders).  No device is used:
+the loader runs with --dump, images with --convert."""
+import json
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from hikari_amd import desc as D
+from tests.xmlscene import parse_test
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+CLI = os.path.join(ROOT, "hikari_amd", "hikari_render")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(CLI), reason="hikari_render not built")
+
+
+def run(*args, check=True):
+    r = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120)
+    if check and r.returncode != 0:
+        raise AssertionError(r.stderr)
+    return r
+
+
+def dump(path):
+    return json.loads(run(path, "--dump").stdout)
+
+
+def read_pfm(path):
+    with open(path, "rb") as f:
+        assert f.readline().strip() == b"PF"
+        w, h = map(int, f.readline().split())
+        scale = float(f.readline())
+        a = np.frombuffer(f.read(), "<f4" if scale < 0 else ">f4").reshape(h, w, 3)
+    return a[::-1].copy()
+
+
+def write_pfm(path, img):
+    h, w, _ = img.shape
+    with open(path, "wb") as f:
+        f.write(f"PF\n{w} {h}\n-1\n".encode())
+        f.write(np.ascontiguousarray(img[::-1], "<f4").tobytes())
+
+
+@pytest.mark.parametrize("name", ["test-direct.xml", "test-furnace.xml"])
+def test_loader_matches_test_reader(name):
+    """Every scene of the reference's test file: same meshes (vertex / face
+    counts, bounds), material records, emitters, camera and integrator as the
+    Python reader builds."""
+    path = os.path.join(GOLDEN, name)
+    d = dump(path)
+    refs, scenes = parse_test(path)
+    assert np.allclose(d["references"], refs, rtol=1e-6)
+    assert len(d["scenes"]) == len(scenes)
+    for ds, ps in zip(d["scenes"], scenes):
+        assert ps.integrator in ds["integrator"] and f"depth = {ps.max_depth}" in ds["integrator"]
+        cam = ds["camera"]
+        assert (cam["width"], cam["height"]) == (ps.width, ps.height)
+        assert np.isclose(cam["fov"], ps.fov)
+        assert np.allclose(np.array(cam["to_world"]).reshape(4, 4), ps.cam_to_world, atol=1e-5)
+        assert len(ds["meshes"]) == len(ps.meshes)
+        for dm, pm in zip(ds["meshes"], ps.meshes):
+            assert dm["nv"] == pm.v.shape[0] and dm["nt"] == pm.f.shape[0]
+            assert np.allclose(dm["lo"], pm.v.min(0), atol=1e-5) and np.allclose(dm["hi"], pm.v.max(0), atol=1e-5)
+            rec = np.array(dm["materials"][-D.MAT_STRIDE:], np.float32)
+            assert np.allclose(rec, ps.materials[pm.material], rtol=1e-6)
+            if pm.radiance is not None:
+                assert np.allclose(dm["radiance"], pm.radiance)
+            else:
+                assert "radiance" not in dm
+
+
+def test_loader_reads_reference_scene_with_transforms():
+    """odyssey-mis.xml (scene/pa5/odyssey): translate / scale chains applied in
+    document order, each premultiplied (src/core/Parser.cpp:308-369)."""
+    d = dump(os.path.join(GOLDEN, "odyssey", "odyssey-mis.xml"))
+    assert d["spp"] == 32 and (d["camera"]["width"], d["camera"]["height"]) == (768, 432)
+    # cube.obj spans [-0.5, 0.5]^3 (checked here) -> translate(0,0.5,0), scale(1,9,4)
+    v, f, n, uv = D.load_obj(os.path.join(GOLDEN, "odyssey", "cube.obj"))
+    assert np.allclose(v.min(0), -0.5) and np.allclose(v.max(0), 0.5)
+    mono, floor, wall = d["meshes"]
+    assert np.allclose(mono["lo"], [-0.5, 0, -2], atol=1e-5) and np.allclose(mono["hi"], [0.5, 9, 2], atol=1e-5)
+    assert np.allclose(floor["lo"], [-12.5, -1, -9], atol=1e-5) and np.allclose(floor["hi"], [12.5, 0, 9], atol=1e-5)
+    assert np.allclose(wall["lo"], [-12.5, 0, -9], atol=1e-5) and np.allclose(wall["hi"], [-11.5, 10, 9], atol=1e-5)
+    assert wall["radiance"] == [1, 1, 1]
+
+
+@pytest.mark.parametrize("xml,needle", [
+    ("<scene><bogus/></scene>", "unexpected tag"),
+    ('<scene><integrator type="pathMIS"><integer name="depth" value="x"/></integrator></scene>', "integer"),
+    ('<scene><integrator type="pathMIS"/></scene>', "Property 'depth' is missing"),
+    ('<scene><integrator type="nosuch"/></scene>', "could not be found"),
+    ('<scene><integrator type="pathMIS"><integer name="depth" value="3" extra="1"/></integrator></scene>',
+     "Unexpected attribute"),
+    ('<scene><integrator type="pathMIS"><float name="depth" value="3"/></integrator></scene>', "wrong type"),
+    ("<scene><translate value='1,2,3'/></scene>", "transform"),
+    ('<scene><camera type="perspective"><color name="x" value="1, 2"/></camera></scene>', "Expected 3 values"),
+    ("<scene><mesh type='obj'>", "missing closing tag"),
+])
+def test_loader_errors(tmp_path, xml, needle):
+    p = tmp_path / "bad.xml"
+    p.write_text(xml)
+    r = run(str(p), "--dump", check=False)
+    assert r.returncode != 0 and needle in r.stderr, r.stderr
+
+
+def test_loader_requires_camera(tmp_path):
+    p = tmp_path / "nocam.xml"
+    p.write_text('<scene><integrator type="pathMIS"><integer name="depth" value="3"/></integrator></scene>')
+    r = run(str(p), "--dump", check=False)
+    assert r.returncode != 0 and "No camera" in r.stderr
+
+
+def test_exr_round_trip(tmp_path):
+    rng = np.random.default_rng(5)
+    img = rng.uniform(-2, 40, (17, 23, 3)).astype(np.float32)
+    img[0, 0] = [0.0, np.float32(1e-38), np.float32(3e38)]
+    write_pfm(tmp_path / "a.pfm", img)
+    run("--convert", str(tmp_path / "a.pfm"), str(tmp_path / "b.exr"))
+    run("--convert", str(tmp_path / "b.exr"), str(tmp_path / "c.pfm"))
+    back = read_pfm(tmp_path / "c.pfm")
+    assert np.array_equal(back.view(np.uint32), img.view(np.uint32))
+
+
+def _write_half_exr(path, img16):
+    """Uncompressed scanline EXR with HALF R, G, B channels (OpenEXR layout)."""
+    h, w, _ = img16.shape
+
+    def attr(name, typ, data):
+        return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<i", len(data)) + data
+    ch = b"".join(c.encode() + b"\0" + struct.pack("<iB3xii", 1, 0, 1, 1) for c in "BGR") + b"\0"
+    hdr = struct.pack("<II", 20000630, 2)
+    hdr += attr("channels", "chlist", ch) + attr("compression", "compression", b"\0")
+    hdr += attr("dataWindow", "box2i", struct.pack("<4i", 0, 0, w - 1, h - 1))
+    hdr += attr("displayWindow", "box2i", struct.pack("<4i", 0, 0, w - 1, h - 1))
+    hdr += attr("lineOrder", "lineOrder", b"\0") + attr("pixelAspectRatio", "float", struct.pack("<f", 1))
+    hdr += attr("screenWindowCenter", "v2f", struct.pack("<2f", 0, 0)) + attr("screenWindowWidth", "float",
+                                                                              struct.pack("<f", 1)) + b"\0"
+    table_at = len(hdr)
+    body, offsets = b"", []
+    for y in range(h):
+        offsets.append(table_at + 8 * h + len(body))
+        row = b"".join(np.ascontiguousarray(img16[y, :, c], "<f2").tobytes() for c in (2, 1, 0))
+        body += struct.pack("<ii", y, len(row)) + row
+    with open(path, "wb") as f:
+        f.write(hdr + struct.pack(f"<{h}Q", *offsets) + body)
+
+
+def test_exr_half_channels_match_numpy_float16(tmp_path):
+    rng = np.random.default_rng(6)
+    vals = np.concatenate([rng.uniform(-60000, 60000, 300), rng.uniform(-1e-4, 1e-4, 300),  # normals / subnormals
+                           [0.0, -0.0, 65504.0, 6e-8, np.inf, -np.inf]]).astype(np.float16)
+    img16 = np.resize(vals, (11, 19 * 3)).reshape(11, 19, 3)
+    _write_half_exr(tmp_path / "h.exr", img16)
+    run("--convert", str(tmp_path / "h.exr"), str(tmp_path / "h.pfm"))
+    back = read_pfm(tmp_path / "h.pfm")
+    assert np.array_equal(back.view(np.uint32), img16.astype(np.float32).view(np.uint32))
+
+
+def test_exr_decodes_reference_renders(tmp_path):
+    """scene/pa5/odyssey/odyssey-{mis,ems,mats}.exr (ZIP-compressed FLOAT,
+    written by OpenEXR): three independent estimators of one scene; decoded
+    correctly they agree to within their Monte Carlo noise."""
+    imgs = {}
+    for k in ("mis", "ems", "mats"):
+        run("--convert", os.path.join(GOLDEN, "odyssey", f"odyssey-{k}.exr"), str(tmp_path / f"{k}.pfm"))
+        imgs[k] = read_pfm(tmp_path / f"{k}.pfm")
+        assert imgs[k].shape == (432, 768, 3) and np.isfinite(imgs[k]).all() and imgs[k].min() >= 0
+    lum = {k: (v @ np.array([0.212671, 0.715160, 0.072169], np.float32)) for k, v in imgs.items()}
+    for k in ("ems", "mats"):
+        assert abs(lum[k].mean() / lum["mis"].mean() - 1) < 0.02
+        # 24x24-pixel block means agree far better than single pixels
+        b = lambda a: a.reshape(18, 24, 32, 24).mean(axis=(1, 3))  # noqa: E731
+        rel = np.abs(b(lum[k]) - b(lum["mis"])) / (b(lum["mis"]) + 0.05)
+        assert np.median(rel) < 0.03, (k, float(np.median(rel)))

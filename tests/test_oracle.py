@@ -249,18 +249,34 @@ def test_oracle_furnace(albedo, expect):
     assert abs(L.mean() - expect) < 4 * L.std() / np.sqrt(len(L)) + 1e-3
 
 
-def test_oracle_direct_lighting_references():
-    """scene/pa5/tests/test-direct.xml, the five pathMIS scenes and their reference values (t-test)."""
+@pytest.mark.parametrize("integrator", ["pathMIS", "pathEMS", "pathMATS"])
+def test_oracle_direct_lighting_references(integrator):
+    """scene/pa5/tests/test-direct.xml: the five scenes of each integrator against the file's reference values
+    (Student's t-test, Sidak-corrected over the 15 references as the reference harness does)."""
     from scipy import stats
     refs, scs = parse_test(os.path.join(GOLDEN, "test-direct.xml"))
-    mis = [(r, s) for r, s in zip(refs, scs) if s.integrator == "pathMIS"]
-    assert len(mis) == 5
-    for ref, sc in mis:
+    sel = [(r, s) for r, s in zip(refs, scs) if s.integrator == integrator]
+    assert len(sel) == 5
+    alpha = 1 - (1 - 0.01) ** (1 / len(refs))
+    for ref, sc in sel:
         vals = O.pixel_samples(sc, 0, 0, 0, 50000)
         L = lum(vals[:, :3])
         t = (L.mean() - ref) / (L.std(ddof=1) / np.sqrt(len(L)))
         p = 2 * (1 - stats.t.cdf(abs(t), len(L) - 1))
-        assert p > 0.01 / len(mis), (ref, L.mean(), p)
+        assert p > alpha, (integrator, ref, L.mean(), p)
+
+
+def test_oracle_furnace_file_references():
+    """scene/pa5/tests/test-furnace.xml: every scene (all three integrators) against its reference value."""
+    from scipy import stats
+    refs, scs = parse_test(os.path.join(GOLDEN, "test-furnace.xml"))
+    alpha = 1 - (1 - 0.01) ** (1 / len(refs))
+    for ref, sc in zip(refs, scs):
+        vals = O.pixel_samples(sc, 0, 0, 0, 20000)
+        L = lum(vals[:, :3])
+        t = (L.mean() - ref) / (L.std(ddof=1) / np.sqrt(len(L)))
+        p = 2 * (1 - stats.t.cdf(abs(t), len(L) - 1))
+        assert p > alpha, (sc.integrator, ref, L.mean(), p)
 
 
 def test_cbox_oracle_event_counts():
