@@ -37,9 +37,11 @@ def test_reference_ttest_files(name, total):
 
 @pytest.mark.parametrize("kind", ["mis", "ems", "mats"])
 def test_odyssey_matches_reference_render(tmp_path, kind):
-    """scene/pa5/odyssey/odyssey-<kind>.xml rendered by hikari_render against
-    the reference's own EXR of the same scene (32 spp, its own random
-    stream): mean luminance within 1%, 24x24 block means within 3% (median)."""
+    """scene/pa5/odyssey/odyssey-<kind>.xml (the three files are identical
+    pathMIS scenes) rendered by hikari_render against the reference's own EXR
+    odyssey-<kind>.exr (rendered with the named estimator, 32 spp, its own
+    random stream): mean luminance within 1%, 24x24 block means within 3%
+    (median)."""
     out = tmp_path / f"{kind}.pfm"
     r = run(os.path.join(GOLDEN, "odyssey", f"odyssey-{kind}.xml"), "--spp", "128", "-o", str(out))
     assert r.returncode == 0
