@@ -83,3 +83,44 @@ def rel_mse(img, ref, eps=1e-2):
     img = np.asarray(img, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))
+
+
+def env_plane_scene(albedo=0.5, env_w=64, env_h=32, radiance=1.0):
+    """Upward-facing diffuse plane (z = 0, +z normal) under a constant z-up
+    environment map; 1 x 1 film looking straight down at its centre."""
+    from hikari_amd import desc as D
+    from hikari_amd import scenes as S
+    sc = D.Scene(width=1, height=1, fov=1.0, max_depth=1000)
+    m = sc.add_material(D.diffuse(albedo))
+    v, f = S.quad((-10, -10, 0), (10, -10, 0), (10, 10, 0), (-10, 10, 0))
+    assert np.cross(v[1] - v[0], v[2] - v[0])[2] > 0
+    sc.add_mesh(v, f, m)
+    sc.env = np.full((env_h, env_w, 3), radiance, np.float32)
+    sc.env_to_world = np.eye(4)
+    sc.cam_to_world = D.lookat((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    return sc
+
+
+def env_plane_expectation(albedo=0.5, env_h=32, radiance=1.0):
+    """Expected pathMIS radiance of env_plane_scene, from the mathematics of
+    the estimator (PathMISIntegrator.cpp:59-66 and :83-107):
+      BSDF-sampled escape adds the environment at weight 1:    a L
+      environment sampling adds a L * int_{H+} (cos/pi) w dw,
+        w = p_env / (p_env + cos/pi), p_env the sampler's solid-angle pdf:
+        rows weighted by sin(theta_row centre), so in row i
+        p_env = (sin t_i / mean_j sin t_j) / (2 pi^2 sin theta).
+    The two add (the reference counts the environment twice)."""
+    from scipy.integrate import quad
+    tc = (np.arange(env_h) + 0.5) / env_h * np.pi
+    mean_sin = np.sin(tc).mean()
+    total = 0.0
+    for i in range(env_h // 2):
+        a, b = i * np.pi / env_h, (i + 1) * np.pi / env_h
+        pr = np.sin(tc[i]) / mean_sin
+
+        def g(t):
+            pl = pr / (2 * np.pi ** 2 * np.sin(t))
+            pb = np.cos(t) / np.pi
+            return 2 * np.cos(t) * np.sin(t) * pl / (pl + pb)
+        total += quad(g, a, b, epsabs=1e-12, epsrel=1e-12)[0]
+    return albedo * radiance * (1.0 + total)

@@ -291,3 +291,17 @@ def test_server_hbm_budget_admission():
     assert e.value.code == -6
     srv.close()
     c.close()
+
+
+@pytest.mark.parametrize("env_h", [16, 32])
+def test_environment_sampling_closed_form_gpu(ctx, env_h):
+    """The GPU environment sampler + estimator on the constant-sky plane
+    against the expectation integrated from the estimator's mathematics."""
+    from tests.scenes_testlib import env_plane_expectation, env_plane_scene
+    gs = ctx.scene(env_plane_scene(env_h=env_h))
+    vals = gs.pixel_samples(0, 0, 0, 400000)
+    L = lum(vals[:, :3])
+    se = L.std(ddof=1) / np.sqrt(len(L))
+    expect = env_plane_expectation(env_h=env_h)
+    print("gpu env plane", env_h, L.mean(), "+-", se, "expect", expect)
+    assert abs(L.mean() - expect) < 4 * se

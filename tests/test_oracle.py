@@ -279,6 +279,23 @@ def test_oracle_furnace_file_references():
         assert p > alpha, (sc.integrator, ref, L.mean(), p)
 
 
+@pytest.mark.parametrize("env_h", [16, 32])
+def test_oracle_environment_sampling_closed_form(env_h):
+    """Environment light (EnvironmentLight.cpp:8-111) + its MIS weights and the
+    double counting of PathMISIntegrator: the plane under a constant sky
+    matches the expectation integrated from the estimator's mathematics
+    (a wrong row weight, pdf Jacobian, sign or orientation moves it by > 1%)."""
+    from tests.scenes_testlib import env_plane_expectation, env_plane_scene
+    sc = env_plane_scene(env_h=env_h)
+    expect = env_plane_expectation(env_h=env_h)
+    vals = O.pixel_samples(sc, 0, 0, 0, 200000)
+    L = lum(vals[:, :3])
+    se = L.std(ddof=1) / np.sqrt(len(L))
+    print("env plane", env_h, L.mean(), "+-", se, "expect", expect)
+    assert abs(L.mean() - expect) < 4 * se
+    assert 0.2 < expect / 0.5 - 1 < 1.0  # the environment-sampling share is a sizeable part
+
+
 def test_cbox_oracle_event_counts():
     """Config-0 box: about 14.5 extension and 11.8 shadow rays per path in the reference's measurement."""
     sc = scenes.cbox(width=32, height=32)
