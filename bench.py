@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--max-paths", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["c1", "batch", "serve"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "batch", "serve"], default="c1")
     ap.add_argument("--requests", type=int, default=0, help="batch/serve: number of requests (default 64 / 40)")
     ap.add_argument("--qps", type=float, default=1.0, help="serve: Poisson arrival rate")
     ap.add_argument("--budget-gb", type=float, default=96.0, help="serve: HBM budget of the server")
@@ -201,7 +201,16 @@ def main():
     from hikari_amd import scenes
 
     t = time.time()
-    scene = scenes.displaced_sphere(alpha=a.alpha)
+    if a.workload == "c2":
+        scene = scenes.heightfield()
+        if a.spp == 256:
+            a.spp = 512
+        workload = ("1024^2 heightfield (2,093,058 tris) in 8 patches of 5 materials (rough dielectric, rough "
+                    f"plastic, coated rough conductor, dielectric, diffuse), 2 quad lights + env, 1280x720, {a.spp} spp")
+    else:
+        scene = scenes.displaced_sphere(alpha=a.alpha)
+        workload = ("displaced icosphere-8 (1,310,720 tris), GGX rough conductor alpha "
+                    f"{a.alpha}, env map 512x256, 1024x768, {a.spp} spp, plain gpuPathMIS")
     print(f"# scene built in {time.time() - t:.1f} s: {scene.n_triangles} triangles", file=sys.stderr, flush=True)
     ctx = hk.Context(0)
     ctx.set_option("profile", 1)
@@ -246,8 +255,7 @@ def main():
         "value": value, "unit": "path samples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": "displaced icosphere-8 (1,310,720 tris), GGX rough conductor alpha "
-                               f"{a.alpha}, env map 512x256, 1024x768, {a.spp} spp, plain gpuPathMIS",
+        "config": {"workload": workload,
                    "spp": a.spp, "width": scene.width, "height": scene.height, "triangles": scene.n_triangles},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "rays_per_s": rays / wall,

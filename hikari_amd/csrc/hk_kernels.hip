@@ -675,6 +675,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
 #endif
 
 // ------------------------------------------------------------------ k_shade
+constexpr uint32_t kSortKeys = 32;  // material keys of the block sort (last two: miss, idle lane)
 // One bounce of PathMISIntegrator::Li for every queued path.
 __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
   const DevScene &sc = A.sc;
@@ -682,8 +683,40 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
   if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[4] = 0u;  // k_extend's fetch counter
   int qn = qi ^ 1;
   uint32_t stride = gridDim.x * blockDim.x;
+#ifndef HK_NO_MATERIAL_SORT
+  // Block-local material sort: the block's 256 entries are counting-sorted by
+  // the material they hit (misses and idle lanes last), and lane t shades
+  // the entry of rank t, so waves see one material where they can.  Nothing
+  // moves in memory; a path's result does not depend on its lane.
+  __shared__ uint32_t s_cnt[kSortKeys];
+  __shared__ uint32_t s_perm[kBlock];
+#endif
   for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+#ifndef HK_NO_MATERIAL_SORT
+    uint32_t key = kSortKeys - 1;
+    if (base + threadIdx.x < count) {
+      int32_t hp = __float_as_int(A.hit[base + threadIdx.x].w);
+      key = hp < 0 ? kSortKeys - 2 : min(sc.meshes[sc.prim_mesh[hp]].material, kSortKeys - 3);
+    }
+    if (threadIdx.x < kSortKeys) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    uint32_t rank = atomicAdd(&s_cnt[key], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (uint32_t k = 0; k < kSortKeys; k++) {
+        uint32_t c = s_cnt[k];
+        s_cnt[k] = run;
+        run += c;
+      }
+    }
+    __syncthreads();
+    s_perm[s_cnt[key] + rank] = threadIdx.x;
+    __syncthreads();
+    uint32_t i = base + s_perm[threadIdx.x];
+#else
     uint32_t i = base + threadIdx.x;
+#endif
     bool active = i < count;
     bool cont = false;
     uint32_t smask = 0, pix = 0, local = 0, islot = 0, meta_out = 0;
