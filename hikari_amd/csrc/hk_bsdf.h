@@ -1,8 +1,9 @@
-// hk_bsdf.h -- Sample / Eval / Pdf of the nine BSDFs on the pathMIS hot path
-// (src/bsdf/{Diffuse,Conductor,Dielectric,Plastic,RoughConductor,
-// RoughDielectric,RoughPlastic,Coating,RoughCoating}BSDF.cpp), FP32 on the GPU.
-// Directions are in the local shading frame (z = shading normal).  A coating's
-// nested BSDF must be one of the seven non-coating types (checked on upload).
+// hk_bsdf.h -- Sample / Eval / Pdf of the BSDFs (src/bsdf/{Diffuse,Conductor,
+// Dielectric,Plastic,RoughConductor,RoughDielectric,RoughPlastic,Coating,
+// RoughCoating,Mirror,Microfacet,RoughDiffuse,TwoSided}BSDF.cpp), FP32 on the
+// GPU.  Directions are in the local shading frame (z = shading normal).  A
+// coating nests a base (non-coating, non-twoSided) BSDF; a twoSided nests
+// base BSDFs or coatings without transmission (checked on upload).
 #pragma once
 #include "hk_math.h"
 
@@ -167,6 +168,120 @@ HK_HD vec3 rough_plastic_sample(const DevMaterial &m, const float *rft, BRec &re
   return rough_plastic_eval(m, rft, rec) / p;
 }
 
+// ---------------------------------------------------------------- microfacet BSDF
+// src/bsdf/MicrofacetBSDF.cpp: a Beckmann lobe weighted by Ks = 1 - max(kd)
+// (held in spec_weight) over a Lambertian base.  sin^2 of a unit direction
+// is evaluated as x^2 + y^2 (the reference's 1 - z^2 loses FP32 digits near
+// the normal, where alpha = 0.005 plates put the lobe).
+HK_HD float mf_beckmann_d(vec3 h, float alpha) {  // :240-246
+  float c2 = h.z * h.z;
+  float e2 = (h.x * h.x + h.y * h.y) / c2 / (alpha * alpha);
+  return expf(-e2) / (kPiF * alpha * alpha * c2 * c2);
+}
+HK_HD float mf_smith_g1(vec3 v, vec3 h, float alpha) {  // :248-272
+  float tt = tan_theta(v);
+  if (tt == 0.0f) return 1.0f;
+  if (dot(h, v) * v.z <= 0.0f) return 0.0f;
+  float b = 1.0f / (alpha * tt);
+  if (b >= 1.6f) return 1.0f;
+  float b2 = b * b;
+  return (3.535f * b + 2.181f * b2) / (1.0f + 2.276f * b + 2.577f * b2);
+}
+HK_HD float mf_beckmann_pdf(vec3 h, float alpha) {  // Sampling.cpp:135-159
+  if (!(norm(h) - 1.0f <= 1e-6f && h.z >= 0.0f)) return 0.0f;
+  float c = h.z;
+  if (c == 0.0f) return 0.0f;
+  float t2 = (h.x * h.x + h.y * h.y) / (c * c);
+  if (isinf(t2)) return 0.0f;
+  float a2 = alpha * alpha;
+  return expf(-t2 / a2) / (kPiF * a2 * (c * c * c));
+}
+HK_HD vec3 mf_beckmann_sample(float sx, float sy, float alpha) {  // Sampling.cpp:115-133
+  float ln = logf(1.0f - sx);
+  if (isnan(ln)) ln = 1.0f;
+  float t2 = -alpha * alpha * ln;
+  float c = 1.0f / sqrtf(1.0f + t2);
+  float st = sqrtf(t2) * c;  // sqrt(1 - c^2) without the cancellation
+  float phi = 2.0f * kPiF * sy;
+  return v3(st * cosf(phi), st * sinf(phi), c);
+}
+// h: the half vector normalize(wi + wo) of the reference.  Sample() passes the
+// sampled microfacet normal instead, which is that vector exactly: at grazing
+// incidence wi + wo cancels to a few ulps of its terms and the FP32 sum would
+// cost the sharp lobes (alpha = 0.005) percents.
+HK_HD vec3 microfacet_value(const DevMaterial &m, const BRec &rec, vec3 h) {  // :96-135 (after its checks)
+  float a = m.alpha_u, ct;
+  float d = mf_beckmann_d(h, a);
+  float f = fresnel_dielectric(dot(h, rec.wi), m.eta, m.inv_eta, ct);
+  float g = mf_smith_g1(rec.wi, h, a) * mf_smith_g1(rec.wo, h, a);
+  float spec = m.spec_weight * f * d * g / (4.0f * rec.wi.z * rec.wo.z);
+  if (!(spec >= 0.0f) || isinf(spec)) spec = 0.0f;
+  return v3(spec) + mat_a(m) * kInvPiF;
+}
+HK_HD float microfacet_pdf(const DevMaterial &m, const BRec &rec, vec3 h) {  // :137-165
+  if (rec.measure != kSolidAngle || rec.wi.z <= 0.0f || rec.wo.z <= 0.0f) return 0.0f;
+  float j = 0.25f / dot(h, rec.wo);
+  float ks = m.spec_weight;
+  return ks * mf_beckmann_pdf(h, m.alpha_u) * j + (1.0f - ks) * cosine_hemisphere_pdf(rec.wo);
+}
+HK_HD vec3 microfacet_sample(const DevMaterial &m, BRec &rec, float sx, float sy) {  // :34-94
+  if (rec.wi.z <= 0.0f) return v3(0.0f);
+  rec.measure = kSolidAngle;
+  float ks = m.spec_weight;
+  vec3 h;
+  if (sx < ks) {
+    h = mf_beckmann_sample(sx / ks, sy, m.alpha_u);
+    rec.wo = reflect_m(rec.wi, h);
+    if (dot(rec.wi, h) < 0.0f) h = -h;  // normalize(wi + wo) = sign(wi.h) h
+  } else {
+    rec.wo = cosine_hemisphere((sx - ks) / (1.0f - ks), sy);
+    h = normalized(rec.wi + rec.wo);
+  }
+  rec.eta = 1.0f;
+  float p = microfacet_pdf(m, rec, h);
+  if (p == 0.0f) return v3(0.0f);
+  return microfacet_value(m, rec, h) * rec.wo.z / p;
+}
+
+// ---------------------------------------------------------------- rough diffuse
+// RoughDiffuseBSDF::Eval (src/bsdf/RoughDiffuseBSDF.cpp:51-159): Oren-Nayar,
+// full or fast form; the value carries the cos(theta_o) factor.
+HK_HD vec3 rough_diffuse_eval(const DevMaterial &m, const BRec &rec) {
+  float ci = rec.wi.z, co = rec.wo.z;
+  if (ci <= 0.0f || co <= 0.0f || rec.measure != kSolidAngle) return v3(0.0f);
+  float sigma = m.alpha_u * 0.70710678118655f;
+  float s2 = sigma * sigma;
+  float sti = sin_theta(rec.wi), sto = sin_theta(rec.wo);
+  float cpd = 0.0f;
+  if (sti > 1e-4f && sto > 1e-4f) cpd = cos_phi(rec.wi) * cos_phi(rec.wo) + sin_phi(rec.wi) * sin_phi(rec.wo);
+  if (m.flags & kMatFlagFast) {
+    float A = 1.0f - 0.5f * s2 / (s2 + 0.33f);
+    float B = 0.45f * s2 / (s2 + 0.09f);
+    float sa = ci > co ? sto : sti;
+    float tb = ci > co ? sti / ci : sto / co;
+    return mat_a(m) * (kInvPiF * co * (A + B * fmaxf(cpd, 0.0f) * sa * tb));
+  }
+  float ti = acosf(clampf(ci, -1.0f, 1.0f)), to = acosf(clampf(co, -1.0f, 1.0f));
+  float al = fmaxf(ti, to), be = fminf(ti, to);
+  float sa, sb, tb;
+  if (ci > co) { sa = sto; sb = sti; tb = sti / ci; }
+  else { sa = sti; sb = sto; tb = sto / co; }
+  float t1 = s2 / (s2 + 0.09f);
+  float t2 = (4.0f * kInvPiF * kInvPiF) * al * be;
+  float t3 = 2.0f * be * kInvPiF;
+  float c1 = 1.0f - 0.5f * s2 / (s2 + 0.33f);
+  float c2 = 0.45f * t1;
+  float c3 = 0.125f * t1 * t2 * t2;
+  float c4 = 0.17f * s2 / (s2 + 0.13f);
+  c2 *= cpd > 0.0f ? sa : sa - t3 * t3 * t3;
+  float th = (sa + sb) / (sqrtf(fmaxf(0.0f, 1.0f - sa * sa)) + sqrtf(fmaxf(0.0f, 1.0f - sb * sb)));
+  if (isinf(th) || isnan(th)) return v3(0.0f);
+  vec3 rho = mat_a(m);
+  vec3 lr1 = rho * (c1 + cpd * c2 * tb + (1.0f - fabsf(cpd)) * c3 * th);
+  vec3 lr2 = rho * rho * (c4 * (1.0f - cpd * t3 * t3));
+  return (lr1 + lr2) * (kInvPiF * co);
+}
+
 HK_HD vec3 base_sample(const DevMaterial &m, const float *rft, BRec &rec, float sx, float sy, U1 &u1) {
   switch (m.type) {
     case kDiffuse: {
@@ -237,6 +352,23 @@ HK_HD vec3 base_sample(const DevMaterial &m, const float *rft, BRec &rec, float 
     }
     case kRoughDielectric: return rough_dielectric_sample(m, rec, sx, sy, u1);
     case kRoughPlastic: return rough_plastic_sample(m, rft, rec, sx, sy);
+    case kMirror: {  // MirrorBSDF.cpp:12-27
+      if (rec.wi.z <= 0.0f) return v3(0.0f);
+      rec.wo = reflect_z(rec.wi);
+      rec.measure = kDiscrete;
+      rec.eta = 1.0f;
+      return v3(1.0f);
+    }
+    case kMicrofacet: return microfacet_sample(m, rec, sx, sy);
+    case kRoughDiffuse: {  // RoughDiffuseBSDF.cpp:28-49
+      rec.measure = kSolidAngle;
+      if (rec.wi.z <= 0.0f) return v3(0.0f);
+      rec.wo = cosine_hemisphere(sx, sy);
+      rec.eta = 1.0f;
+      float p = cosine_hemisphere_pdf(rec.wo);
+      if (p == 0.0f) return v3(0.0f);
+      return rough_diffuse_eval(m, rec) / p;
+    }
     default: return v3(0.0f);
   }
 }
@@ -291,6 +423,14 @@ HK_HD vec3 base_eval(const DevMaterial &m, const float *rft, const BRec &rec) {
     }
     case kRoughDielectric: return rough_dielectric_eval(m, rec);
     case kRoughPlastic: return rough_plastic_eval(m, rft, rec);
+    case kMirror:  // MirrorBSDF.cpp:29-41
+      if (rec.wi.z > 0.0f && rec.wo.z > 0.0f && rec.measure == kDiscrete && near_dir(reflect_z(rec.wi), rec.wo))
+        return v3(1.0f);
+      return v3(0.0f);
+    case kMicrofacet:
+      if (rec.measure != kSolidAngle || rec.wi.z <= 0.0f || rec.wo.z <= 0.0f) return v3(0.0f);
+      return microfacet_value(m, rec, normalized(rec.wi + rec.wo));
+    case kRoughDiffuse: return rough_diffuse_eval(m, rec);
     default: return v3(0.0f);
   }
 }
@@ -333,6 +473,14 @@ HK_HD float base_pdf(const DevMaterial &m, const float *rft, const BRec &rec) {
     }
     case kRoughDielectric: return rough_dielectric_pdf(m, rec);
     case kRoughPlastic: return rough_plastic_pdf(m, rft, rec);
+    case kMirror:  // MirrorBSDF.cpp:43-55
+      if (rec.wi.z > 0.0f && rec.wo.z > 0.0f && rec.measure == kDiscrete && near_dir(reflect_z(rec.wi), rec.wo))
+        return 1.0f;
+      return 0.0f;
+    case kMicrofacet: return microfacet_pdf(m, rec, normalized(rec.wi + rec.wo));
+    case kRoughDiffuse:  // RoughDiffuseBSDF.cpp:161-172
+      if (rec.wi.z <= 0.0f || rec.wo.z <= 0.0f || rec.measure != kSolidAngle) return 0.0f;
+      return cosine_hemisphere_pdf(rec.wo);
     default: return 0.0f;
   }
 }
@@ -488,23 +636,64 @@ HK_HD vec3 rough_coating_sample(const DevMaterial &m, const DevMaterial &nm, con
 }
 
 // ---------------------------------------------------------------- dispatch
-HK_HD vec3 bsdf_sample(const DevMaterial *mats, const float *rft, int idx, BRec &rec, float sx, float sy, U1 &u1) {
+// Three levels, no recursion: a twoSided picks its side's material, which may
+// be a coating over a base BSDF (both nestings are checked on upload).
+HK_HD vec3 coated_sample(const DevMaterial *mats, const float *rft, int idx, BRec &rec, float sx, float sy, U1 &u1) {
   const DevMaterial &m = mats[idx];
   if (m.type == kCoating) return coating_sample(m, mats[m.nested], rft, rec, sx, sy, u1);
   if (m.type == kRoughCoating) return rough_coating_sample(m, mats[m.nested], rft, rec, sx, sy, u1);
   return base_sample(m, rft, rec, sx, sy, u1);
 }
-HK_HD vec3 bsdf_eval(const DevMaterial *mats, const float *rft, int idx, const BRec &rec) {
+HK_HD vec3 coated_eval(const DevMaterial *mats, const float *rft, int idx, const BRec &rec) {
   const DevMaterial &m = mats[idx];
   if (m.type == kCoating) return coating_eval(m, mats[m.nested], rft, rec);
   if (m.type == kRoughCoating) return rough_coating_eval(m, mats[m.nested], rft, rec);
   return base_eval(m, rft, rec);
 }
-HK_HD float bsdf_pdf(const DevMaterial *mats, const float *rft, int idx, const BRec &rec) {
+HK_HD float coated_pdf(const DevMaterial *mats, const float *rft, int idx, const BRec &rec) {
   const DevMaterial &m = mats[idx];
   if (m.type == kCoating) return coating_pdf(m, mats[m.nested], rft, rec);
   if (m.type == kRoughCoating) return rough_coating_pdf(m, mats[m.nested], rft, rec);
   return base_pdf(m, rft, rec);
+}
+// A twoSided is resolved to one side's material before the single call site
+// of each entry point (one inlined copy of the BSDF code per caller).
+// TwoSidedBSDF.cpp:21-75: a lower-side query is mirrored to the upper side;
+// Sample() mirrors the sampled direction back unless the weight is (fuzzy)
+// zero; Eval/Pdf take the upper side only for cos(wi) > 0.
+HK_HD int two_sided_side(const DevMaterial *mats, int idx, bool lower) {
+  const DevMaterial &m = mats[idx];
+  if (m.type != kTwoSided) return idx;
+  return lower ? m.nested_back : m.nested;
+}
+HK_HD vec3 bsdf_sample(const DevMaterial *mats, const float *rft, int idx, BRec &rec, float sx, float sy, U1 &u1) {
+  bool flip = mats[idx].type == kTwoSided && rec.wi.z < 0.0f;
+  int eff = two_sided_side(mats, idx, flip);
+  if (flip) rec.wi.z = -rec.wi.z;
+  vec3 res = coated_sample(mats, rft, eff, rec, sx, sy, u1);
+  if (flip) {
+    rec.wi.z = -rec.wi.z;
+    if (!is_zero_fuzzy(res)) rec.wo.z = -rec.wo.z;
+  }
+  return res;
+}
+HK_HD vec3 bsdf_eval(const DevMaterial *mats, const float *rft, int idx, const BRec &rec) {
+  bool flip = mats[idx].type == kTwoSided && !(rec.wi.z > 0.0f);
+  BRec c = rec;
+  if (flip) {
+    c.wi.z = -c.wi.z;
+    c.wo.z = -c.wo.z;
+  }
+  return coated_eval(mats, rft, two_sided_side(mats, idx, flip), c);
+}
+HK_HD float bsdf_pdf(const DevMaterial *mats, const float *rft, int idx, const BRec &rec) {
+  bool flip = mats[idx].type == kTwoSided && !(rec.wi.z > 0.0f);
+  BRec c = rec;
+  if (flip) {
+    c.wi.z = -c.wi.z;
+    c.wo.z = -c.wo.z;
+  }
+  return coated_pdf(mats, rft, two_sided_side(mats, idx, flip), c);
 }
 
 }  // namespace hk

@@ -677,7 +677,13 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
 // ------------------------------------------------------------------ k_shade
 constexpr uint32_t kSortKeys = 32;  // material keys of the block sort (last two: miss, idle lane)
 // One bounce of PathMISIntegrator::Li for every queued path.
+// HK_SHADE_MIN_WAVES (tuning builds): ask the register allocator for that
+// many waves per SIMD (it spills to scratch to get there).
+#ifdef HK_SHADE_MIN_WAVES
+__global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs A, int qi) {
+#else
 __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
+#endif
   const DevScene &sc = A.sc;
   uint32_t count = A.counters[qi];
   if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[4] = 0u;  // k_extend's fetch counter
@@ -734,7 +740,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
       rng.state = (uint64_t)aux.x | ((uint64_t)aux.y << 32);
       rng.inc = rng_inc(pix);
       uint32_t depth = meta & kDepthMask, flags = (meta >> kFlagShift) & 0xffu;
-      const int mode = sc.integrator;  // 0 pathMIS, 1 pathEMS, 2 pathMATS
+      const int mode = sc.integrator;  // 0 pathMIS, 1 pathEMS, 2 pathMATS, 3 whitted
       beta = v3(b4.x, b4.y, b4.z);
       L = v3(l4.x, l4.y, l4.z);
       pdf_b = b4.w;
@@ -742,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
       vec3 ro = v3(o4.x, o4.y, o4.z), rd = v3(d4.x, d4.y, d4.z);
       int32_t prim = __float_as_int(h4.w);
       if (prim < 0) {  // escaped (PathMISIntegrator.cpp:33-66)
-        if (mode != 0) {  // PathEMSIntegrator.cpp:30-42, PathMATSIntegrator.cpp:28-40
+        if (mode != 0) {  // PathEMSIntegrator.cpp:30-42, PathMATSIntegrator.cpp:28-40, WhittedIntegrator.cpp:33-52
           if (sc.has_env && !sc.force_background) L = L + beta * env_eval(sc, rd) / 1.0f;
           else if (depth == 0) L = ld3(sc.background);
         } else if (depth == 0) {
@@ -758,12 +764,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
         const DevMesh &mesh = sc.meshes[mesh_id];
         int mat = (int)mesh.material;
         bool diffuse_mat = (sc.mats[mat].flags & kMatFlagDiffuse) != 0;
+        vec3 le = v3(0.0f);
         if (mesh.emitter >= 0 && mode != 0) {
-          // PathMATSIntegrator.cpp:42-50 (always); PathEMSIntegrator.cpp:44-60
-          // (camera rays, after a non-diffuse surface, or the same emitter again)
-          bool count = mode == 2 || depth == 0 || (flags & kFlagLastSpecular) ||
+          // PathMATSIntegrator.cpp:42-50 and WhittedIntegrator.cpp:54-62
+          // (always); PathEMSIntegrator.cpp:44-60 (camera rays, after a
+          // non-diffuse surface, or the same emitter again)
+          bool count = mode >= 2 || depth == 0 || (flags & kFlagLastSpecular) ||
                        (flags >> 2) == (uint32_t)(mesh.emitter + 1);
-          if (count && dot(fr.n, rd) < 0.0f) L = L + beta * ld3(mesh.radiance);
+          if (dot(fr.n, rd) < 0.0f) le = ld3(mesh.radiance);
+          if (count) L = L + beta * le;
         } else if (mesh.emitter >= 0) {  // :73-81 (+ the MATS weight of :122-140)
           vec3 wi = P - ro;
           float dist = norm(wi);
@@ -780,64 +789,87 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
         vec3 wi_loc = to_local(fr, -1.0f * rd);
         // emitter sampling over every emitter (:83-109); pathEMS: only at
         // diffuse surfaces, skipping the surface's own emitter, no MIS weight
-        // (PathEMSIntegrator.cpp:61-86); pathMATS: none
-        bool ems = mode == 0 || (mode == 1 && diffuse_mat);
+        // (PathEMSIntegrator.cpp:61-86); whitted: the same, where the
+        // surface's own emitter adds Eval / Pdf |cos| Le along the shading
+        // normal (WhittedIntegrator.cpp:64-98); pathMATS: none
+        bool ems = mode == 0 || ((mode == 1 || mode == 3) && diffuse_mat);
+        // One Eval / Pdf site serves both the emitter samples and whitted's
+        // own-emitter term (each inlined BSDF copy costs registers).
         for (int e = 0; ems && e < sc.n_emitters; e++) {
-          if (mode == 1 && e == mesh.emitter) continue;
-          float sx = rng.next(), sy = rng.next(), s1 = rng.next();
-          DevEmitter em = sc.emitters[e];
+          bool self = mode != 0 && e == mesh.emitter;
+          if (self && mode != 3) continue;
+          BRec br;
+          br.wi = wi_loc;
+          br.measure = kSolidAngle;
+          br.eta = 1.0f;
           ERec er;
-          vec3 ld = em.kind == 0 ? area_sample(sc, sc.meshes[em.mesh], P, sx, sy, s1, er)
-                                 : env_sample(sc, P, sc.bbox_radius, sx, sy, er);
-          if (!is_zero_fuzzy(ld)) {
-            BRec br;
-            br.wi = wi_loc;
+          vec3 ld = v3(0.0f);
+          if (self) {
+            br.wo = to_local(fr, fr.n);
+          } else {
+            float sx = rng.next(), sy = rng.next(), s1 = rng.next();
+            DevEmitter em = sc.emitters[e];
+            ld = em.kind == 0 ? area_sample(sc, sc.meshes[em.mesh], P, sx, sy, s1, er)
+                              : env_sample(sc, P, sc.bbox_radius, sx, sy, er);
+            // whitted traces every sample whose radiance is not exactly zero
+            bool try_ems = mode == 3 ? !(ld.x == 0.0f && ld.y == 0.0f && ld.z == 0.0f) : !is_zero_fuzzy(ld);
+            if (!try_ems) continue;
             br.wo = to_local(fr, er.wi);
-            br.measure = kSolidAngle;
-            br.eta = 1.0f;
-            vec3 c;
-            if (mode == 0) {
-              float pb = bsdf_pdf(sc.mats, sc.rft, mat, br);
-              float w_ems = (er.pdf + pb != 0.0f) ? er.pdf / (er.pdf + pb) : 1.0f;
-              c = beta * bsdf_eval(sc.mats, sc.rft, mat, br) * fabsf(br.wo.z) * ld * w_ems;
-            } else {
-              c = beta * bsdf_eval(sc.mats, sc.rft, mat, br) * fabsf(br.wo.z) * ld;
-            }
-            if (!(c.x == 0.0f && c.y == 0.0f && c.z == 0.0f)) {
-              size_t slot = size_t(e) * A.n_paths + i;
-              vec3 sd = er.p - P;
-              A.sh_o[slot] = make_float4(P.x, P.y, P.z, 0.0f);
-              A.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, 0.0f);
-              A.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.0f);
-              smask |= 1u << e;
-            }
+          }
+          float pb = (mode == 0 || self) ? bsdf_pdf(sc.mats, sc.rft, mat, br) : 0.0f;
+          vec3 fe = bsdf_eval(sc.mats, sc.rft, mat, br);
+          if (self) {
+            if (pb != 0.0f) L = L + beta * (fe / pb * fabsf(br.wo.z) * le);
+            continue;
+          }
+          // x * 1 is exact: the weight is 1 where no MIS applies
+          float w_ems = (mode == 0 && er.pdf + pb != 0.0f) ? er.pdf / (er.pdf + pb) : 1.0f;
+          vec3 c = beta * fe * fabsf(br.wo.z) * ld * w_ems;
+          if (!(c.x == 0.0f && c.y == 0.0f && c.z == 0.0f)) {
+            size_t slot = size_t(e) * A.n_paths + i;
+            vec3 sd = er.p - P;
+            A.sh_o[slot] = make_float4(P.x, P.y, P.z, 0.0f);
+            A.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, 0.0f);
+            A.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.0f);
+            smask |= 1u << e;
           }
         }
-        // BSDF sampling (:111-120), MIS bookkeeping (:122-140), RR (:143-151)
-        BRec br;
-        br.wi = wi_loc;
-        br.wo = v3(0.0f);
-        br.eta = 1.0f;
-        br.measure = kUnknownMeasure;
-        float sx = rng.next(), sy = rng.next();
-        U1 u1{&rng, nullptr, 0, 0};
-        vec3 f = bsdf_sample(sc.mats, sc.rft, mat, br, sx, sy, u1);
-        nd = to_world(fr, br.wo);
-        beta = beta * f;
-        if (!is_zero_fuzzy(beta)) {
-          pdf_b = bsdf_pdf(sc.mats, sc.rft, mat, br);
-          flags = 0u;
-          if (br.measure == kDiscrete) {
-            w_mats = 1.0f;
-            flags = kFlagPrevDiscrete;
-          }
-          if (rng.next() < 0.95f) {
+        // BSDF sampling (:111-120), MIS bookkeeping (:122-140), RR (:143-151).
+        // whitted (WhittedIntegrator.cpp:100-113): a diffuse vertex ends the
+        // path; any other continues with probability 0.95, drawn before the
+        // BSDF sample, while depth < the depth property.
+        bool sample_bsdf = mode != 3 || (!diffuse_mat && rng.next() < 0.95f && depth < (uint32_t)sc.max_depth);
+        if (sample_bsdf) {
+          BRec br;
+          br.wi = wi_loc;
+          br.wo = v3(0.0f);
+          br.eta = 1.0f;
+          br.measure = kUnknownMeasure;
+          float sx = rng.next(), sy = rng.next();
+          U1 u1{&rng, nullptr, 0, 0};
+          vec3 f = bsdf_sample(sc.mats, sc.rft, mat, br, sx, sy, u1);
+          nd = to_world(fr, br.wo);
+          beta = beta * f;
+          if (mode == 3) {
             beta = beta * (1.0f / 0.95f);
-            depth++;
-            cont = depth < (uint32_t)sc.max_depth;
+            flags = 0u;
+            cont = !(beta.x == 0.0f && beta.y == 0.0f && beta.z == 0.0f);
+            if (cont) depth++;
+          } else if (!is_zero_fuzzy(beta)) {
+            pdf_b = bsdf_pdf(sc.mats, sc.rft, mat, br);
+            flags = 0u;
+            if (br.measure == kDiscrete) {
+              w_mats = 1.0f;
+              flags = kFlagPrevDiscrete;
+            }
+            if (rng.next() < 0.95f) {
+              beta = beta * (1.0f / 0.95f);
+              depth++;
+              cont = depth < (uint32_t)sc.max_depth;
+            }
+            if (mode == 1)
+              flags = (diffuse_mat ? 0u : kFlagLastSpecular) | ((uint32_t)(mesh.emitter + 1) << 2);
           }
-          if (mode == 1)
-            flags = (diffuse_mat ? 0u : kFlagLastSpecular) | ((uint32_t)(mesh.emitter + 1) << 2);
         }
       }
       meta_out = (depth & kDepthMask) | (flags << kFlagShift) | (islot << kSlotShift);

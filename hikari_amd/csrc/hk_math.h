@@ -135,6 +135,18 @@ HK_HD Frame frame_from_n_dpdu(vec3 n, vec3 dpdu) {  // src/core/Frame.cpp:9-14
 HK_HD vec3 to_local(const Frame &f, vec3 v) { return {dot(v, f.s), dot(v, f.t), dot(v, f.n)}; }
 HK_HD vec3 to_world(const Frame &f, vec3 v) { return (f.s * v.x + f.t * v.y) + f.n * v.z; }
 HK_HD float sin_theta2(vec3 v) { return 1.0f - v.z * v.z; }
+HK_HD float sin_theta(vec3 v) {  // Frame.cpp:38-46
+  float t = sin_theta2(v);
+  return t <= 0.0f ? 0.0f : sqrtf(t);
+}
+HK_HD float sin_phi(vec3 v) {  // Frame.cpp:68-76
+  float st = sin_theta(v);
+  return st == 0.0f ? 1.0f : clampf(v.y / st, -1.0f, 1.0f);
+}
+HK_HD float cos_phi(vec3 v) {  // Frame.cpp:78-86
+  float st = sin_theta(v);
+  return st == 0.0f ? 1.0f : clampf(v.x / st, -1.0f, 1.0f);
+}
 HK_HD float tan_theta(vec3 v) {
   float tmp = 1.0f - v.z * v.z;
   if (tmp <= 0.0f) return 0.0f;

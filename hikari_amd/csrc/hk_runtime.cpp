@@ -173,7 +173,9 @@ void build_materials(const float *recs, int n, const char *rft_dir, std::vector<
     m.dist = int(r[1]);
     m.nested = int(r[2]);
     m.flags = int(r[3]) & 3;
+    if (int(r[3]) & 4) m.flags |= kMatFlagFast;
     m.rft_off = -1;
+    m.nested_back = -1;
     if (m.type < 0 || m.type >= kMatTypeCount) throw HkError(HK_ERR_INVALID, "material %d: unknown type %d", i, m.type);
     if (m.dist != kBeckmann && m.dist != kGGX) throw HkError(HK_ERR_INVALID, "material %d: unknown distribution", i);
     for (int c = 0; c < 3; c++) { m.a[c] = r[4 + c]; m.b[c] = r[8 + c]; }
@@ -227,8 +229,8 @@ void build_materials(const float *recs, int n, const char *rft_dir, std::vector<
         if (m.nested < 0 || m.nested >= n)
           throw HkError(HK_ERR_INVALID, "material %d: a coating needs a nested BSDF", i);
         int nt = int(recs[size_t(m.nested) * HK_MATERIAL_STRIDE]);
-        if (nt == kCoating || nt == kRoughCoating)
-          throw HkError(HK_ERR_UNSUPPORTED, "material %d: nested coatings are not supported", i);
+        if (nt == kCoating || nt == kRoughCoating || nt == kTwoSided)
+          throw HkError(HK_ERR_UNSUPPORTED, "material %d: a coating over a coating or twoSided is not supported", i);
         if (m.type == kRoughCoating) {
           m.alpha_u = m.alpha_v = clamp_alpha(r[22]);
           RftTable ext = table(m.dist);
@@ -240,6 +242,32 @@ void build_materials(const float *recs, int n, const char *rft_dir, std::vector<
         }
         break;
       }
+      case kMicrofacet: {  // MicrofacetBSDF.cpp:10-28; Ks = 1 - max(kd) (:56)
+        m.alpha_u = m.alpha_v = clamp_alpha(r[22]);
+        m.spec_weight = 1.0f - std::max(m.a[0], std::max(m.a[1], m.a[2]));
+        break;
+      }
+      case kRoughDiffuse:  // RoughDiffuseBSDF.cpp:10-21
+        m.alpha_u = m.alpha_v = clamp_alpha(r[22]);
+        break;
+      case kTwoSided: {  // TwoSidedBSDF.cpp:93-122
+        int back = int(r[12]);
+        if (m.nested < 0 || m.nested >= n)
+          throw HkError(HK_ERR_INVALID, "material %d: TwoSidedBSDF needs at least one nested BSDF!", i);
+        if (back >= n) throw HkError(HK_ERR_INVALID, "material %d: lower-side material out of range", i);
+        m.nested_back = back < 0 ? m.nested : back;
+        for (int side : {m.nested, m.nested_back}) {
+          int t = int(recs[size_t(side) * HK_MATERIAL_STRIDE]);
+          if (t == kTwoSided) throw HkError(HK_ERR_UNSUPPORTED, "material %d: twoSided inside twoSided", i);
+          if (t == kCoating || t == kRoughCoating) {
+            int inner = int(recs[size_t(side) * HK_MATERIAL_STRIDE + 2]);
+            if (inner >= 0 && inner < n) t = int(recs[size_t(inner) * HK_MATERIAL_STRIDE]);
+          }
+          if (t == kDielectric || t == kRoughDielectric)
+            throw HkError(HK_ERR_INVALID, "material %d: Only BSDF without transmission can be nested!", i);
+        }
+        break;
+      }
       default:
         break;
     }
@@ -248,11 +276,17 @@ void build_materials(const float *recs, int n, const char *rft_dir, std::vector<
   // BSDF::IsDiffuse (DiffuseBSDF.cpp:71, PlasticBSDF.cpp:165,
   // RoughPlasticBSDF.cpp:198 true; coatings forward to the nested BSDF,
   // CoatingBSDF.cpp:194, RoughCoatingBSDF.cpp:223; false otherwise)
-  auto diffuse_type = [](int t) { return t == kDiffuse || t == kPlastic || t == kRoughPlastic; };
+  // MicrofacetBSDF.cpp:167 true; a twoSided is diffuse if either side is
+  // (TwoSidedBSDF.cpp:77-80)
+  auto diffuse_type = [](int t) { return t == kDiffuse || t == kPlastic || t == kRoughPlastic || t == kMicrofacet; };
+  auto coated_diffuse = [&](int k) {
+    const DevMaterial &m = out[k];
+    return diffuse_type(m.type) ||
+           ((m.type == kCoating || m.type == kRoughCoating) && diffuse_type(out[m.nested].type));
+  };
   for (int i = 0; i < n; i++) {
     DevMaterial &m = out[i];
-    bool d = diffuse_type(m.type) ||
-             ((m.type == kCoating || m.type == kRoughCoating) && diffuse_type(out[m.nested].type));
+    bool d = m.type == kTwoSided ? coated_diffuse(m.nested) || coated_diffuse(m.nested_back) : coated_diffuse(i);
     if (d) m.flags |= kMatFlagDiffuse;
   }
   if (rft_pool.empty()) rft_pool.push_back(0.0f);

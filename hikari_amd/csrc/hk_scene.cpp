@@ -234,10 +234,12 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
   ds.n_emitters = int32_t(emitters.size());
   ds.n_meshes = d->n_meshes;
   ds.bbox_radius = bbox_radius(lo, hi);
-  // the depth lives in 12 bits of the path state; RR ends a path long before
-  // (0.95^4095 ~ 1e-91)
-  ds.max_depth = std::min(d->max_depth, 4095);
-  if (d->integrator < 0 || d->integrator > 2) throw HkError(HK_ERR_INVALID, "unknown integrator %d", d->integrator);
+  // The depth property is a uint32 in the reference (PathMISIntegrator.cpp:13,
+  // WhittedIntegrator.cpp:13): negative means unlimited.  The depth lives in
+  // 12 bits of the path state, capping paths at 4095 vertices; RR ends a path
+  // long before (0.95^4095 ~ 1e-91).
+  ds.max_depth = d->max_depth < 0 ? 4095 : std::min(d->max_depth, 4095);
+  if (d->integrator < 0 || d->integrator > 3) throw HkError(HK_ERR_INVALID, "unknown integrator %d", d->integrator);
   ds.integrator = d->integrator;
   for (int c = 0; c < 3; c++) ds.background[c] = d->background[c];
   ds.force_background = d->force_background;

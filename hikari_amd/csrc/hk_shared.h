@@ -7,7 +7,8 @@ namespace hk {
 
 enum MatType : int32_t {
   kDiffuse = 0, kConductor = 1, kDielectric = 2, kPlastic = 3, kRoughConductor = 4,
-  kRoughDielectric = 5, kRoughPlastic = 6, kCoating = 7, kRoughCoating = 8, kMatTypeCount = 9
+  kRoughDielectric = 5, kRoughPlastic = 6, kCoating = 7, kRoughCoating = 8, kMirror = 9, kMicrofacet = 10,
+  kRoughDiffuse = 11, kTwoSided = 12, kMatTypeCount = 13
 };
 enum Measure : int32_t { kUnknownMeasure = 0, kSolidAngle = 1, kDiscrete = 2 };
 enum Dist : int32_t { kBeckmann = 0, kGGX = 1 };
@@ -16,17 +17,19 @@ constexpr int kRftThetaN = 100;  // reduced rough-transmittance table length (th
 
 // Device material record (128 B).  Derived quantities are computed on the
 // host exactly as the reference's constructors / Activate() do.
-constexpr int32_t kMatFlagDiffuse = 4;  // BSDF::IsDiffuse (pathEMS samples emitters only there)
+constexpr int32_t kMatFlagDiffuse = 4;  // BSDF::IsDiffuse (pathEMS / whitted sample emitters only there)
+constexpr int32_t kMatFlagFast = 8;     // roughDiffuse fastApprox
 
 struct alignas(16) DevMaterial {
-  int32_t type, dist, nested, flags;    // flags: 1 nonlinear, 2 anisotropic, 4 diffuse
+  int32_t type, dist, nested, flags;    // flags: 1 nonlinear, 2 anisotropic, 4 diffuse, 8 fast
   float a[4];                           // colour A (see include/hikari_amd.h)
   float b[4];                           // colour B
   float eta, inv_eta, inv_eta2, spec_weight;
   float alpha_u, alpha_v, thickness, fdr_int;  // fdr_int: 1 - internal diffuse transmittance
   float eta3[4], eta_k[4];              // conductor eta, k / extIOR
   int32_t rft_off;                      // offset of the reduced theta table (kRftThetaN floats) or -1
-  int32_t pad[3];
+  int32_t nested_back;                  // twoSided: lower-side material
+  int32_t pad[2];
 };
 static_assert(sizeof(DevMaterial) == 128, "DevMaterial layout");
 

@@ -16,23 +16,27 @@ from typing import List, Optional
 import numpy as np
 
 # Material type ids (record slot 0).  Parameter slots per type:
-#   [1] distribution (0 beckmann, 1 ggx)   [2] nested material index (coatings)
-#   [3] flags (1 nonlinear, 2 anisotropic) [4:7] colour A  [8:11] colour B
+#   [1] distribution (0 beckmann, 1 ggx)
+#   [2] nested material index (coatings; the upper side of a twoSided)
+#   [3] flags (1 nonlinear, 2 anisotropic, 4 fastApprox) [4:7] colour A  [8:11] colour B
+#   [12] lower-side material of a twoSided (-1: the upper one)
 #   [20] intIOR [21] extIOR [22] alpha / alphaU [23] alphaV [24] thickness
 # Colour A / B per type:
 #   diffuse: A albedo | conductor, roughConductor: A ks, B k
 #   dielectric, roughDielectric: A ksReflect, B ksRefract | plastic, roughPlastic: A ks, B kd
-#   coating, roughCoating: A sigmaA, B ks
-DIFFUSE, CONDUCTOR, DIELECTRIC, PLASTIC, ROUGH_CONDUCTOR, ROUGH_DIELECTRIC, ROUGH_PLASTIC, COATING, ROUGH_COATING = range(9)
+#   coating, roughCoating: A sigmaA, B ks | microfacet: A kd | roughDiffuse: A albedo
+(DIFFUSE, CONDUCTOR, DIELECTRIC, PLASTIC, ROUGH_CONDUCTOR, ROUGH_DIELECTRIC, ROUGH_PLASTIC, COATING, ROUGH_COATING,
+ MIRROR, MICROFACET, ROUGH_DIFFUSE, TWO_SIDED) = range(13)
 BECKMANN, GGX = 0, 1
 MAT_STRIDE = 32
 AIR = 1.000277
 
 
 def _rec(t, a=(0, 0, 0), b=(0, 0, 0), int_ior=0.0, ext_ior=0.0, alpha_u=0.0, alpha_v=0.0, thickness=0.0,
-         dist=BECKMANN, nested=-1, flags=0):
+         dist=BECKMANN, nested=-1, flags=0, nested_back=-1):
     r = np.zeros(MAT_STRIDE, np.float32)
     r[0], r[1], r[2], r[3] = t, dist, nested, flags
+    r[12] = nested_back
     r[4:7] = np.broadcast_to(np.asarray(a, np.float32), 3)
     r[8:11] = np.broadcast_to(np.asarray(b, np.float32), 3)
     r[20], r[21], r[22], r[23], r[24] = int_ior, ext_ior, alpha_u, alpha_v, thickness
@@ -82,6 +86,22 @@ def rough_coating(nested, alpha=0.1, dist=BECKMANN, int_ior=1.49, ext_ior=AIR, t
                 thickness=thickness, dist=dist, nested=nested)
 
 
+def mirror():
+    return _rec(MIRROR)
+
+
+def microfacet(alpha=0.1, int_ior=1.5046, ext_ior=AIR, kd=0.5):
+    return _rec(MICROFACET, a=kd, int_ior=int_ior, ext_ior=ext_ior, alpha_u=alpha, alpha_v=alpha)
+
+
+def rough_diffuse(albedo=0.5, alpha=0.2, fast_approx=False):
+    return _rec(ROUGH_DIFFUSE, a=albedo, alpha_u=alpha, alpha_v=alpha, flags=4 if fast_approx else 0)
+
+
+def two_sided(front, back=-1):
+    return _rec(TWO_SIDED, nested=front, nested_back=back)
+
+
 FILTER_BOX, FILTER_GAUSSIAN, FILTER_TENT, FILTER_MITCHELL = 0, 1, 2, 3
 
 
@@ -104,7 +124,7 @@ class SceneDesc(C.Structure):
                 ("max_depth", C.c_int32), ("background", C.c_float * 3), ("force_background", C.c_int32),
                 ("rft_dir", C.c_char_p), ("integrator", C.c_int32)]
 
-INTEGRATORS = {"pathMIS": 0, "pathEMS": 1, "pathMATS": 2}
+INTEGRATORS = {"pathMIS": 0, "pathEMS": 1, "pathMATS": 2, "whitted": 3}
 
 
 def lookat(origin, target, up):

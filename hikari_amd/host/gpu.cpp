@@ -1,7 +1,8 @@
 // gpu.cpp -- the GPU plugins: integrator `gpuPathMIS` (registered also as
-// `pathMIS`, the reference's name for the estimator it replaces) and the
-// accelerator `gpuBVH` (also `bvh`, `hlbvh`, `bruto`).  Both reach the
-// device through the C ABI of include/hikari_amd.h only.
+// `pathMIS`, the reference's name for the estimator it replaces; likewise
+// pathEMS, pathMATS and whitted) and the accelerator `gpuBVH` (also `bvh`,
+// `hlbvh`, `bruto`).  Both reach the device through the C ABI of
+// include/hikari_amd.h only.
 #include <cstring>
 #include <mutex>
 
@@ -36,13 +37,16 @@ namespace {
 
 // PathMISIntegrator::Li (src/integrator/PathMISIntegrator.cpp:16-157) for
 // whole images, on the wavefront kernels; the same class serves pathEMS
-// (PathEMSIntegrator.cpp) and pathMATS (PathMATSIntegrator.cpp), which the
-// bounce kernel selects by the scene's estimator id.  Per-ray Li() is not
+// (PathEMSIntegrator.cpp), pathMATS (PathMATSIntegrator.cpp) and whitted
+// (WhittedIntegrator.cpp), which the bounce kernel selects by the scene's
+// estimator id.  Per-ray Li() is not
 // offered: the estimators run only on the device.
 class GPUPathMISIntegrator : public Integrator {
  public:
+  // depth: required by the path integrators (PathMISIntegrator.cpp:13),
+  // optional for whitted (-1, no limit: WhittedIntegrator.cpp:13)
   explicit GPUPathMISIntegrator(const PropertyList &p, int estimator = 0) : estimator_(estimator) {
-    depth_ = p.GetInteger("depth");
+    depth_ = estimator == 3 ? p.GetInteger("depth", -1) : p.GetInteger("depth");
   }
   int EstimatorId() const override { return estimator_; }
   ~GPUPathMISIntegrator() override {
@@ -76,7 +80,7 @@ class GPUPathMISIntegrator : public Integrator {
     check_ctx(hk_render_pixel_samples(scene_, seed, x, y, 0, n, out.data()), "gpuPathMIS: pixel samples");
   }
   std::string ToString() const override {
-    static const char *names[] = {"pathMIS", "pathEMS", "pathMATS"};
+    static const char *names[] = {"pathMIS", "pathEMS", "pathMATS", "whitted"};
     return std::string("GPUPathIntegrator[") + names[estimator_] + ", depth = " + std::to_string(depth_) + "]";
   }
 
@@ -146,6 +150,9 @@ struct GPUPathEMSIntegrator : GPUPathMISIntegrator {
 struct GPUPathMATSIntegrator : GPUPathMISIntegrator {
   explicit GPUPathMATSIntegrator(const PropertyList &p) : GPUPathMISIntegrator(p, 2) {}
 };
+struct GPUWhittedIntegrator : GPUPathMISIntegrator {
+  explicit GPUWhittedIntegrator(const PropertyList &p) : GPUPathMISIntegrator(p, 3) {}
+};
 
 REGISTER_CLASS(GPUPathMISIntegrator, "gpuPathMIS")
 REGISTER_CLASS(GPUPathMISIntegrator, "pathMIS")
@@ -153,6 +160,8 @@ REGISTER_CLASS(GPUPathEMSIntegrator, "gpuPathEMS")
 REGISTER_CLASS(GPUPathEMSIntegrator, "pathEMS")
 REGISTER_CLASS(GPUPathMATSIntegrator, "gpuPathMATS")
 REGISTER_CLASS(GPUPathMATSIntegrator, "pathMATS")
+REGISTER_CLASS(GPUWhittedIntegrator, "gpuWhitted")
+REGISTER_CLASS(GPUWhittedIntegrator, "whitted")
 REGISTER_CLASS(GPUBVHAcceleration, "gpuBVH")
 REGISTER_CLASS(GPUBVHAcceleration, "bvh")
 REGISTER_CLASS(GPUBVHAcceleration, "hlbvh")

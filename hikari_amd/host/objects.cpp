@@ -26,7 +26,7 @@ namespace {
 constexpr float kAir = 1.000277f;
 
 enum RecType { kDiffuse = 0, kConductor, kDielectric, kPlastic, kRoughConductor, kRoughDielectric, kRoughPlastic,
-               kCoating, kRoughCoating };
+               kCoating, kRoughCoating, kMirror, kMicrofacet, kRoughDiffuse, kTwoSided };
 
 // One material record (HK_MATERIAL_STRIDE floats; layout in hikari_amd.h).
 struct Record {
@@ -35,6 +35,7 @@ struct Record {
     std::memset(r, 0, sizeof r);
     r[0] = float(type);
     r[2] = -1.0f;
+    r[12] = -1.0f;
   }
   void a(const Color3f &c) { r[4] = c.r; r[5] = c.g; r[6] = c.b; }
   void b(const Color3f &c) { r[8] = c.r; r[9] = c.g; r[10] = c.b; }
@@ -171,6 +172,63 @@ Object *make_rough_coating(const PropertyList &p) {
   return new MaterialBSDF("roughCoating", r, true);
 }
 
+Object *make_mirror(const PropertyList &) { return new MaterialBSDF("mirror", Record(kMirror), false); }
+Object *make_microfacet(const PropertyList &p) {  // MicrofacetBSDF.cpp:10-28
+  Record r(kMicrofacet);
+  r.r[22] = r.r[23] = p.GetFloat("alpha", 0.1f);
+  r.r[20] = p.GetFloat("intIOR", 1.5046f);
+  r.r[21] = p.GetFloat("extIOR", kAir);
+  r.a(p.GetColor("kd", Color3f(0.5f)));
+  return new MaterialBSDF("microfacet", r, false);
+}
+Object *make_rough_diffuse(const PropertyList &p) {  // RoughDiffuseBSDF.cpp:10-21
+  Record r(kRoughDiffuse);
+  r.a(p.GetColor("albedo", Color3f(0.5f)));
+  r.r[22] = r.r[23] = p.GetFloat("alpha", 0.2f);
+  r.r[3] = p.GetBoolean("fastApprox", false) ? 4.0f : 0.0f;
+  return new MaterialBSDF("roughDiffuse", r, false);
+}
+
+// TwoSidedBSDF (src/bsdf/TwoSidedBSDF.cpp): up to two nested BSDFs, upper side
+// first; a missing lower side repeats the upper one.  Children that are not
+// BSDFs are ignored, as in the reference's AddChild (:85-104).
+class TwoSidedBSDF : public BSDF {
+ public:
+  explicit TwoSidedBSDF(const PropertyList &) {}
+  ~TwoSidedBSDF() override {
+    delete side_[0];
+    delete side_[1];
+  }
+  void AddChild(Object *child, const std::string &) override {
+    if (child->GetClassType() != EClassType::EBSDF) return;
+    if (!side_[0]) side_[0] = static_cast<BSDF *>(child);
+    else if (!side_[1]) side_[1] = static_cast<BSDF *>(child);
+    else throw HikariException("TwoSidedBSDF: tried to specify the third nested BSDF");
+  }
+  void Activate() override {
+    if (!side_[0]) throw HikariException("TwoSidedBSDF needs at least one nested BSDF!");
+  }
+  int AppendRecords(std::vector<float> &records) const override {
+    Record r(kTwoSided);
+    r.r[2] = float(side_[0]->AppendRecords(records));
+    r.r[12] = side_[1] ? float(side_[1]->AppendRecords(records)) : -1.0f;
+    for (int k : {int(r.r[2]), int(r.r[12])}) {  // :113-122
+      if (k < 0) continue;
+      int t = int(records[size_t(k) * HK_MATERIAL_STRIDE]);
+      if (t == kCoating || t == kRoughCoating) t = int(records[size_t(records[size_t(k) * HK_MATERIAL_STRIDE + 2]) * HK_MATERIAL_STRIDE]);
+      if (t == kDielectric || t == kRoughDielectric) throw HikariException("Only BSDF without transmission can be nested!");
+    }
+    int idx = int(records.size() / HK_MATERIAL_STRIDE);
+    records.insert(records.end(), r.r, r.r + HK_MATERIAL_STRIDE);
+    return idx;
+  }
+  std::string ToString() const override { return "TwoSidedBSDF[]"; }
+
+ private:
+  BSDF *side_[2] = {nullptr, nullptr};
+};
+Object *make_two_sided(const PropertyList &p) { return new TwoSidedBSDF(p); }
+
 const bool kBsdfsRegistered = [] {
   ObjectFactory::RegisterClz("diffuse", make_diffuse);
   ObjectFactory::RegisterClz("conductor", make_conductor);
@@ -181,6 +239,10 @@ const bool kBsdfsRegistered = [] {
   ObjectFactory::RegisterClz("roughPlastic", make_rough_plastic);
   ObjectFactory::RegisterClz("coating", make_coating);
   ObjectFactory::RegisterClz("roughCoating", make_rough_coating);
+  ObjectFactory::RegisterClz("mirror", make_mirror);
+  ObjectFactory::RegisterClz("microfacet", make_microfacet);
+  ObjectFactory::RegisterClz("roughDiffuse", make_rough_diffuse);
+  ObjectFactory::RegisterClz("twoSided", make_two_sided);
   return true;
 }();
 

@@ -88,12 +88,17 @@ int hk_accel_intersect_device(hk_accel *accel, const hk_ray *d_rays, uint32_t n,
 /* ------------------------------------------------------------ BSDFs */
 /* Material records: 32 floats each.  [0] type (0 diffuse, 1 conductor,
  * 2 dielectric, 3 plastic, 4 roughConductor, 5 roughDielectric, 6 roughPlastic,
- * 7 coating, 8 roughCoating), [1] distribution (0 beckmann, 1 ggx), [2] nested
- * material index (coatings), [3] flags (1 nonlinear, 2 anisotropic),
- * [4..6] colour A, [8..10] colour B, [20] intIOR, [21] extIOR, [22] alpha or
- * alphaU, [23] alphaV, [24] thickness.  Colour A/B: diffuse albedo/-;
- * (rough)conductor ks/k; (rough)dielectric ksReflect/ksRefract; (rough)plastic
- * ks/kd; (rough)coating sigmaA/ks.  The XML properties of the src/bsdf classes. */
+ * 7 coating, 8 roughCoating, 9 mirror, 10 microfacet, 11 roughDiffuse,
+ * 12 twoSided), [1] distribution (0 beckmann, 1 ggx), [2] nested material
+ * index (coatings; a twoSided's upper side), [3] flags (1 nonlinear,
+ * 2 anisotropic, 4 fastApprox), [4..6] colour A, [8..10] colour B, [12] a
+ * twoSided's lower-side material index (-1: same as the upper), [20] intIOR,
+ * [21] extIOR, [22] alpha or alphaU, [23] alphaV, [24] thickness.  Colour A/B:
+ * diffuse albedo/-; (rough)conductor ks/k; (rough)dielectric
+ * ksReflect/ksRefract; (rough)plastic ks/kd; (rough)coating sigmaA/ks;
+ * microfacet kd/-; roughDiffuse albedo/-.  The XML properties of the src/bsdf
+ * classes.  Coatings nest one non-coating, non-twoSided BSDF; a twoSided nests
+ * BSDFs without transmission (TwoSidedBSDF.cpp:106-122). */
 #define HK_MATERIAL_STRIDE 32
 
 /* Batched BSDF::Eval / Pdf / Sample (include/core/BSDF.hpp:67-94) of
@@ -128,13 +133,15 @@ typedef struct hk_scene_desc {
   float filter_radius, filter_stddev, filter_b, filter_c;
   int32_t max_depth; float background[3]; int32_t force_background;
   const char *rft_dir;                          /* nullable */
-  int32_t integrator;                           /* 0 pathMIS, 1 pathEMS, 2 pathMATS */
+  int32_t integrator;                           /* 0 pathMIS, 1 pathEMS, 2 pathMATS, 3 whitted */
 } hk_scene_desc;
 
 /* Integrators (estimators of the bounce kernel): pathMIS
  * (src/integrator/PathMISIntegrator.cpp:16-157), pathEMS (emitter sampling
- * only, PathEMSIntegrator.cpp:16-119) and pathMATS (BSDF sampling only,
- * PathMATSIntegrator.cpp:16-87).  max_depth is clamped to 4095.
+ * only, PathEMSIntegrator.cpp:16-119), pathMATS (BSDF sampling only,
+ * PathMATSIntegrator.cpp:16-87) and whitted (WhittedIntegrator.cpp:26-116).
+ * max_depth: negative means unlimited (the reference reads it as uint32);
+ * clamped to 4095.
  *
  * Replaces Scene::Activate (src/core/Scene.cpp:128-165) plus the Activate()
  * of every mesh / BSDF / emitter / camera: uploads geometry, BVH, material

@@ -168,7 +168,19 @@ struct Frame {
 };
 template <typename R> inline R cos_theta(V3<R> v) { return v.z; }
 template <typename R> inline R sin_theta2(V3<R> v) { return R(1) - v.z * v.z; }
-template <typename R> inline R tan_theta(V3<R> v) {  // Frame.cpp:57-65
+template <typename R> inline R sin_theta(V3<R> v) {  // Frame.cpp:38-46
+  R tmp = sin_theta2(v);
+  return tmp <= R(0) ? R(0) : std::sqrt(tmp);
+}
+template <typename R> inline R sin_phi(V3<R> v) {  // Frame.cpp:68-76
+  R st = sin_theta(v);
+  return st == R(0) ? R(1) : clampr(v.y / st, R(-1), R(1));
+}
+template <typename R> inline R cos_phi(V3<R> v) {  // Frame.cpp:78-86
+  R st = sin_theta(v);
+  return st == R(0) ? R(1) : clampr(v.x / st, R(-1), R(1));
+}
+template <typename R> inline R tan_theta(V3<R> v) {  // Frame.cpp:48-56
   R tmp = R(1) - v.z * v.z;
   if (tmp <= R(0)) return R(0);
   return std::sqrt(tmp) / v.z;
@@ -261,6 +273,30 @@ inline V3<R> square_to_cosine_hemisphere(R sx, R sy) {
 template <typename R>
 inline R square_to_cosine_hemisphere_pdf(V3<R> v) {  // Sampling.cpp:109-112
   return (norm(v) - R(1) <= R(1e-6f) && v.z >= R(0)) ? v.z / R(float(kPi)) : R(0);
+}
+// Sampling.cpp:115-133: tan^2(theta) = -alpha^2 log(1 - x), phi = 2 pi y
+template <typename R>
+inline V3<R> square_to_beckmann(R sx, R sy, R alpha) {
+  R ln = std::log(R(1) - sx);
+  if (std::isnan(ln)) ln = R(1);
+  R tan2 = R(-1) * alpha * alpha * ln;
+  R c = R(1) / std::sqrt(R(1) + tan2);
+  R st = std::sqrt(R(1) - c * c);
+  R phi = R(2) * R(float(kPi)) * sy;
+  return {st * std::cos(phi), st * std::sin(phi), c};
+}
+// Sampling.cpp:135-159
+template <typename R>
+inline R square_to_beckmann_pdf(V3<R> m, R alpha) {
+  if (double(norm(m) - R(1)) <= 1e-6 && m.z >= R(0)) {
+    R c = cos_theta(m);
+    if (c == R(0)) return R(0);
+    R tt = tan_theta(m);
+    if (std::isinf(tt)) return R(0);
+    R a2 = alpha * alpha;
+    return std::exp(R(-1) * std::pow(tt, R(2)) / a2) / (R(float(kPi)) * a2 * std::pow(c, R(3)));
+  }
+  return R(0);
 }
 
 // ---------------------------------------------------------------- microfacet
@@ -527,7 +563,8 @@ struct Rft {
 // Flat material record shared with the HIP path (32 floats, see include/hikari_amd.h).
 enum MatType {
   kDiffuse = 0, kConductor = 1, kDielectric = 2, kPlastic = 3, kRoughConductor = 4,
-  kRoughDielectric = 5, kRoughPlastic = 6, kCoating = 7, kRoughCoating = 8
+  kRoughDielectric = 5, kRoughPlastic = 6, kCoating = 7, kRoughCoating = 8,
+  kMirror = 9, kMicrofacet = 10, kRoughDiffuse = 11, kTwoSided = 12
 };
 enum Measure { kUnknownMeasure = 0, kSolidAngle = 1, kDiscrete = 2 };
 
@@ -551,6 +588,8 @@ struct Material {
   int dist = kBeckmann;
   bool nonlinear = false, aniso = false;
   const Material *nested = nullptr;
+  const Material *nested_back = nullptr;  // twoSided: the BSDF seen from below
+  bool fast = false;                      // roughDiffuse fastApprox
   V3<R> a, b;  // colour parameters (see include/hikari_amd.h for their meaning per type)
   R int_ior = 0, ext_ior = 0, alpha_u = 0, alpha_v = 0, thickness = 0;
   // derived (constructor + Activate() of each class)
@@ -565,7 +604,7 @@ struct Material {
   void from_record(const float *r) {
     type = int(r[0]); dist = int(r[1]);
     int flags = int(r[3]);
-    nonlinear = (flags & 1) != 0; aniso = (flags & 2) != 0;
+    nonlinear = (flags & 1) != 0; aniso = (flags & 2) != 0; fast = (flags & 4) != 0;
     a = V3<R>(r[4], r[5], r[6]);
     b = V3<R>(r[8], r[9], r[10]);
     int_ior = r[20]; ext_ior = r[21]; alpha_u = r[22]; alpha_v = r[23]; thickness = r[24];
@@ -575,9 +614,11 @@ struct Material {
   // RoughPlasticBSDF.cpp:198; the coatings forward to their nested BSDF
   // (CoatingBSDF.cpp:194, RoughCoatingBSDF.cpp:223); false otherwise
   // (the BSDF base class).
+  // MicrofacetBSDF.cpp:167 true; TwoSidedBSDF.cpp:77-80 either side.
   bool is_diffuse() const {
-    if (type == kDiffuse || type == kPlastic || type == kRoughPlastic) return true;
+    if (type == kDiffuse || type == kPlastic || type == kRoughPlastic || type == kMicrofacet) return true;
     if ((type == kCoating || type == kRoughCoating) && nested) return nested->is_diffuse();
+    if (type == kTwoSided) return nested->is_diffuse() || nested_back->is_diffuse();
     return false;
   }
   // Constructor/Activate() derived quantities.
@@ -633,6 +674,13 @@ struct Material {
         }
         break;
       }
+      case kMicrofacet:  // MicrofacetBSDF.cpp:10-28: alpha clamped, eta = int/ext
+        alpha_u = alpha_v = clamp_alpha(alpha_u);
+        spec_weight = R(1) - std::max(a.x, std::max(a.y, a.z));  // Ks = 1 - max(kd)
+        break;
+      case kRoughDiffuse:  // RoughDiffuseBSDF.cpp:10-21
+        alpha_u = alpha_v = clamp_alpha(alpha_u);
+        break;
       default: break;
     }
   }
@@ -710,6 +758,37 @@ struct Material {
       case kRoughPlastic: return sample_rough_plastic(rec, sx, sy);
       case kCoating: return sample_coating(rec, sx, sy, u1);
       case kRoughCoating: return sample_rough_coating(rec, sx, sy, u1);
+      case kMirror: {  // MirrorBSDF.cpp:12-27
+        if (cos_theta(rec.wi) <= R(0)) return V3<R>(R(0));
+        rec.wo = reflect(rec.wi);
+        rec.measure = kDiscrete;
+        rec.eta = R(1);
+        return V3<R>(R(1));
+      }
+      case kMicrofacet: return sample_microfacet(rec, sx, sy);
+      case kRoughDiffuse: {  // RoughDiffuseBSDF.cpp:28-49
+        R ci = cos_theta(rec.wi);
+        rec.measure = kSolidAngle;
+        if (ci <= R(0)) return V3<R>(R(0));
+        rec.wo = square_to_cosine_hemisphere(sx, sy);
+        rec.eta = R(1);
+        R p = square_to_cosine_hemisphere_pdf(rec.wo);
+        if (p == R(0)) return V3<R>(R(0));
+        return eval_rough_diffuse(rec) / p;
+      }
+      case kTwoSided: {  // TwoSidedBSDF.cpp:21-43
+        bool flip = false;
+        if (cos_theta(rec.wi) < R(0)) {
+          rec.wi.z = -rec.wi.z;
+          flip = true;
+        }
+        V3<R> res = (flip ? nested_back : nested)->sample(rec, sx, sy, u1);
+        if (flip) {
+          rec.wi.z = -rec.wi.z;
+          if (!is_zero_fuzzy(res)) rec.wo.z = -rec.wo.z;
+        }
+        return res;
+      }
     }
     return V3<R>(R(0));
   }
@@ -770,6 +849,24 @@ struct Material {
       case kRoughPlastic: return eval_rough_plastic(rec);
       case kCoating: return eval_coating(rec);
       case kRoughCoating: return eval_rough_coating(rec);
+      case kMirror:  // MirrorBSDF.cpp:29-41
+        if (cos_theta(rec.wi) > R(0) && cos_theta(rec.wo) > R(0) && rec.measure == kDiscrete &&
+            std::abs(dot(reflect(rec.wi), rec.wo) - R(1)) <= R(1e-3f))
+          return V3<R>(R(1));
+        return V3<R>(R(0));
+      case kMicrofacet: {  // MicrofacetBSDF.cpp:96-135
+        if (rec.measure != kSolidAngle || cos_theta(rec.wi) <= R(0) || cos_theta(rec.wo) <= R(0))
+          return V3<R>(R(0));
+        return microfacet_value(rec);
+      }
+      case kRoughDiffuse: return eval_rough_diffuse(rec);
+      case kTwoSided: {  // TwoSidedBSDF.cpp:45-59
+        BRec<R> c = rec;
+        if (cos_theta(c.wi) > R(0)) return nested->eval(c);
+        c.wi.z = -c.wi.z;
+        c.wo.z = -c.wo.z;
+        return nested_back->eval(c);
+      }
     }
     return V3<R>(R(0));
   }
@@ -819,8 +916,118 @@ struct Material {
       case kRoughPlastic: return pdf_rough_plastic(rec);
       case kCoating: return pdf_coating(rec);
       case kRoughCoating: return pdf_rough_coating(rec);
+      case kMirror:  // MirrorBSDF.cpp:43-55
+        if (cos_theta(rec.wi) > R(0) && cos_theta(rec.wo) > R(0) && rec.measure == kDiscrete &&
+            std::abs(dot(reflect(rec.wi), rec.wo) - R(1)) <= R(1e-3f))
+          return R(1);
+        return R(0);
+      case kMicrofacet: {  // MicrofacetBSDF.cpp:137-165
+        if (rec.measure != kSolidAngle || cos_theta(rec.wi) <= R(0) || cos_theta(rec.wo) <= R(0)) return R(0);
+        R ks = spec_weight;
+        V3<R> wh = normalized(rec.wi + rec.wo);
+        R j = R(0.25f) / dot(wh, rec.wo);
+        R spec = ks * square_to_beckmann_pdf(wh, alpha_u) * j;
+        R diff = (R(1) - ks) * square_to_cosine_hemisphere_pdf(rec.wo);
+        return spec + diff;
+      }
+      case kRoughDiffuse: {  // RoughDiffuseBSDF.cpp:161-172
+        if (cos_theta(rec.wi) <= R(0) || cos_theta(rec.wo) <= R(0) || rec.measure != kSolidAngle) return R(0);
+        return square_to_cosine_hemisphere_pdf(rec.wo);
+      }
+      case kTwoSided: {  // TwoSidedBSDF.cpp:61-75
+        BRec<R> c = rec;
+        if (cos_theta(c.wi) > R(0)) return nested->pdf(c);
+        c.wi.z = -c.wi.z;
+        c.wo.z = -c.wo.z;
+        return nested_back->pdf(c);
+      }
     }
     return R(0);
+  }
+
+  // ---- MicrofacetBSDF (src/bsdf/MicrofacetBSDF.cpp): Beckmann specular lobe
+  // weighted by Ks = 1 - max(kd) over a Lambertian base.
+  static R beckmann_d(V3<R> m, R alpha) {  // :240-246
+    R expon = tan_theta(m) / alpha;
+    R c2 = cos_theta(m) * cos_theta(m);
+    return std::exp(-expon * expon) / (R(float(kPi)) * alpha * alpha * c2 * c2);
+  }
+  static R smith_beckmann_g1(V3<R> v, V3<R> m, R alpha) {  // :248-272
+    R tt = tan_theta(v);
+    if (tt == R(0)) return R(1);
+    if (dot(m, v) * cos_theta(v) <= R(0)) return R(0);
+    R bb = R(1) / (alpha * tt);
+    if (bb >= R(1.6f)) return R(1);
+    R b2 = bb * bb;
+    return (R(3.535f) * bb + R(2.181f) * b2) / (R(1) + R(2.276f) * bb + R(2.577f) * b2);
+  }
+  // Ks F D G / (4 cos_i cos_o) (zeroed when not a valid colour) + kd / pi.
+  V3<R> microfacet_value(const BRec<R> &rec) const {
+    R alpha = alpha_u, ks = spec_weight, ct;
+    V3<R> wh = normalized(rec.wi + rec.wo);
+    R d = beckmann_d(wh, alpha);
+    R f = fresnel_dielectric(dot(wh, rec.wi), eta, inv_eta, ct);
+    R g = smith_beckmann_g1(rec.wi, wh, alpha) * smith_beckmann_g1(rec.wo, wh, alpha);
+    R spec = ks * f * d * g / (R(4) * cos_theta(rec.wi) * cos_theta(rec.wo));
+    if (spec < R(0) || !std::isfinite(spec)) spec = R(0);
+    return V3<R>(spec) + a / R(float(kPi));
+  }
+  V3<R> sample_microfacet(BRec<R> &rec, R sx, R sy) const {  // :34-94
+    if (cos_theta(rec.wi) <= R(0)) return V3<R>(R(0));
+    rec.measure = kSolidAngle;
+    R ks = spec_weight;
+    if (sx < ks) {
+      V3<R> wh = square_to_beckmann(sx / ks, sy, alpha_u);
+      rec.wo = reflect(rec.wi, wh);
+    } else {
+      rec.wo = square_to_cosine_hemisphere((sx - ks) / (R(1) - ks), sy);
+    }
+    rec.eta = R(1);
+    R p = pdf(rec);
+    if (p == R(0)) return V3<R>(R(0));
+    return microfacet_value(rec) * cos_theta(rec.wo) / p;
+  }
+
+  // ---- RoughDiffuseBSDF::Eval (src/bsdf/RoughDiffuseBSDF.cpp:51-159): the
+  // Oren-Nayar model; the returned value already carries the cos(theta_o)
+  // factor (the integrators multiply by |cos theta_o| once more).
+  V3<R> eval_rough_diffuse(const BRec<R> &rec) const {
+    R ci = cos_theta(rec.wi), co = cos_theta(rec.wo);
+    if (ci <= R(0) || co <= R(0) || rec.measure != kSolidAngle) return V3<R>(R(0));
+    R sigma = alpha_u * R(0.70710678118655f);
+    R s2 = sigma * sigma;
+    R sti = sin_theta(rec.wi), sto = sin_theta(rec.wo);
+    R cpd = R(0);
+    if (sti > R(1e-4f) && sto > R(1e-4f))
+      cpd = cos_phi(rec.wi) * cos_phi(rec.wo) + sin_phi(rec.wi) * sin_phi(rec.wo);
+    const R inv_pi = R(float(kInvPi));
+    if (fast) {
+      R A = R(1) - R(0.5f) * s2 / (s2 + R(0.33f));
+      R B = R(0.45f) * s2 / (s2 + R(0.09f));
+      R sa, tb;
+      if (ci > co) { sa = sto; tb = sti / ci; }
+      else { sa = sti; tb = sto / co; }
+      return a * (inv_pi * co * (A + B * std::max(cpd, R(0)) * sa * tb));
+    }
+    R ti = std::acos(clampr(ci, R(-1), R(1))), to = std::acos(clampr(co, R(-1), R(1)));
+    R al = std::max(ti, to), be = std::min(ti, to);
+    R sa, sb, tb;
+    if (ci > co) { sa = sto; sb = sti; tb = sti / ci; }
+    else { sa = sti; sb = sto; tb = sto / co; }
+    R t1 = s2 / (s2 + R(0.09f));
+    R t2 = R(float(4.0 * kInvPi * kInvPi)) * al * be;
+    R t3 = R(2) * be * inv_pi;
+    R c1 = R(1) - R(0.5f) * s2 / (s2 + R(0.33f));
+    R c2 = R(0.45f) * t1;
+    R c3 = R(0.125f) * t1 * t2 * t2;
+    R c4 = R(0.17f) * s2 / (s2 + R(0.13f));
+    if (cpd > R(0)) c2 *= sa;
+    else c2 *= sa - t3 * t3 * t3;
+    R th = (sa + sb) / (std::sqrt(std::max(R(0), R(1) - sa * sa)) + std::sqrt(std::max(R(0), R(1) - sb * sb)));
+    if (std::isinf(th) || std::isnan(th)) return V3<R>(R(0));
+    V3<R> lr1 = a * (c1 + cpd * c2 * tb + (R(1) - std::abs(cpd)) * c3 * th);
+    V3<R> lr2 = a * a * (c4 * (R(1) - cpd * t3 * t3));
+    return (lr1 + lr2) * (inv_pi * co);
   }
 
   // src/bsdf/RoughDielectricBSDF.cpp:57-157
@@ -1140,5 +1347,34 @@ struct Material {
     return res;
   }
 };
+
+// Links nested BSDFs by record index: [2] the nested BSDF of a coating, or
+// the upper-side BSDF of a twoSided; [12] a twoSided's lower-side BSDF (-1:
+// the same as the upper, TwoSidedBSDF.cpp:106-112).  A twoSided may not nest
+// a transmissive BSDF (:113-122) or another twoSided.
+template <typename R>
+inline void link_materials(std::vector<Material<R>> &m, const float *recs) {
+  int n = int(m.size());
+  auto at = [&](int i, int k) -> const Material<R> * {
+    if (k < 0 || k >= n) throw std::runtime_error("nested BSDF index out of range");
+    return &m[k];
+  };
+  for (int i = 0; i < n; i++) {
+    int k = int(recs[i * 32 + 2]), kb = int(recs[i * 32 + 12]);
+    if (m[i].type == kCoating || m[i].type == kRoughCoating) {
+      m[i].nested = at(i, k);
+    } else if (m[i].type == kTwoSided) {
+      m[i].nested = at(i, k);
+      m[i].nested_back = kb < 0 ? m[i].nested : at(i, kb);
+      for (int c : {k, kb < 0 ? k : kb}) {
+        int t = m[c].type;
+        if (t == kTwoSided) throw std::runtime_error("twoSided cannot nest a twoSided BSDF");
+        if (t == kCoating || t == kRoughCoating) t = m[at(c, int(recs[c * 32 + 2])) - m.data()].type;
+        if (t == kDielectric || t == kRoughDielectric)
+          throw std::runtime_error("Only BSDF without transmission can be nested!");
+      }
+    }
+  }
+}
 
 }  // namespace hko

@@ -342,7 +342,7 @@ struct SceneIn {
   float background[3] = {0, 0, 0};
   int force_background = 0;
   const char *rft_dir = nullptr;
-  int integrator = 0;  // 0 pathMIS, 1 pathEMS, 2 pathMATS
+  int integrator = 0;  // 0 pathMIS, 1 pathEMS, 2 pathMATS, 3 whitted
 };
 
 template <typename R>
@@ -381,7 +381,7 @@ struct Scene {
   float filter_table[33];
   float filter_radius = 0.5f, lookup = 64;
   int max_depth = 1000;
-  int integrator = 0;  // 0 pathMIS, 1 pathEMS, 2 pathMATS
+  int integrator = 0;  // 0 pathMIS, 1 pathEMS, 2 pathMATS, 3 whitted
   V3<R> background;
   bool force_background = false;
   double bbox_lo[3], bbox_hi[3];
@@ -403,14 +403,8 @@ struct Scene {
     }
     mats.resize(in.n_materials);
     for (int i = 0; i < in.n_materials; i++) mats[i].from_record(in.materials + i * 32);
-    for (int i = 0; i < in.n_materials; i++) {
-      int nested = int(in.materials[i * 32 + 2]);
-      if (mats[i].type == kCoating || mats[i].type == kRoughCoating) {
-        if (nested < 0 || nested >= in.n_materials) throw std::runtime_error("coating needs a nested BSDF");
-        mats[i].nested = &mats[nested];
-      }
-      mats[i].activate(&beck, &ggx, rft_pool);
-    }
+    link_materials(mats, in.materials);
+    for (int i = 0; i < in.n_materials; i++) mats[i].activate(&beck, &ggx, rft_pool);
     // meshes (src/core/Mesh.cpp:233-261 Activate, Scene.cpp:128-165)
     for (int a = 0; a < 3; a++) { bbox_lo[a] = 1e300; bbox_hi[a] = -1e300; }
     float flo[3] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity()};
@@ -512,7 +506,10 @@ struct Scene {
     }
     filter_table[32] = 0.0f;
     lookup = 32 / filter_radius;
-    max_depth = in.max_depth;
+    // the depth property is read as uint32 (PathMISIntegrator.cpp:13,
+    // WhittedIntegrator.cpp:13): a negative value means no limit; paths are
+    // capped at 4095 vertices as on the device
+    max_depth = in.max_depth < 0 ? 4095 : std::min(in.max_depth, 4095);
     integrator = in.integrator;
     background = V3<R>(in.background[0], in.background[1], in.background[2]);
     force_background = in.force_background != 0;
@@ -877,6 +874,80 @@ V3<R> path_ems_li(const Scene<R> &sc, Pcg32 &rng, V3<R> ro, V3<R> rd, R rtmin, R
   return li;
 }
 
+// src/integrator/WhittedIntegrator.cpp:26-116 (LiRecursive), unrolled: every
+// vertex adds its emission; a diffuse vertex (BSDF::IsDiffuse) adds one
+// emitter sample per emitter (its own emitter contributes
+// Eval / Pdf |cos| Le along the shading normal, :66-76) and ends the path; any
+// other vertex continues with probability 0.95 (drawn before the BSDF sample)
+// while depth < the depth property.  The reference traces a shadow ray for
+// every emitter sample; this restatement traces it only when the sample's
+// radiance is not exactly zero (the contribution is then exactly zero).
+template <typename R>
+V3<R> whitted_li(const Scene<R> &sc, Pcg32 &rng, V3<R> ro, V3<R> rd, R rtmin, R rtmax, PathStats *st) {
+  using Isect = typename Scene<R>::Isect;
+  using ERec = typename Scene<R>::ERec;
+  PcgU1 u1(&rng);
+  V3<R> li(R(0)), beta(R(1));
+  int depth = 0;
+  V3<R> to = ro, td = rd;
+  R ttmin = rtmin, ttmax = rtmax;
+  for (;;) {
+    Isect is;
+    if (st) st->ext++;
+    if (!sc.intersect(Scene<R>::to_rayf(to, td, ttmin, ttmax), is)) {  // :33-52
+      if (sc.has_env && !sc.force_background) li = li + beta * sc.env_eval(td);
+      else if (depth == 0) return sc.background;
+      break;
+    }
+    const auto &mesh = sc.meshes[is.mesh];
+    const Material<R> &bsdf = sc.mats[mesh.material];
+    V3<R> le(R(0));
+    if (mesh.emitter >= 0) {  // :54-62
+      ERec er;
+      er.ref = to; er.p = is.p; er.n = is.sh.n; er.wi = td;
+      le = sc.area_eval(mesh, er);
+      li = li + beta * le;
+    }
+    if (bsdf.is_diffuse()) {  // :64-98
+      for (int e = 0; e < int(sc.emitters.size()); e++) {
+        const auto &em = sc.emitters[e];
+        BRec<R> br;
+        br.wi = is.sh.to_local(R(-1) * td);
+        br.measure = kSolidAngle;
+        if (em.kind == 0 && em.mesh == int(is.mesh)) {
+          br.wo = is.sh.to_local(is.sh.n);
+          R p = bsdf.pdf(br);
+          if (p != R(0)) li = li + beta * (bsdf.eval(br) / p * std::abs(cos_theta(br.wo)) * le);
+          continue;
+        }
+        ERec er;
+        er.ref = is.p;
+        if (em.kind == 1) er.dist = sc.bbox_radius;
+        R sx = rng.next_float(), sy = rng.next_float(), s1 = rng.next_float();
+        V3<R> ld = em.kind == 0 ? sc.area_sample(sc.meshes[em.mesh], er, sx, sy, s1) : sc.env_sample(er, sx, sy);
+        if (ld.x == R(0) && ld.y == R(0) && ld.z == R(0)) continue;
+        if (st) st->shadow++;
+        RayF sr = Scene<R>::to_rayf(is.p, er.p - is.p, R(1e-4f), R(1.0f - 1e-4f));
+        if (!sc.occluded(sr)) {
+          br.wo = is.sh.to_local(er.wi);
+          li = li + beta * bsdf.eval(br) * std::abs(cos_theta(br.wo)) * ld;
+        }
+      }
+      break;
+    }
+    if (!(rng.next_float() < 0.95f && depth < sc.max_depth)) break;  // :100-113
+    BRec<R> br;
+    br.wi = is.sh.to_local(R(-1) * td);
+    R sx = rng.next_float(), sy = rng.next_float();
+    V3<R> c = bsdf.sample(br, sx, sy, &u1);
+    beta = beta * c * R(1.0f / 0.95f);
+    if (beta.x == R(0) && beta.y == R(0) && beta.z == R(0)) break;
+    to = is.p; td = is.sh.to_world(br.wo); ttmin = R(1e-4f); ttmax = std::numeric_limits<R>::infinity();
+    depth++;
+  }
+  return li;
+}
+
 // ---------------------------------------------------------------- film + driver
 // ImageBlock::Put (src/core/Block.cpp:95-134) on one image-sized block:
 // pixel centres at +0.5, tabulated filter, invalid samples dropped.
@@ -923,6 +994,7 @@ V3<R> render_sample(const Scene<R> &sc, uint32_t req_seed, uint32_t x, uint32_t 
   if (st) st->paths++;
   if (sc.integrator == 1) return path_ems_li(sc, rng, o, d, tmin, tmax, st);
   if (sc.integrator == 2) return path_mats_li(sc, rng, o, d, tmin, tmax, st);
+  if (sc.integrator == 3) return whitted_li(sc, rng, o, d, tmin, tmax, st);
   return path_mis_li(sc, rng, o, d, tmin, tmax, st);
 }
 

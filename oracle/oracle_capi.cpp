@@ -136,11 +136,8 @@ static int bsdf_batch(const float *mats, int nmat, int idx, const char *rft_dir,
   std::vector<Material<R>> m(nmat);
   std::vector<Rft<R> *> pool;
   for (int i = 0; i < nmat; i++) m[i].from_record(mats + 32 * i);
-  for (int i = 0; i < nmat; i++) {
-    int ne = int(mats[32 * i + 2]);
-    if (m[i].type == kCoating || m[i].type == kRoughCoating) m[i].nested = &m[ne];
-    m[i].activate(&beck, &ggx, pool);
-  }
+  link_materials(m, mats);
+  for (int i = 0; i < nmat; i++) m[i].activate(&beck, &ggx, pool);
   const Material<R> &mat = m[idx];
   for (int i = 0; i < n; i++) {
     BRec<R> rec;

@@ -21,6 +21,14 @@ def material_table():
         D.rough_coating(0, alpha=0.2, dist=D.GGX, sigma_a=(0.1, 0.2, 0.3), thickness=1.0),  # 11
         D.rough_conductor(alpha=0.1, dist=D.GGX, alpha_v=0.4, k=(3.9, 2.4, 2.1)),  # 12 anisotropic
         D.coating(0, int_ior=1.49, sigma_a=(0.2, 0.1, 0.05)),                   # 13 coated diffuse
+        D.mirror(),                                                             # 14
+        D.microfacet(alpha=0.1, int_ior=1.5, kd=(0.1, 0.2, 0.15)),              # 15 (ttest-microfacet.xml)
+        D.microfacet(alpha=0.005, kd=(0.0175, 0.0225, 0.0325)),                 # 16 (veach_mi plate 1)
+        D.rough_diffuse(albedo=(0.6, 0.4, 0.3), alpha=0.5),                     # 17
+        D.rough_diffuse(albedo=0.7, alpha=0.8, fast_approx=True),               # 18
+        D.two_sided(0, 17),                                                     # 19 diffuse above, rough diffuse below
+        D.two_sided(10),                                                        # 20 coated rough conductor, both sides
+        D.microfacet(alpha=0.3, kd=0.0),                                        # 21 kd = 0: specular lobe only
     ]
     return np.stack(recs).astype(np.float32)
 
@@ -124,3 +132,27 @@ def env_plane_expectation(albedo=0.5, env_h=32, radiance=1.0):
             return 2 * np.cos(t) * np.sin(t) * pl / (pl + pb)
         total += quad(g, a, b, epsabs=1e-12, epsrel=1e-12)[0]
     return albedo * radiance * (1.0 + total)
+
+
+def materials_box(width=48, height=48, integrator="pathMIS"):
+    """The config-0 box with the second group of BSDFs: a twoSided floor
+    (rough diffuse above, diffuse below), a microfacet back wall, a fast
+    Oren-Nayar left wall, a mirror sphere and a near-specular microfacet
+    sphere (alpha = 0.005)."""
+    from hikari_amd import desc as D
+    from hikari_amd import scenes as S
+    sc = S.cbox(width=width, height=height, sphere_subdiv=2)
+    sc.integrator = integrator
+    if integrator == "whitted":
+        sc.max_depth = -1
+    rd = sc.add_material(D.rough_diffuse(albedo=(0.7, 0.65, 0.6), alpha=0.5))
+    dif = sc.add_material(D.diffuse(0.4))
+    floor = sc.add_material(D.two_sided(rd, dif))
+    back = sc.add_material(D.microfacet(alpha=0.1, int_ior=1.5, kd=(0.3, 0.3, 0.35)))
+    left = sc.add_material(D.rough_diffuse(albedo=(0.63, 0.065, 0.05), alpha=0.8, fast_approx=True))
+    mir = sc.add_material(D.mirror())
+    shiny = sc.add_material(D.microfacet(alpha=0.005, kd=(0.05, 0.1, 0.2)))
+    # mesh order of scenes.cbox: floor, ceiling, back, left, right, front, light, sphere, sphere
+    for k, m in ((0, floor), (2, back), (3, left), (7, mir), (8, shiny)):
+        sc.meshes[k].material = m
+    return sc

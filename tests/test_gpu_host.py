@@ -27,10 +27,12 @@ def run(*args):
     return r
 
 
-@pytest.mark.parametrize("name,total", [("test-direct.xml", 15), ("test-furnace.xml", 6)])
+@pytest.mark.parametrize("name,total", [("test-direct.xml", 15), ("test-furnace.xml", 6),
+                                        ("ttest-microfacet.xml", 5)])
 def test_reference_ttest_files(name, total):
     """scene/pa5/tests/*.xml through the C++ harness (src/core/StudentTTest.cpp
-    semantics): every scene, pathMIS, pathEMS and pathMATS, passes."""
+    semantics): every scene (pathMIS, pathEMS, pathMATS) and every incidence
+    angle of the microfacet BSDF test passes."""
     r = run(os.path.join(GOLDEN, name))
     assert r.returncode == 0 and f"Passed {total}/{total} tests." in r.stdout
 
@@ -70,3 +72,26 @@ def test_estimator_paths_track_oracle(ctx, integrator):
         close = np.all(np.abs(g[:, :3] - o[:, :3]) <= 1e-4 * np.maximum(1.0, np.abs(o[:, :3])), axis=1)
         print(integrator, "identical paths:", close.mean())
         assert close.mean() > 0.99
+
+
+@pytest.mark.parametrize("kind", ["mis", "ems", "mats"])
+def test_veach_matches_reference_render(tmp_path, kind):
+    """scene/pa5/veach_mi/veach-<kind>.xml (microfacet plates of four
+    roughnesses, five spherical lights, pathMIS / pathEMS / pathMATS) rendered
+    by hikari_render against 16x16 block means of the reference's own
+    veach-mats.exr (pathMATS, 256 spp, its own random stream; the fixture is
+    written by tests/golden/make_veach_fixture.py)."""
+    out = tmp_path / f"{kind}.pfm"
+    spp = {"mis": "256", "ems": "512", "mats": "1024"}[kind]
+    r = run(os.path.join(GOLDEN, "veach_mi", f"veach-{kind}.xml"), "--spp", spp, "-o", str(out))
+    assert r.returncode == 0
+    img = read_pfm(out)
+    assert img.shape == (512, 768, 3)
+    ref = np.load(os.path.join(GOLDEN, "veach_mi", "veach-mats-blocks16.npy"))
+    blk = img.reshape(32, 16, 48, 16, 3).mean(axis=(1, 3))
+    lb, lr = lum(blk), lum(ref)
+    rel = np.abs(lb - lr) / (lr + 0.05)
+    print(kind, "mean", lb.mean(), lr.mean(), "block rel median", np.median(rel), "p90", np.quantile(rel, 0.9),
+          "max", rel.max())
+    assert abs(lb.mean() / lr.mean() - 1) < 0.02
+    assert np.median(rel) < 0.03

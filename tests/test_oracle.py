@@ -172,7 +172,9 @@ def _chi2_sampling(index, wi, n=200000, res=10, seed=0, glossy_discrete=False):
 
 
 @pytest.mark.parametrize("index,wi", [(0, (0.48, 0.0, 0.877)), (4, (0.48, 0.0, 0.877)), (5, (0.48, 0.0, 0.877)),
-                                      (6, (0.48, 0.0, 0.877)), (8, (0.48, 0.0, 0.877)), (12, (0.48, 0.0, 0.877))])
+                                      (6, (0.48, 0.0, 0.877)), (8, (0.48, 0.0, 0.877)), (12, (0.48, 0.0, 0.877)),
+                                      (15, (0.48, 0.0, 0.877)), (15, (0.94, 0.0, 0.342)), (17, (0.48, 0.0, 0.877)),
+                                      (18, (0.7, 0.3, 0.648)), (19, (0.48, 0.0, -0.877)), (21, (0.6, 0.0, 0.8))])
 def test_bsdf_sampling_chi2(index, wi):
     """Sample() draws directions with the density Pdf() reports (significance 0.01 / 5 tests).
 
@@ -188,7 +190,8 @@ def test_bsdf_sampling_chi2(index, wi):
     assert p > 0.01 / 5, (chi2, dof, p)
 
 
-@pytest.mark.parametrize("index,cos_in_eval", [(0, False), (4, True), (5, True), (12, True)])
+@pytest.mark.parametrize("index,cos_in_eval", [(0, False), (4, True), (5, True), (12, True), (15, False),
+                                                (17, True), (18, True), (19, False), (21, False)])
 def test_sample_weight_is_eval_over_pdf(index, cos_in_eval):
     """Sample() returns Eval / Pdf of the sampled direction, with the cosine convention of each class.
 
@@ -207,6 +210,125 @@ def test_sample_weight_is_eval_over_pdf(index, cos_in_eval):
     cos = np.ones((good.sum(), 1)) if cos_in_eval else np.abs(r["wo"][good, 2:3])
     expect = q["eval"][good] * cos / q["pdf"][good, None]
     assert np.allclose(r["f"][good], expect, rtol=2e-3, atol=1e-6)
+
+
+def test_mirror_closed_form():
+    """MirrorBSDF: wo = (-x, -y, z), weight 1, discrete; Eval = Pdf = 1 only there and only for the
+    discrete measure; nothing from below."""
+    rng = np.random.default_rng(3)
+    mats = material_table()
+    n = 2000
+    wi = random_dirs(rng, n)
+    r = O.bsdf_batch(mats, 14, wi, wi, np.full(n, 2, np.int32), rng.uniform(size=(n, 4)).astype(np.float32))
+    up = wi[:, 2] > 0
+    assert np.array_equal(r["wo"][up], wi[up] * np.array([-1, -1, 1], np.float32))
+    assert np.all(r["f"][up] == 1) and np.all(r["f"][~up] == 0) and np.all(r["measure"][up] == 2)
+    refl = wi * np.array([-1, -1, 1], np.float32)
+    q = O.bsdf_batch(mats, 14, wi, refl, np.full(n, 2, np.int32), np.zeros((n, 4), np.float32))
+    assert np.all(q["eval"][up] == 1) and np.all(q["pdf"][up] == 1) and np.all(q["pdf"][~up] == 0)
+    q = O.bsdf_batch(mats, 14, wi, refl, np.full(n, 1, np.int32), np.zeros((n, 4), np.float32))
+    assert np.all(q["eval"] == 0) and np.all(q["pdf"] == 0)
+
+
+def test_microfacet_ttest_references():
+    """scene/pa5/tests/ttest-microfacet.xml: mean luminance of Sample() at five incidence angles
+    against the file's reference albedos (Student's t-test, Sidak-corrected over the five, as
+    src/core/StudentTTest.cpp:86-133 does, 100k samples each)."""
+    from scipy import stats
+    angles = [0, 45, 60, 80, 85]
+    refs = [0.207067, 0.215733, 0.247884, 0.430936, 0.519016]
+    mats = np.stack([D.microfacet(alpha=0.1, int_ior=1.5, ext_ior=1.000277, kd=(0.1, 0.2, 0.15))])
+    rng = np.random.default_rng(7)
+    n = 100000
+    alpha = 1 - (1 - 0.01) ** (1 / len(refs))
+    for ang, ref in zip(angles, refs):
+        t = np.radians(ang)
+        wi = np.repeat(np.array([[np.sin(t), 0, np.cos(t)]], np.float32), n, 0)
+        r = O.bsdf_batch(mats, 0, wi, wi, np.ones(n, np.int32), rng.uniform(size=(n, 4)).astype(np.float32))
+        L = lum(r["f"])
+        tt = (L.mean() - ref) / (L.std(ddof=1) / np.sqrt(n))
+        p = 2 * (1 - stats.t.cdf(abs(tt), n - 1))
+        assert p > alpha, (ang, ref, L.mean(), p)
+
+
+@pytest.mark.parametrize("ang", [0.0, 40.0, 70.0])
+def test_microfacet_smooth_limit_is_fresnel(ang):
+    """kd = 0, alpha = 0.005: the Beckmann lobe is nearly a mirror, so the mean Sample() weight is the
+    dielectric Fresnel reflectance at the incidence angle (closed form, within 1%)."""
+    mats = np.stack([D.microfacet(alpha=0.005, int_ior=1.5, ext_ior=1.0, kd=0.0)])
+    rng = np.random.default_rng(8)
+    n = 50000
+    t = np.radians(ang)
+    wi = np.repeat(np.array([[np.sin(t), 0, np.cos(t)]], np.float32), n, 0)
+    r = O.bsdf_batch(mats, 0, wi, wi, np.ones(n, np.int32), rng.uniform(size=(n, 4)).astype(np.float32))
+    eta = 1.5
+    ci = np.cos(t)
+    ct = np.sqrt(1 - (1 - ci ** 2) / eta ** 2)
+    rs = (ci - eta * ct) / (ci + eta * ct)
+    rp = (eta * ci - ct) / (eta * ci + ct)
+    fr = 0.5 * (rs ** 2 + rp ** 2)
+    assert abs(r["f"][:, 0].mean() / fr - 1) < 0.01, (r["f"][:, 0].mean(), fr)
+
+
+def _oren_nayar_textbook(wi, wo, rho, sigma, full=True):
+    """Oren & Nayar (1994), 'Generalization of Lambert's reflectance model': the full model (direct
+    term C1..C3 plus the 0.17 interreflection term) and the qualitative A + B form, written from the
+    angles themselves (theta by arccos, phi by atan2, tan((alpha + beta) / 2) directly).  rho: RGB."""
+    ti, to = np.arccos(wi[:, 2:3]), np.arccos(wo[:, 2:3])
+    cpd = np.cos(np.arctan2(wi[:, 1:2], wi[:, 0:1]) - np.arctan2(wo[:, 1:2], wo[:, 0:1]))
+    al, be = np.maximum(ti, to), np.minimum(ti, to)
+    s2 = sigma ** 2
+    rho = np.asarray(rho, np.float64)[None, :]
+    if not full:
+        A = 1 - 0.5 * s2 / (s2 + 0.33)
+        B = 0.45 * s2 / (s2 + 0.09)
+        return rho / np.pi * (A + B * np.maximum(cpd, 0) * np.sin(al) * np.tan(be)) * wo[:, 2:3]
+    c1 = 1 - 0.5 * s2 / (s2 + 0.33)
+    c2 = 0.45 * s2 / (s2 + 0.09) * np.where(cpd >= 0, np.sin(al), np.sin(al) - (2 * be / np.pi) ** 3)
+    c3 = 0.125 * s2 / (s2 + 0.09) * (4 * al * be / np.pi ** 2) ** 2
+    l1 = rho / np.pi * (c1 + cpd * c2 * np.tan(be) + (1 - np.abs(cpd)) * c3 * np.tan((al + be) / 2))
+    l2 = 0.17 * rho ** 2 / np.pi * s2 / (s2 + 0.13) * (1 - cpd * (2 * be / np.pi) ** 2)
+    return (l1 + l2) * wo[:, 2:3]
+
+
+@pytest.mark.parametrize("index,full", [(17, True), (18, False)])
+def test_rough_diffuse_is_oren_nayar(index, full):
+    """RoughDiffuseBSDF::Eval equals the published Oren-Nayar model (sigma = alpha / sqrt 2) times
+    cos(theta_o); with alpha at its floor it is Lambertian, albedo / pi * cos(theta_o), to 1e-4."""
+    rng = np.random.default_rng(index)
+    mats = material_table()
+    n = 4000
+    wi = random_dirs(rng, n, upper=True)
+    wo = random_dirs(rng, n, upper=True)
+    keep = (wi[:, 2] > 0.02) & (wo[:, 2] > 0.02)
+    wi, wo = wi[keep], wo[keep]
+    r = O.bsdf_batch(mats, index, wi, wo, np.ones(len(wi), np.int32), np.zeros((len(wi), 4), np.float32))
+    rec = mats[index]
+    expect = _oren_nayar_textbook(wi.astype(np.float64), wo.astype(np.float64), rec[4:7], rec[22] / np.sqrt(2), full)
+    assert np.allclose(r["eval"], expect, rtol=1e-5, atol=1e-9)
+    lam = np.stack([D.rough_diffuse(albedo=(0.6, 0.4, 0.3), alpha=0.0)])
+    q = O.bsdf_batch(lam, 0, wi, wo, np.ones(len(wi), np.int32), np.zeros((len(wi), 4), np.float32))
+    # alpha floors at 5e-4 (sigma^2 = 1.25e-7): Lambertian up to C2 tan(beta) ~ 6e-7 * 50
+    assert np.allclose(q["eval"], np.array([0.6, 0.4, 0.3]) / np.pi * wo[:, 2:3], rtol=1e-4)
+
+
+def test_two_sided_mirrors_the_lower_side():
+    """TwoSidedBSDF: from below it is its second BSDF seen through z -> -z (Eval, Pdf and Sample,
+    with the sampled direction mirrored back); from above, its first."""
+    rng = np.random.default_rng(19)
+    mats = material_table()
+    n = 3000
+    wi = random_dirs(rng, n)
+    wo = random_dirs(rng, n)
+    u = rng.uniform(size=(n, 4)).astype(np.float32)
+    flip = np.array([1, 1, -1], np.float32)
+    r = O.bsdf_batch(mats, 19, wi, wo, np.ones(n, np.int32), u)
+    lo = wi[:, 2] < 0
+    a = O.bsdf_batch(mats, 0, wi, wo, np.ones(n, np.int32), u)
+    b = O.bsdf_batch(mats, 17, wi * flip, wo * flip, np.ones(n, np.int32), u)
+    assert np.array_equal(r["eval"][~lo], a["eval"][~lo]) and np.array_equal(r["eval"][lo], b["eval"][lo])
+    assert np.array_equal(r["pdf"][~lo], a["pdf"][~lo]) and np.array_equal(r["pdf"][lo], b["pdf"][lo])
+    assert np.array_equal(r["f"][lo], b["f"][lo]) and np.array_equal(r["wo"][lo], b["wo"][lo] * flip)
 
 
 # ----------------------------------------------------------------- rough transmittance
@@ -294,6 +416,62 @@ def test_oracle_environment_sampling_closed_form(env_h):
     print("env plane", env_h, L.mean(), "+-", se, "expect", expect)
     assert abs(L.mean() - expect) < 4 * se
     assert 0.2 < expect / 0.5 - 1 < 1.0  # the environment-sampling share is a sizeable part
+
+
+def test_oracle_whitted_direct_lighting_references():
+    """WhittedIntegrator on the scenes of scene/pa5/tests/test-direct.xml: at the diffuse floor it
+    samples each emitter once, which is the direct lighting the file's reference values hold."""
+    from scipy import stats
+    refs, scs = parse_test(os.path.join(GOLDEN, "test-direct.xml"))
+    sel = [(r, s) for r, s in zip(refs, scs) if s.integrator == "pathMIS"]
+    alpha = 1 - (1 - 0.01) ** (1 / len(sel))
+    for ref, sc in sel:
+        sc.integrator, sc.max_depth = "whitted", -1
+        vals = O.pixel_samples(sc, 0, 0, 0, 50000)
+        L = lum(vals[:, :3])
+        t = (L.mean() - ref) / (L.std(ddof=1) / np.sqrt(len(L)))
+        p = 2 * (1 - stats.t.cdf(abs(t), len(L) - 1))
+        assert p > alpha, (ref, L.mean(), p)
+
+
+def _whitted(sc):
+    sc.integrator, sc.max_depth = "whitted", -1
+    return sc
+
+
+def test_oracle_whitted_environment_closed_form():
+    """Whitted at a diffuse plane under a constant sky L: one environment sample per vertex, so the
+    expectation is a L int_{H+} cos / pi = a L (a wrong env-sampling pdf or Jacobian moves it)."""
+    from tests.scenes_testlib import env_plane_scene
+    sc = _whitted(env_plane_scene(albedo=0.5, radiance=1.0))
+    L = lum(O.pixel_samples(sc, 0, 0, 0, 100000)[:, :3])
+    assert abs(L.mean() - 0.5) < 4 * L.std(ddof=1) / np.sqrt(len(L)), L.mean()
+
+
+def test_oracle_two_sided_plane_seen_from_its_back():
+    """A plane whose normal faces away from the camera and sky: black when diffuse (one-sided,
+    DiffuseBSDF.cpp:20-38), a L when wrapped in twoSided (TwoSidedBSDF.cpp:21-75)."""
+    from tests.scenes_testlib import env_plane_scene
+    for two_sided, expect in ((False, 0.0), (True, 0.5)):
+        sc = _whitted(env_plane_scene(albedo=0.5))
+        m = sc.meshes[0]
+        m.f = np.ascontiguousarray(m.f[:, ::-1])  # normal -z
+        if two_sided:
+            sc.materials[m.material] = D.two_sided(sc.add_material(D.diffuse(0.5)))
+        L = lum(O.pixel_samples(sc, 0, 0, 0, 50000)[:, :3])
+        assert abs(L.mean() - expect) < 4 * L.std(ddof=1) / np.sqrt(len(L)) + 1e-9, (two_sided, L.mean())
+
+
+def test_oracle_whitted_mirror_russian_roulette():
+    """Whitted at a mirror continues with probability 0.95 and weight 1 / 0.95: a mirror under a
+    constant sky L returns L / 0.95 or 0, with mean L."""
+    from tests.scenes_testlib import env_plane_scene
+    sc = _whitted(env_plane_scene(radiance=2.0))
+    sc.materials[sc.meshes[0].material] = D.mirror()
+    sc.cam_to_world = D.lookat((0.0, -3.0, 4.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    v = lum(O.pixel_samples(sc, 0, 0, 0, 40000)[:, :3])
+    assert np.all(np.isclose(v, 2.0 / 0.95, rtol=1e-6) | (v == 0))
+    assert abs((v == 0).mean() - 0.05) < 4 * np.sqrt(0.05 * 0.95 / len(v))
 
 
 def test_cbox_oracle_event_counts():

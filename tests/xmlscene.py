@@ -86,6 +86,15 @@ def bsdf_record(node, table):
     if t == "roughPlastic":
         return D.rough_plastic(g("alpha", 0.1), dist, g("intIOR", 1.49), g("extIOR", D.AIR), g("ks", 1.0),
                                g("kd", 0.5), g("nonlinear", False))
+    if t == "mirror":
+        return D.mirror()
+    if t == "microfacet":
+        return D.microfacet(g("alpha", 0.1), g("intIOR", 1.5046), g("extIOR", D.AIR), g("kd", 0.5))
+    if t == "roughDiffuse":
+        return D.rough_diffuse(g("albedo", 0.5), g("alpha", 0.2), g("fastApprox", False))
+    if t == "twoSided":
+        sides = [table(bsdf_record(c, table)) for c in node.findall("bsdf")]
+        return D.two_sided(sides[0], sides[1] if len(sides) > 1 else -1)
     raise ValueError(f"unsupported bsdf {t}")
 
 
@@ -105,7 +114,7 @@ def parse_scene(node, base_dir):
         mp = props(m)
         v, f, n, uv = D.load_obj(os.path.join(base_dir, mp["filename"]), mp.get("toWorld"))
         b = m.find("bsdf")
-        mat = sc.add_material(bsdf_record(b, sc.materials) if b is not None else D.diffuse(0.5))
+        mat = sc.add_material(bsdf_record(b, sc.add_material) if b is not None else D.diffuse(0.5))
         e = m.find("emitter")
         rad = tuple(props(e)["radiance"]) if e is not None else None
         sc.add_mesh(v, f, mat, n=n, uv=uv, radiance=rad)
