@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--alpha", type=float, default=0.3)
     ap.add_argument("--max-paths", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
+                    help="BVH construction: binned SAH on the host or linear BVH on the device")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["c1", "c2", "batch", "serve"], default="c1")
     ap.add_argument("--requests", type=int, default=0, help="batch/serve: number of requests (default 64 / 40)")
@@ -216,10 +218,12 @@ def main():
     ctx.set_option("profile", 1)
     if a.max_paths:
         ctx.set_option("max_paths", a.max_paths)
+    ctx.set_option("bvh_builder", 1 if a.bvh_builder == "device" else 0)
     t = time.time()
     gs = ctx.scene(scene)
-    print(f"# scene resident in HBM ({gs.device_bytes() / 2**20:.0f} MiB) after {time.time() - t:.1f} s",
-          file=sys.stderr, flush=True)
+    upload_s = time.time() - t
+    print(f"# scene resident in HBM ({gs.device_bytes() / 2**20:.0f} MiB) after {upload_s:.2f} s "
+          f"({a.bvh_builder} BVH build)", file=sys.stderr, flush=True)
     for i in range(a.warmup):
         _, st = gs.render(a.spp, seed=1000 + i)
         print(f"# warmup {i}: {st['gpu_ms']:.1f} ms", file=sys.stderr, flush=True)
@@ -256,7 +260,9 @@ def main():
         "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded procedural scene)",
         "config": {"workload": workload,
-                   "spp": a.spp, "width": scene.width, "height": scene.height, "triangles": scene.n_triangles},
+                   "spp": a.spp, "width": scene.width, "height": scene.height, "triangles": scene.n_triangles,
+                   "bvh_builder": a.bvh_builder},
+        "scene_create_s": upload_s,
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "rays_per_s": rays / wall,
         "rays_per_path": rays / max(1, tot["paths"]),

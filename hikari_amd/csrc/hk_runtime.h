@@ -39,6 +39,10 @@ struct DevBvh {
   uint32_t n_nodes = 0, n_tris = 0, n_leaves = 0, max_depth = 0;
 };
 
+// Device LBVH build (hk_bvh_gpu.hip) into the FP32 node format; tri9: n x 9
+// floats on the host.  Synchronizes `st`.
+void build_bvh4_device(hipStream_t st, const float *tri9, uint32_t n, DevBvh &out);
+
 constexpr uint32_t kDefaultMaxPaths = 1u << 22;     // path slots of the wavefront
 constexpr uint32_t kDefaultMaxSamples = 1u << 28;   // samples per segment (16 B each in HBM)
 constexpr int kMaxEmitters = 32;
@@ -82,6 +86,7 @@ struct hk_ctx {
   uint32_t max_paths = hk::kDefaultMaxPaths;
   uint32_t max_samples = hk::kDefaultMaxSamples;
   uint32_t grid = 0;  // persistent grid for queue kernels (0: automatic)
+  int bvh_builder = 0;  // 0 host binned SAH, 1 device LBVH
   bool profile = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   uint32_t *host_counters = nullptr;  // pinned

@@ -25,6 +25,17 @@ namespace hk {
 extern thread_local std::string g_create_error;
 
 void upload_bvh(hk_ctx *ctx, const float *tri9, uint32_t nt, DevBvh &out) {
+  if (ctx->bvh_builder == 1 && nt >= 2) {
+#ifdef HK_NODE_QUANTIZED
+    throw HkError(HK_ERR_UNSUPPORTED, "the device BVH builder emits FP32 nodes only");
+#else
+    build_bvh4_device(ctx->stream, tri9, nt, out);
+    if (3 * out.max_depth + 1 > 96)
+      throw HkError(HK_ERR_UNSUPPORTED, "device BVH depth %u exceeds the traversal stack (use the host builder)",
+                    out.max_depth);
+    return;
+#endif
+  }
   Bvh4Result r;
   try {
     build_bvh4(tri9, nt, r);

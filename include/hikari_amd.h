@@ -67,8 +67,11 @@ typedef struct hk_accel_info {
 } hk_accel_info;
 
 /* Replaces Acceleration::AddMesh + Build (include/core/Acceleration.hpp:27-45,
- * src/acceleration/BVHAcceleration.cpp:46-248): builds a 4-wide SAH BVH over
- * nt triangles (idx: nt x 3 into v: nv x 3) and uploads it to HBM. */
+ * src/acceleration/BVHAcceleration.cpp:46-248,
+ * src/acceleration/HLBVHAcceleration.cpp:81-204): builds a 4-wide BVH over nt
+ * triangles (idx: nt x 3 into v: nv x 3) in HBM, with the binned-SAH host
+ * builder or, under option "bvh_builder" = 1, the device linear-BVH builder
+ * (both give the same closest hits). */
 int hk_accel_build(hk_ctx *ctx, const float *v, uint32_t nv, const uint32_t *idx, uint32_t nt, hk_accel **out);
 void hk_accel_destroy(hk_accel *accel);
 int hk_accel_get_info(const hk_accel *accel, hk_accel_info *info);
@@ -231,7 +234,11 @@ int hk_server_destroy(hk_server *server);
  *                 floor(max_samples / pixels) samples per pixel, resolved
  *                 into the film when their segment ends
  *   "grid"        cap on blocks per queue-kernel launch (0: one lane per item)
- *   "profile"     1: time the closest-hit kernel with HIP events */
+ *   "profile"     1: time the closest-hit kernel with HIP events
+ *   "bvh_builder" 0: binned-SAH build on the host (default); 1: linear BVH
+ *                 built on the device (Morton order, SAH leaf collapse), for
+ *                 callers that add scenes while rendering.  Both give the
+ *                 same closest hits bit for bit. */
 int hk_set_option(hk_ctx *ctx, const char *name, int64_t value);
 
 #ifdef __cplusplus
