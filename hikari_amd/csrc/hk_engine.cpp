@@ -157,6 +157,7 @@ std::unique_ptr<RequestRec> prepare_request(hk_scene *s, const hk_request &req, 
   camera_setup(c2w, fov, w, h, d.near_clip, d.far_clip, r->dev);
   filter_setup(d, r->dev);
   r->dev.seed = req.seed;
+  r->dev.tiled = 1u;  // whole images start their camera rays in 8x8 tiles
   r->npix = uint32_t(w) * uint32_t(h);
   if (r->npix > chunk_cap)
     throw HkError(HK_ERR_INVALID, "film of %u pixels exceeds the sample capacity (%u)", r->npix, chunk_cap);
@@ -364,7 +365,7 @@ void Engine::resolve_ready(std::vector<RequestRec *> &finished) {
                   "samples D2H");
         hip_check(hipStreamSynchronize(st_), "samples sync");
       } else {
-        hip_check(launch_resolve(A, uint32_t(slot), r->npix, st_), "k_resolve");
+        hip_check(launch_resolve(A, uint32_t(slot), r->dev, r->npix, s_->ctx->resolve_tiles, st_), "k_resolve");
       }
       free_range(c.buf_base, c.size);
       free_items_.push_back(uint32_t(slot));

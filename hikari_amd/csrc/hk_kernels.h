@@ -44,7 +44,10 @@ hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t n
 hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st);
 hipError_t launch_shade(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st);
 hipError_t launch_shadow(const RenderArgs &A, int qn, uint32_t grid, hipStream_t st);
-hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, uint32_t npix, hipStream_t st);
+// tiles: allow the LDS-tiled kernel (whole images, filter halo <= 4 pixels);
+// both kernels give the same sums bit for bit.
+hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, const DevRequest &rq, uint32_t npix, bool tiles,
+                          hipStream_t st);
 hipError_t launch_finalize(const void *accum, float *rgb, uint32_t n, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st);
 

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "hk_bsdf.h"
 #include "hk_kernels.h"
@@ -539,6 +540,25 @@ constexpr uint32_t kShadowLast = 0x80000000u;  // shadow entry flag: the path en
 // (request, pixel, sample index), never on its queue position.
 constexpr uint32_t kSlotShift = 20;
 
+// Pixel order of a whole image's sample slots: 8x8 tiles, row-major inside a
+// tile, tiles left to right in bands of 8 rows (the last band and the last
+// tile of a band may be smaller), so that a wave's camera rays cover a
+// compact patch.  tile_pixel maps a rank to the pixel, tile_rank back.
+__device__ __forceinline__ uint32_t tile_pixel(uint32_t r, uint32_t W, uint32_t H) {
+  uint32_t band = r / (8u * W), y0 = band * 8u;
+  uint32_t hb = min(8u, H - y0);
+  uint32_t rr = r - y0 * W;
+  uint32_t x0 = (rr / (8u * hb)) * 8u;
+  uint32_t wb = min(8u, W - x0);
+  uint32_t r2 = rr - x0 * hb;
+  return (y0 + r2 / wb) * W + x0 + r2 % wb;
+}
+__device__ __forceinline__ uint32_t tile_rank(uint32_t x, uint32_t y, uint32_t W, uint32_t H) {
+  uint32_t y0 = y & ~7u, x0 = x & ~7u;
+  uint32_t hb = min(8u, H - y0), wb = min(8u, W - x0);
+  return y0 * W + x0 * hb + (y - y0) * wb + (x - x0);
+}
+
 __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t q_base, uint32_t n_new) {
   uint32_t stride = gridDim.x * blockDim.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -550,7 +570,8 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t
     const WorkItem it = A.items[e.slot];
     const DevRequest &rq = A.reqs[it.req];
     uint32_t local = e.local0 + (j - e.prefix);
-    uint32_t q = rq.pix0 + local % it.npix, s = it.s0 + local / it.npix;
+    uint32_t rank = local % it.npix, s = it.s0 + local / it.npix;
+    uint32_t q = rq.tiled ? tile_pixel(rank, (uint32_t)rq.width, (uint32_t)rq.height) : rq.pix0 + rank;
     uint32_t x = q % (uint32_t)rq.width, y = q / (uint32_t)rq.width;
     Rng rng = rng_for_path(rq.seed, q, s);
     float jx = rng.next(), jy = rng.next();
@@ -1031,7 +1052,11 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
 // ------------------------------------------------------------------ k_resolve
 // ImageBlock::Put (src/core/Block.cpp:95-134) as a gather: each output pixel
 // owns its accumulator and sums the filtered samples of its neighbours.
-__global__ __launch_bounds__(kBlock) void k_resolve(RenderArgs A, uint32_t slot) {
+// ImageBlock::Put (src/core/Block.cpp:95-134) as a gather: each output pixel
+// accumulates, in (sample, neighbour row, neighbour column) order, every
+// sample of the chunk whose filter footprint covers it.  A sample's jitter is
+// recomputed from its path stream.  This plain form serves any film.
+__global__ __launch_bounds__(kBlock) void k_resolve_plain(RenderArgs A, uint32_t slot) {
   const WorkItem it = A.items[slot];
   const DevRequest &rq = A.reqs[it.req];
   int W = rq.width, H = rq.height;
@@ -1052,7 +1077,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(RenderArgs A, uint32_t slot)
           float cx = ((float)qx + jx) - 0.5f, cy = ((float)qy + jy) - 0.5f;
           if (x < (int)ceilf(cx - R) || x > (int)floorf(cx + R) || y < (int)ceilf(cy - R) || y > (int)floorf(cy + R))
             continue;
-          float4 l4 = A.sample_L[it.buf_base + s * it.npix + q];
+          uint32_t rank = rq.tiled ? tile_rank((uint32_t)qx, (uint32_t)qy, (uint32_t)W, (uint32_t)H) : q;
+          float4 l4 = A.sample_L[it.buf_base + s * it.npix + rank];
           vec3 L = v3(l4.x, l4.y, l4.z);
           if (!color_valid(L)) continue;  // invalid samples are dropped (Block.cpp:97-101)
           float wx = rq.filter[(int)(fabsf((float)x - cx) * rq.lookup)];
@@ -1065,6 +1091,69 @@ __global__ __launch_bounds__(kBlock) void k_resolve(RenderArgs A, uint32_t slot)
       }
     }
     accum[p] = acc;
+  }
+}
+
+// The same accumulation for whole images, a 16x16 pixel tile per block: for
+// each sample index the tile and its filter halo (jitter recomputed once per
+// halo pixel, radiance read once) are staged in LDS, then every pixel sums its
+// neighbours in the plain kernel's order, so the sums are bit-identical.
+constexpr int kResTile = 16, kResHalo = 4, kResEdge = kResTile + 2 * kResHalo;
+__global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t slot) {
+  __shared__ float4 s_l[kResEdge * kResEdge];   // radiance, w: 1 valid sample, 0 dropped or outside
+  __shared__ float2 s_c[kResEdge * kResEdge];   // sample position - 0.5
+  const WorkItem it = A.items[slot];
+  const DevRequest &rq = A.reqs[it.req];
+  int W = rq.width, H = rq.height;
+  float R = rq.filter_radius;
+  int rad = (int)floorf(R + 0.5f);  // <= kResHalo (checked on the host)
+  int ew = kResTile + 2 * rad;
+  int tiles_x = (W + kResTile - 1) / kResTile, tiles_y = (H + kResTile - 1) / kResTile;
+  float4 *accum = reinterpret_cast<float4 *>(rq.accum);
+  for (int tile = blockIdx.x; tile < tiles_x * tiles_y; tile += gridDim.x) {
+    int tx0 = (tile % tiles_x) * kResTile, ty0 = (tile / tiles_x) * kResTile;
+    int x = tx0 + (int)threadIdx.x % kResTile, y = ty0 + (int)threadIdx.x / kResTile;
+    bool own = x < W && y < H;
+    float4 acc = own ? accum[y * W + x] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (uint32_t s = 0; s < it.k; s++) {
+      __syncthreads();  // the previous sample's tile has been read
+      for (int e = threadIdx.x; e < ew * ew; e += blockDim.x) {
+        int qx = tx0 - rad + e % ew, qy = ty0 - rad + e / ew;
+        float4 l = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float2 c = make_float2(0.0f, 0.0f);
+        if (qx >= 0 && qx < W && qy >= 0 && qy < H) {
+          uint32_t q = (uint32_t)(qy * W + qx);
+          Rng rng = rng_for_path(rq.seed, q, it.s0 + s);
+          float jx = rng.next(), jy = rng.next();
+          c = make_float2(((float)qx + jx) - 0.5f, ((float)qy + jy) - 0.5f);
+          float4 l4 = A.sample_L[it.buf_base + s * it.npix + tile_rank((uint32_t)qx, (uint32_t)qy, W, H)];
+          // invalid samples are dropped (Block.cpp:97-101)
+          l = make_float4(l4.x, l4.y, l4.z, color_valid(v3(l4.x, l4.y, l4.z)) ? 1.0f : 0.0f);
+        }
+        s_l[e] = l;
+        s_c[e] = c;
+      }
+      __syncthreads();
+      if (!own) continue;
+      for (int qy = max(0, y - rad); qy <= min(H - 1, y + rad); qy++) {
+        for (int qx = max(0, x - rad); qx <= min(W - 1, x + rad); qx++) {
+          int e = (qy - ty0 + rad) * ew + (qx - tx0 + rad);
+          float2 c = s_c[e];
+          float cx = c.x, cy = c.y;
+          if (x < (int)ceilf(cx - R) || x > (int)floorf(cx + R) || y < (int)ceilf(cy - R) || y > (int)floorf(cy + R))
+            continue;
+          float4 l4 = s_l[e];
+          if (l4.w == 0.0f) continue;
+          float wx = rq.filter[(int)(fabsf((float)x - cx) * rq.lookup)];
+          float wy = rq.filter[(int)(fabsf((float)y - cy) * rq.lookup)];
+          acc.x += l4.x * wx * wy;
+          acc.y += l4.y * wx * wy;
+          acc.z += l4.z * wx * wy;
+          acc.w += 1.0f * wx * wy;
+        }
+      }
+    }
+    if (own) accum[y * W + x] = acc;
   }
 }
 
@@ -1139,8 +1228,15 @@ hipError_t launch_shadow(const RenderArgs &A, int qn, uint32_t grid, hipStream_t
   hipLaunchKernelGGL(k_shadow, dim3(std::min(grid, cap)), dim3(kBlock), 0, st, A, qn);
   return hipGetLastError();
 }
-hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, uint32_t npix, hipStream_t st) {
-  hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, 1u << 16)), dim3(kBlock), 0, st, A, slot);
+hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, const DevRequest &rq, uint32_t npix, bool tiles,
+                          hipStream_t st) {
+  int rad = (int)std::floor(rq.filter_radius + 0.5f);
+  if (tiles && rq.tiled && rad <= kResHalo && uint64_t(rq.width) * uint64_t(rq.height) == npix) {
+    uint32_t tiles = uint32_t((rq.width + kResTile - 1) / kResTile) * uint32_t((rq.height + kResTile - 1) / kResTile);
+    hipLaunchKernelGGL(k_resolve_tiles, dim3(std::min(tiles, 1u << 16)), dim3(kBlock), 0, st, A, slot);
+  } else {
+    hipLaunchKernelGGL(k_resolve_plain, dim3(grid_for(npix, 1u << 16)), dim3(kBlock), 0, st, A, slot);
+  }
   return hipGetLastError();
 }
 hipError_t launch_finalize(const void *accum, float *rgb, uint32_t n, hipStream_t st) {

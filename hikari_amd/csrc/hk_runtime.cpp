@@ -455,6 +455,8 @@ int hk_set_option(hk_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "grid") {
     if (value < 0 || value > (1 << 20)) throw HkError(HK_ERR_INVALID, "grid out of range");
     ctx->grid = uint32_t(value);
+  } else if (n == "resolve_tiles") {
+    ctx->resolve_tiles = value != 0;
   } else if (n == "bvh_builder") {
     if (value < 0 || value > 1) throw HkError(HK_ERR_INVALID, "bvh_builder: 0 (host SAH) or 1 (device LBVH)");
     ctx->bvh_builder = int(value);

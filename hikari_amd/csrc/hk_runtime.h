@@ -87,6 +87,7 @@ struct hk_ctx {
   uint32_t max_samples = hk::kDefaultMaxSamples;
   uint32_t grid = 0;  // persistent grid for queue kernels (0: automatic)
   int bvh_builder = 0;  // 0 host binned SAH, 1 device LBVH
+  bool resolve_tiles = true;  // film resolve through LDS tiles (false: the plain gather)
   bool profile = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   uint32_t *host_counters = nullptr;  // pinned

@@ -319,6 +319,7 @@ int hk_render_pixel_samples(hk_scene *s, uint32_t seed, uint32_t x, uint32_t y, 
   uint32_t cap = std::max(ctx->max_samples, uint32_t(d.width) * uint32_t(d.height));
   std::unique_ptr<RequestRec> r = prepare_request(s, rq, nullptr, cap);
   r->dev.pix0 = pix;
+  r->dev.tiled = 0u;
   r->npix = 1;
   r->chunk = std::max(1u, std::min(n, ctx->max_samples));
   r->n_chunks = (n + r->chunk - 1) / r->chunk;

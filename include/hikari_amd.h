@@ -235,6 +235,8 @@ int hk_server_destroy(hk_server *server);
  *                 into the film when their segment ends
  *   "grid"        cap on blocks per queue-kernel launch (0: one lane per item)
  *   "profile"     1: time the closest-hit kernel with HIP events
+ *   "resolve_tiles" 0: resolve films with the plain per-pixel gather instead
+ *                 of LDS tiles (same bits; default 1)
  *   "bvh_builder" 0: binned-SAH build on the host (default); 1: linear BVH
  *                 built on the device (Morton order, SAH leaf collapse), for
  *                 callers that add scenes while rendering.  Both give the

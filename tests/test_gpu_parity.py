@@ -319,3 +319,17 @@ def test_heightfield_image_vs_oracle(ctx):
     print("heightfield relMSE vs oracle:", rm, st)
     assert rm < 1e-3
     assert abs(lum(img).mean() - lum(ref).mean()) / lum(ref).mean() < 0.01
+
+
+@pytest.mark.parametrize("filt", [D.FILTER_GAUSSIAN, D.FILTER_BOX, D.FILTER_MITCHELL])
+def test_tiled_resolve_equals_plain(ctx, filt):
+    """The LDS-tiled film resolve sums exactly what the plain per-pixel gather
+    sums (a ragged 70 x 45 film, so edge tiles are partial)."""
+    sc = scenes.cbox(width=70, height=45, sphere_subdiv=2, filter=filt)
+    a, _ = ctx.scene(sc).render(6, seed=8)
+    ctx.set_option("resolve_tiles", 0)
+    try:
+        b, _ = ctx.scene(sc).render(6, seed=8)
+    finally:
+        ctx.set_option("resolve_tiles", 1)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
