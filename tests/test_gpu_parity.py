@@ -333,3 +333,25 @@ def test_tiled_resolve_equals_plain(ctx, filt):
     finally:
         ctx.set_option("resolve_tiles", 1)
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("which", ["cbox", "materials"])
+def test_image_vs_high_spp_fp64_oracle(ctx, which):
+    """GPU image vs an independent FP64 CPU render (different random streams),
+    both at 6 x 1024 spp: relMSE <= 1e-3 and a per-pixel z-test (variances of
+    the mean from the six batches of each side; at most 1% of pixels beyond
+    |z| = 4, which a t-distribution with ~10 dof exceeds for ~0.3%)."""
+    from tests.scenes_testlib import materials_box
+    sc = scenes.cbox(width=24, height=24, sphere_subdiv=2) if which == "cbox" else materials_box(24, 24)
+    gs = ctx.scene(sc)
+    k = 6
+    g = np.stack([gs.render(1024, seed=100 + i)[0].astype(np.float64) for i in range(k)])
+    o = np.stack([O.render(sc, req_seed=200 + i, spp=(0, 1024), fp64=True)[0].astype(np.float64) for i in range(k)])
+    gm, om = g.mean(0), o.mean(0)
+    rm = rel_mse(gm, om)
+    se = np.sqrt(g.var(0, ddof=1) / k + o.var(0, ddof=1) / k)
+    z = np.abs(gm - om) / np.maximum(se, 1e-12)
+    frac = float(np.mean(z > 4))
+    print(which, "relMSE vs FP64 oracle", rm, "fraction |z| > 4:", frac, "mean", gm.mean(), om.mean())
+    assert rm <= 1e-3
+    assert frac <= 0.01
