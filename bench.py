@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--alpha", type=float, default=0.3)
     ap.add_argument("--max-paths", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: pick max_paths / grid / tile with hk_autotune before timing (results do not change)")
     ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
                     help="BVH construction: binned SAH on the host or linear BVH on the device")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -224,6 +226,13 @@ def main():
     upload_s = time.time() - t
     print(f"# scene resident in HBM ({gs.device_bytes() / 2**20:.0f} MiB) after {upload_s:.2f} s "
           f"({a.bvh_builder} BVH build)", file=sys.stderr, flush=True)
+    tune = None
+    if a.autotune and not a.max_paths:
+        tune = gs.autotune(a.spp, seed=999)
+        print(f"# autotune: max_paths {tune['max_paths']}, tile {tune['tile']}, grid {tune['grid']} in "
+              f"{tune['total_ms']:.0f} ms: " + ", ".join(f"({t['max_paths']}, {t['tile']}, {t['grid']}) "
+                                                       f"{t['samples_per_s']:.3e}" for t in tune["tried"]),
+              file=sys.stderr, flush=True)
     for i in range(a.warmup):
         _, st = gs.render(a.spp, seed=1000 + i)
         print(f"# warmup {i}: {st['gpu_ms']:.1f} ms", file=sys.stderr, flush=True)
@@ -263,6 +272,7 @@ def main():
                    "spp": a.spp, "width": scene.width, "height": scene.height, "triangles": scene.n_triangles,
                    "bvh_builder": a.bvh_builder},
         "scene_create_s": upload_s,
+        "autotune": None if tune is None else {k: tune[k] for k in ("max_paths", "grid", "tile", "total_ms")},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "rays_per_s": rays / wall,
         "rays_per_path": rays / max(1, tot["paths"]),

@@ -24,7 +24,7 @@ EXPORTS = ["hk_ctx_create", "hk_ctx_destroy", "hk_last_error", "hk_version", "hk
            "hk_accel_destroy", "hk_accel_get_info", "hk_accel_intersect", "hk_accel_intersect_device",
            "hk_bsdf_batch", "hk_scene_create", "hk_scene_destroy", "hk_scene_device_bytes", "hk_render",
            "hk_render_pixel_samples", "hk_render_batch", "hk_set_option", "hk_server_create", "hk_server_add_scene",
-           "hk_server_submit", "hk_server_wait", "hk_server_get_stats", "hk_server_destroy"]
+           "hk_server_submit", "hk_server_wait", "hk_server_get_stats", "hk_server_destroy", "hk_autotune"]
 
 
 class HkError(RuntimeError):
@@ -107,6 +107,7 @@ def lib():
         L.hk_server_wait.argtypes = [P, C.c_uint64, C.POINTER(ServerResult)]
         L.hk_server_get_stats.argtypes = [P, C.POINTER(ServerStats)]
         L.hk_server_destroy.argtypes = [P]
+        L.hk_autotune.argtypes = [P, C.POINTER(Request), C.c_uint64, C.POINTER(TuneResult)]
         _lib = L
     return _lib
 
@@ -215,6 +216,16 @@ def make_request(scene=None, spp=1, seed=0, spp_offset=0, cam_to_world=None, fov
     return r
 
 
+TUNE_MAX = 32
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("max_paths", C.c_uint32), ("grid", C.c_uint32), ("tile", C.c_uint32), ("n_tried", C.c_uint32),
+                ("tried_max_paths", C.c_uint32 * TUNE_MAX), ("tried_grid", C.c_uint32 * TUNE_MAX),
+                ("tried_tile", C.c_uint32 * TUNE_MAX), ("samples_per_s", C.c_double * TUNE_MAX),
+                ("total_ms", C.c_double)]
+
+
 class GpuScene:
     """A scene resident in HBM (hk_scene)."""
 
@@ -250,6 +261,17 @@ class GpuScene:
         st = RenderStats()
         self.ctx.check(lib().hk_render_batch(self.h, arr, n, ptrs, max_paths, C.byref(st)))
         return imgs, st.as_dict()
+
+    def autotune(self, spp, seed=0, calib_samples=0, request=None):
+        """hk_autotune: picks the context's max_paths / grid / tile for this
+        kind of request (results never depend on them) and returns the choice
+        with every candidate's calibration throughput."""
+        req = request or make_request(self.scene, spp, seed)
+        t = TuneResult()
+        self.ctx.check(lib().hk_autotune(self.h, C.byref(req), C.c_uint64(calib_samples), C.byref(t)))
+        tried = [{"max_paths": t.tried_max_paths[i], "grid": t.tried_grid[i], "tile": t.tried_tile[i],
+                  "samples_per_s": t.samples_per_s[i]} for i in range(t.n_tried)]
+        return {"max_paths": t.max_paths, "grid": t.grid, "tile": t.tile, "total_ms": t.total_ms, "tried": tried}
 
     def pixel_samples(self, x, y, s0, n, seed=0):
         out = np.zeros((n, 5), np.float32)

@@ -157,7 +157,7 @@ std::unique_ptr<RequestRec> prepare_request(hk_scene *s, const hk_request &req, 
   camera_setup(c2w, fov, w, h, d.near_clip, d.far_clip, r->dev);
   filter_setup(d, r->dev);
   r->dev.seed = req.seed;
-  r->dev.tiled = 1u;  // whole images start their camera rays in 8x8 tiles
+  r->dev.tile = s->ctx->tile;  // whole images start their camera rays in tiles
   r->npix = uint32_t(w) * uint32_t(h);
   if (r->npix > chunk_cap)
     throw HkError(HK_ERR_INVALID, "film of %u pixels exceeds the sample capacity (%u)", r->npix, chunk_cap);

@@ -540,22 +540,23 @@ constexpr uint32_t kShadowLast = 0x80000000u;  // shadow entry flag: the path en
 // (request, pixel, sample index), never on its queue position.
 constexpr uint32_t kSlotShift = 20;
 
-// Pixel order of a whole image's sample slots: 8x8 tiles, row-major inside a
-// tile, tiles left to right in bands of 8 rows (the last band and the last
-// tile of a band may be smaller), so that a wave's camera rays cover a
-// compact patch.  tile_pixel maps a rank to the pixel, tile_rank back.
-__device__ __forceinline__ uint32_t tile_pixel(uint32_t r, uint32_t W, uint32_t H) {
-  uint32_t band = r / (8u * W), y0 = band * 8u;
-  uint32_t hb = min(8u, H - y0);
+// Pixel order of a whole image's sample slots: T x T tiles (T a power of two),
+// row-major inside a tile, tiles left to right in bands of T rows (the last
+// band and the last tile of a band may be smaller), so that a wave's camera
+// rays cover a compact patch; T = 1 is scanline order.  tile_pixel maps a
+// rank to the pixel, tile_rank back.
+__device__ __forceinline__ uint32_t tile_pixel(uint32_t r, uint32_t W, uint32_t H, uint32_t T) {
+  uint32_t band = r / (T * W), y0 = band * T;
+  uint32_t hb = min(T, H - y0);
   uint32_t rr = r - y0 * W;
-  uint32_t x0 = (rr / (8u * hb)) * 8u;
-  uint32_t wb = min(8u, W - x0);
+  uint32_t x0 = (rr / (T * hb)) * T;
+  uint32_t wb = min(T, W - x0);
   uint32_t r2 = rr - x0 * hb;
   return (y0 + r2 / wb) * W + x0 + r2 % wb;
 }
-__device__ __forceinline__ uint32_t tile_rank(uint32_t x, uint32_t y, uint32_t W, uint32_t H) {
-  uint32_t y0 = y & ~7u, x0 = x & ~7u;
-  uint32_t hb = min(8u, H - y0), wb = min(8u, W - x0);
+__device__ __forceinline__ uint32_t tile_rank(uint32_t x, uint32_t y, uint32_t W, uint32_t H, uint32_t T) {
+  uint32_t y0 = y & ~(T - 1u), x0 = x & ~(T - 1u);
+  uint32_t hb = min(T, H - y0), wb = min(T, W - x0);
   return y0 * W + x0 * hb + (y - y0) * wb + (x - x0);
 }
 
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t
     const DevRequest &rq = A.reqs[it.req];
     uint32_t local = e.local0 + (j - e.prefix);
     uint32_t rank = local % it.npix, s = it.s0 + local / it.npix;
-    uint32_t q = rq.tiled ? tile_pixel(rank, (uint32_t)rq.width, (uint32_t)rq.height) : rq.pix0 + rank;
+    uint32_t q = rq.tile ? tile_pixel(rank, (uint32_t)rq.width, (uint32_t)rq.height, rq.tile) : rq.pix0 + rank;
     uint32_t x = q % (uint32_t)rq.width, y = q / (uint32_t)rq.width;
     Rng rng = rng_for_path(rq.seed, q, s);
     float jx = rng.next(), jy = rng.next();
@@ -1077,7 +1078,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_plain(RenderArgs A, uint32_t
           float cx = ((float)qx + jx) - 0.5f, cy = ((float)qy + jy) - 0.5f;
           if (x < (int)ceilf(cx - R) || x > (int)floorf(cx + R) || y < (int)ceilf(cy - R) || y > (int)floorf(cy + R))
             continue;
-          uint32_t rank = rq.tiled ? tile_rank((uint32_t)qx, (uint32_t)qy, (uint32_t)W, (uint32_t)H) : q;
+          uint32_t rank = rq.tile ? tile_rank((uint32_t)qx, (uint32_t)qy, (uint32_t)W, (uint32_t)H, rq.tile) : q;
           float4 l4 = A.sample_L[it.buf_base + s * it.npix + rank];
           vec3 L = v3(l4.x, l4.y, l4.z);
           if (!color_valid(L)) continue;  // invalid samples are dropped (Block.cpp:97-101)
@@ -1126,7 +1127,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t
           Rng rng = rng_for_path(rq.seed, q, it.s0 + s);
           float jx = rng.next(), jy = rng.next();
           c = make_float2(((float)qx + jx) - 0.5f, ((float)qy + jy) - 0.5f);
-          float4 l4 = A.sample_L[it.buf_base + s * it.npix + tile_rank((uint32_t)qx, (uint32_t)qy, W, H)];
+          float4 l4 = A.sample_L[it.buf_base + s * it.npix + tile_rank((uint32_t)qx, (uint32_t)qy, W, H, rq.tile)];
           // invalid samples are dropped (Block.cpp:97-101)
           l = make_float4(l4.x, l4.y, l4.z, color_valid(v3(l4.x, l4.y, l4.z)) ? 1.0f : 0.0f);
         }
@@ -1231,7 +1232,7 @@ hipError_t launch_shadow(const RenderArgs &A, int qn, uint32_t grid, hipStream_t
 hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, const DevRequest &rq, uint32_t npix, bool tiles,
                           hipStream_t st) {
   int rad = (int)std::floor(rq.filter_radius + 0.5f);
-  if (tiles && rq.tiled && rad <= kResHalo && uint64_t(rq.width) * uint64_t(rq.height) == npix) {
+  if (tiles && rq.tile && rad <= kResHalo && uint64_t(rq.width) * uint64_t(rq.height) == npix) {
     uint32_t tiles = uint32_t((rq.width + kResTile - 1) / kResTile) * uint32_t((rq.height + kResTile - 1) / kResTile);
     hipLaunchKernelGGL(k_resolve_tiles, dim3(std::min(tiles, 1u << 16)), dim3(kBlock), 0, st, A, slot);
   } else {

@@ -43,7 +43,7 @@ struct DevBvh {
 // floats on the host.  Synchronizes `st`.
 void build_bvh4_device(hipStream_t st, const float *tri9, uint32_t n, DevBvh &out);
 
-constexpr uint32_t kDefaultMaxPaths = 1u << 22;     // path slots of the wavefront
+constexpr uint32_t kDefaultMaxPaths = 1u << 24;     // path slots of the wavefront (~350 B of HBM each)
 constexpr uint32_t kDefaultMaxSamples = 1u << 28;   // samples per segment (16 B each in HBM)
 constexpr int kMaxEmitters = 32;
 
@@ -88,6 +88,7 @@ struct hk_ctx {
   uint32_t grid = 0;  // persistent grid for queue kernels (0: automatic)
   int bvh_builder = 0;  // 0 host binned SAH, 1 device LBVH
   bool resolve_tiles = true;  // film resolve through LDS tiles (false: the plain gather)
+  uint32_t tile = 8;          // camera-ray tile edge (1: scanlines)
   bool profile = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   uint32_t *host_counters = nullptr;  // pinned

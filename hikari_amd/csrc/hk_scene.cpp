@@ -301,6 +301,82 @@ int hk_render_batch(hk_scene *s, const hk_request *reqs, uint32_t n, float **out
   HK_API_END(ctx)
 }
 
+// Coordinate search: wavefront size first (the largest effect), then the
+// camera-ray tile edge, then the queue-kernel grid cap, each with the best
+// setting found so far for the others.  Every calibration render goes
+// through hk_render_batch with the candidate in the context options.
+int hk_autotune(hk_scene *s, const hk_request *req, uint64_t calib_samples, hk_tune_result *out) {
+  hk_ctx *ctx = s ? s->ctx : nullptr;
+  HK_API_BEGIN
+  if (!s || !req || !out) throw HkError(HK_ERR_INVALID, "null argument");
+  auto t0 = std::chrono::steady_clock::now();
+  std::memset(out, 0, sizeof *out);
+  const hk_scene_desc &d = s->desc;
+  uint64_t npix = req->use_scene_camera ? uint64_t(d.width) * d.height : uint64_t(req->width) * req->height;
+  if (npix == 0) throw HkError(HK_ERR_INVALID, "empty film");
+  if (calib_samples == 0) calib_samples = uint64_t(64) << 20;
+  hk_request cal = *req;
+  cal.spp = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(req->spp ? req->spp : 1, calib_samples / npix)));
+  cal.spp_offset = 0;
+  std::vector<float> film(size_t(npix) * 3);
+  float *films[1] = {film.data()};
+  uint32_t best_paths = ctx->max_paths, best_grid = ctx->grid, best_tile = ctx->tile;
+  bool saved_profile = ctx->profile;
+  ctx->profile = false;
+  auto measure = [&](uint32_t paths, uint32_t grid, uint32_t tile) {
+    ctx->max_paths = paths;
+    ctx->grid = grid;
+    ctx->tile = tile;
+    hk_render_stats st;
+    int rc = hk_render_batch(s, &cal, 1, films, 0, &st);
+    if (rc != HK_OK) throw HkError(rc, "calibration render: %s", ctx->err.c_str());
+    double rate = st.gpu_ms > 0 ? double(st.paths) / (st.gpu_ms * 1e-3) : 0.0;
+    if (out->n_tried < HK_TUNE_MAX) {
+      uint32_t k = out->n_tried++;
+      out->tried_max_paths[k] = paths;
+      out->tried_grid[k] = grid;
+      out->tried_tile[k] = tile;
+      out->samples_per_s[k] = rate;
+    }
+    return rate;
+  };
+  try {
+    measure(best_paths, best_grid, best_tile);  // warm-up: workspace, code and caches
+    out->n_tried = 0;
+    double best = 0.0;
+    uint64_t total = npix * cal.spp;
+    for (uint32_t p : {1u << 22, 1u << 23, 1u << 24, 1u << 25}) {
+      if (p > total && p != (1u << 22)) break;  // a wavefront larger than the work is never filled
+      double r = measure(p, best_grid, best_tile);
+      if (r > best) { best = r; best_paths = p; }
+    }
+    for (uint32_t t : {1u, 4u, 8u, 16u}) {
+      if (t == best_tile) continue;
+      double r = measure(best_paths, best_grid, t);
+      if (r > best) { best = r; best_tile = t; }
+    }
+    for (uint32_t g : {0u, uint32_t(ctx->n_cu) * 8u, uint32_t(ctx->n_cu) * 32u}) {
+      if (g == best_grid) continue;
+      double r = measure(best_paths, g, best_tile);
+      if (r > best) { best = r; best_grid = g; }
+    }
+  } catch (...) {
+    ctx->profile = saved_profile;
+    throw;
+  }
+  ctx->profile = saved_profile;
+  ctx->max_paths = best_paths;
+  ctx->grid = best_grid;
+  ctx->tile = best_tile;
+  ctx->ws.release();  // the next render sizes the workspace for the chosen wavefront
+  out->max_paths = best_paths;
+  out->grid = best_grid;
+  out->tile = best_tile;
+  out->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return HK_OK;
+  HK_API_END(ctx)
+}
+
 int hk_render_pixel_samples(hk_scene *s, uint32_t seed, uint32_t x, uint32_t y, uint32_t s0, uint32_t n, float *out) {
   hk_ctx *ctx = s ? s->ctx : nullptr;
   HK_API_BEGIN
@@ -319,7 +395,7 @@ int hk_render_pixel_samples(hk_scene *s, uint32_t seed, uint32_t x, uint32_t y, 
   uint32_t cap = std::max(ctx->max_samples, uint32_t(d.width) * uint32_t(d.height));
   std::unique_ptr<RequestRec> r = prepare_request(s, rq, nullptr, cap);
   r->dev.pix0 = pix;
-  r->dev.tiled = 0u;
+  r->dev.tile = 0u;
   r->npix = 1;
   r->chunk = std::max(1u, std::min(n, ctx->max_samples));
   r->n_chunks = (n + r->chunk - 1) / r->chunk;

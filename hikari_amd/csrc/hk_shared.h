@@ -133,7 +133,8 @@ struct DevRequest {
   uint32_t pix0;           // first pixel of the film slice (0 for whole images)
   float filter[33];
   float filter_radius, lookup;
-  uint32_t tiled;          // 1: sample slots of a pass follow 8x8 tiles (whole images)
+  uint32_t tile;           // whole images: edge of the square pixel tiles that order the
+                           // sample slots (power of two, 1: scanlines); 0: a film slice at pix0
   float *accum;            // width x height x 4 (rgb * w, w)
 };
 
@@ -141,7 +142,7 @@ struct DevRequest {
 // [s0, s0 + k).  It occupies a slot of the device item table while it runs;
 // the radiance of its sample `local` (= sample offset * npix + pixel rank)
 // goes to sample_L[buf_base + local].  The pixel rank is the pixel index, or
-// for whole images (DevRequest::tiled) the position in 8x8-tile order.
+// for whole images the position in tile order (DevRequest::tile).
 struct WorkItem {
   uint32_t req, npix, s0, k;
   uint32_t buf_base, pad0, pad1, pad2;

@@ -180,6 +180,24 @@ typedef struct hk_render_stats {
  * image (ImageBlock::ToBitmap).  Synchronous. */
 int hk_render(hk_scene *scene, const hk_request *req, float *out_rgb, hk_render_stats *stats);
 
+/* ------------------------------------------------------------ autotuning */
+/* No counterpart in the reference.  Picks the context's wavefront size
+ * ("max_paths"), queue-kernel grid cap ("grid") and camera-ray tile edge
+ * ("tile") for requests like `req` on `scene`: each candidate renders a
+ * calibration slice of req (about calib_samples path samples, 0: 64 M; at
+ * least one sample per pixel), timed with HIP events; the fastest setting is
+ * stored in the context options and reported.  No film depends on these
+ * options (the same bits for any setting), so tuning changes only speed. */
+#define HK_TUNE_MAX 32
+typedef struct hk_tune_result {
+  uint32_t max_paths, grid, tile;   /* the chosen setting */
+  uint32_t n_tried;
+  uint32_t tried_max_paths[HK_TUNE_MAX], tried_grid[HK_TUNE_MAX], tried_tile[HK_TUNE_MAX];
+  double samples_per_s[HK_TUNE_MAX];
+  double total_ms;                  /* wall time of the search */
+} hk_tune_result;
+int hk_autotune(hk_scene *scene, const hk_request *req, uint64_t calib_samples, hk_tune_result *out);
+
 /* Per-sample radiance of one pixel for samples [s0, s0 + n) (the t-test
  * harness of src/core/StudentTTest.cpp:137-175): out n x 5 (r, g, b, px, py). */
 int hk_render_pixel_samples(hk_scene *scene, uint32_t seed, uint32_t x, uint32_t y, uint32_t s0, uint32_t n,
@@ -227,13 +245,15 @@ int hk_server_get_stats(hk_server *server, hk_server_stats *stats);
 int hk_server_destroy(hk_server *server);
 
 /* Tuning knobs (value 0 restores the default):
- *   "max_paths"   paths in flight (default 4M); at every bounce finished
+ *   "max_paths"   paths in flight (default 16M); at every bounce finished
  *                 paths are replaced by the next samples
  *   "max_samples" samples per segment (default 256M, 16 B of HBM each): a
  *                 request's samples are cut into chunks of
  *                 floor(max_samples / pixels) samples per pixel, resolved
  *                 into the film when their segment ends
  *   "grid"        cap on blocks per queue-kernel launch (0: one lane per item)
+ *   "tile"        edge of the pixel tiles camera rays start in (1, 2, 4, 8,
+ *                 16 or 32; 1: scanlines; default 8)
  *   "profile"     1: time the closest-hit kernel with HIP events
  *   "resolve_tiles" 0: resolve films with the plain per-pixel gather instead
  *                 of LDS tiles (same bits; default 1)

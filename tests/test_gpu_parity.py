@@ -355,3 +355,40 @@ def test_image_vs_high_spp_fp64_oracle(ctx, which):
     print(which, "relMSE vs FP64 oracle", rm, "fraction |z| > 4:", frac, "mean", gm.mean(), om.mean())
     assert rm <= 1e-3
     assert frac <= 0.01
+
+
+def test_tuning_options_do_not_change_bits():
+    """Camera-ray tile edge, queue-kernel grid cap and wavefront size (the
+    autotuner's knobs) leave every film bit unchanged (ragged 70 x 45 film)."""
+    sc = scenes.cbox(width=70, height=45, sphere_subdiv=2)
+    imgs = []
+    for tile, grid, max_paths in [(8, 0, 0), (1, 0, 0), (4, 64, 0), (16, 0, 5000), (32, 7, 999)]:
+        c = hk.Context(0)
+        c.set_option("tile", tile)
+        c.set_option("grid", grid)
+        c.set_option("max_paths", max_paths)
+        img, _ = c.scene(sc).render(5, seed=12)
+        imgs.append(img)
+        c.close()
+    for im in imgs[1:]:
+        assert np.array_equal(im.view(np.uint32), imgs[0].view(np.uint32))
+
+
+def test_autotune_keeps_results():
+    """hk_autotune tries wavefront sizes, tile edges and grid caps on a
+    calibration slice, keeps the fastest in the context options, and renders
+    after tuning equal those before."""
+    c = hk.Context(0)
+    try:
+        sc = scenes.cbox(width=96, height=64, sphere_subdiv=2)
+        gs = c.scene(sc)
+        before, _ = gs.render(16, seed=3)
+        t = gs.autotune(16, calib_samples=96 * 64 * 16)
+        print("autotune:", t)
+        assert len(t["tried"]) >= 5
+        assert (t["max_paths"], t["grid"], t["tile"]) in {(x["max_paths"], x["grid"], x["tile"]) for x in t["tried"]}
+        assert all(x["samples_per_s"] > 0 for x in t["tried"])
+        after, _ = gs.render(16, seed=3)
+        assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
+    finally:
+        c.close()
