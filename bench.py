@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--alpha", type=float, default=0.3)
     ap.add_argument("--max-paths", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", type=int, default=1, help="1: replay bounces from captured HIP graphs")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: pick max_paths / grid / tile with hk_autotune before timing (results do not change)")
     ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
@@ -221,6 +222,7 @@ def main():
     if a.max_paths:
         ctx.set_option("max_paths", a.max_paths)
     ctx.set_option("bvh_builder", 1 if a.bvh_builder == "device" else 0)
+    ctx.set_option("graph", a.graph)
     t = time.time()
     gs = ctx.scene(scene)
     upload_s = time.time() - t
@@ -237,7 +239,7 @@ def main():
         _, st = gs.render(a.spp, seed=1000 + i)
         print(f"# warmup {i}: {st['gpu_ms']:.1f} ms", file=sys.stderr, flush=True)
     lat, tot = [], {"paths": 0, "extension_rays": 0, "shadow_rays": 0, "trace_ms": 0.0, "trace_launches": 0,
-                    "gpu_ms": 0.0, "node_visits": 0, "triangle_tests": 0}
+                    "gpu_ms": 0.0, "node_visits": 0, "triangle_tests": 0, "bounces_launched": 0, "graph_bounces": 0}
     t_all = time.perf_counter()
     for i in range(a.steps):
         t0 = time.perf_counter()
@@ -273,6 +275,7 @@ def main():
                    "bvh_builder": a.bvh_builder},
         "scene_create_s": upload_s,
         "autotune": None if tune is None else {k: tune[k] for k in ("max_paths", "grid", "tile", "total_ms")},
+        "bounces": tot["bounces_launched"], "graph_bounces": tot["graph_bounces"],
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "rays_per_s": rays / wall,
         "rays_per_path": rays / max(1, tot["paths"]),

@@ -53,7 +53,8 @@ class Request(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("bounces_launched", C.c_uint64), ("gpu_ms", C.c_double), ("trace_ms", C.c_double),
-                ("trace_launches", C.c_uint64), ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64)]
+                ("trace_launches", C.c_uint64), ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64),
+                ("graph_bounces", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

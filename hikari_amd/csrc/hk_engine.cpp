@@ -32,6 +32,7 @@ void Workspace::ensure(uint32_t paths, uint32_t emitters, uint32_t samples) {
     items.alloc(sizeof(WorkItem) * kMaxItems);
     plan.alloc(sizeof(RegenEntry) * kMaxItems);
     item_done.alloc(sizeof(uint32_t) * kMaxItems);
+    hdr.alloc(sizeof(BounceHdr));
     counters.alloc(64);
     stats.alloc(64);
     h_reqs.alloc(sizeof(DevRequest) * kMaxRequests);
@@ -40,6 +41,7 @@ void Workspace::ensure(uint32_t paths, uint32_t emitters, uint32_t samples) {
     h_done.alloc(sizeof(uint32_t) * kMaxItems);
     h_counters.alloc(64);
     h_zero.alloc(sizeof(uint32_t) * kMaxItems);
+    h_hdr.alloc(sizeof(BounceHdr));
     std::memset(h_zero.p, 0, h_zero.bytes);
     tables = true;
   }
@@ -65,7 +67,7 @@ void Workspace::ensure(uint32_t paths, uint32_t emitters, uint32_t samples) {
 uint64_t Workspace::device_bytes() const {
   const DBuf *all[] = {&sample_L, &q_o[0], &q_o[1], &q_d[0], &q_d[1], &q_beta[0], &q_beta[1], &q_L[0], &q_L[1],
                        &q_aux[0], &q_aux[1], &q_meta[0], &q_meta[1], &hit, &s_ent, &sh_o, &sh_d, &sh_c,
-                       &counters, &stats, &reqs, &items, &plan, &item_done};
+                       &counters, &stats, &reqs, &items, &plan, &item_done, &hdr};
   uint64_t b = 0;
   for (const DBuf *x : all) b += x->bytes;
   return b;
@@ -74,9 +76,9 @@ uint64_t Workspace::device_bytes() const {
 void Workspace::release() {
   DBuf *all[] = {&sample_L, &q_o[0], &q_o[1], &q_d[0], &q_d[1], &q_beta[0], &q_beta[1], &q_L[0], &q_L[1],
                  &q_aux[0], &q_aux[1], &q_meta[0], &q_meta[1], &hit, &s_ent, &sh_o, &sh_d, &sh_c, &counters,
-                 &stats, &reqs, &items, &plan, &item_done};
+                 &stats, &reqs, &items, &plan, &item_done, &hdr};
   for (DBuf *b : all) b->release();
-  PinnedBuf *pin[] = {&h_reqs, &h_items, &h_plan, &h_done, &h_counters, &h_zero};
+  PinnedBuf *pin[] = {&h_reqs, &h_items, &h_plan, &h_done, &h_counters, &h_zero, &h_hdr};
   for (PinnedBuf *b : pin) b->release();
   cap_paths = cap_emitters = cap_samples = 0;
   tables = false;
@@ -102,10 +104,13 @@ Engine::Engine(hk_scene *s, Workspace &ws, hipStream_t st, uint32_t pool_paths, 
   free_ranges_[0] = cap_samples_;
   hip_check(hipEventCreate(&ev_start_), "hipEventCreate");
   hip_check(hipEventCreate(&ev_end_), "hipEventCreate");
+  use_graph_ = s->ctx->graph;
   reset_stats();
 }
 
 Engine::~Engine() {
+  for (hipGraphExec_t &x : gexec_)
+    if (x) (void)hipGraphExecDestroy(x);
   if (ev_start_) (void)hipEventDestroy(ev_start_);
   if (ev_end_) (void)hipEventDestroy(ev_end_);
 }
@@ -122,6 +127,7 @@ RenderArgs Engine::args() const {
   A.reqs = ws_.reqs.as<DevRequest>();
   A.items = ws_.items.as<WorkItem>();
   A.plan = ws_.plan.as<RegenEntry>();
+  A.hdr = ws_.hdr.as<BounceHdr>();
   A.n_paths = pool_;
   A.item_done = ws_.item_done.as<uint32_t>();
   A.sample_L = ws_.sample_L.as<float4>();
@@ -271,7 +277,6 @@ std::vector<RequestRec *> Engine::bounce() {
 
 bool Engine::launch() {
   admit();
-  RenderArgs A = args();
   // plan: new samples for the free places of the queue, oldest chunks first
   uint32_t room = pool_ - in_flight_, n_new = 0, n_plan = 0;
   RegenEntry *plan = ws_.h_plan.as<RegenEntry>();
@@ -285,38 +290,90 @@ bool Engine::launch() {
     n_new += take;
     room -= take;
   }
-  if (!timing_) {
-    hip_check(hipEventRecord(ev_start_, st_), "event");
-    timing_ = true;
-  }
-  if (n_new) {
-    hip_check(hipMemcpyAsync(ws_.plan.p, plan, sizeof(RegenEntry) * n_plan, hipMemcpyHostToDevice, st_),
-              "plan upload");
-    A.n_plan = n_plan;
-    hip_check(launch_regen(A, qi_, in_flight_, n_new, st_), "k_regen");
-    in_flight_ += n_new;
-    paths_ += n_new;
-  }
-  if (in_flight_ == 0) {
+  if (in_flight_ + n_new == 0) {
     // nothing runs: every admitted chunk must have been resolved by now
     if (!active_.empty() || !pending_.empty())
       throw HkError(HK_ERR_INVALID, "scheduler stalled with %zu chunks active, %zu requests pending",
                     active_.size(), pending_.size());
     return false;
   }
-  uint32_t g = kernel_grid(in_flight_, s_->ctx->grid);
+  if (!timing_) {
+    hip_check(hipEventRecord(ev_start_, st_), "event");
+    timing_ = true;
+  }
+  *ws_.h_hdr.as<BounceHdr>() = BounceHdr{in_flight_, n_new, n_plan, 0u};
+  // Long renders replay the bounce as a captured graph (one per queue
+  // parity) once a few bounces have run eagerly; short ones stay eager, and
+  // so do bounces with less than half a wavefront (the graph's grids are
+  // sized for a full one).
+  bool full = uint64_t(in_flight_ + n_new) * 2 >= pool_;
+  if (use_graph_ && full && bounces_ >= kGraphAfter && !gexec_[qi_]) capture(qi_);
+  if (full && gexec_[qi_]) {
+    hip_check(hipGraphLaunch(gexec_[qi_], st_), "bounce graph launch");
+    graph_bounces_++;
+  } else {
+    enqueue_bounce(qi_, false, n_plan, in_flight_ + n_new);
+  }
+  in_flight_ += n_new;
+  paths_ += n_new;
+  launched_ = true;
+  return true;
+}
+
+// The work of one bounce on st_: bounce header (and plan) upload, k_regen,
+// k_extend, k_shade, k_shadow, and the read-back of the queue counters and
+// per-chunk done counts.  graph: the form captured into a graph, whose
+// launches and copies must serve every bounce (k_regen always runs, grids and
+// the plan copy are sized for the full wavefront); otherwise sized for this
+// bounce (n_plan plan entries, `paths` paths in the queue after k_regen).
+void Engine::enqueue_bounce(int qi, bool graph, uint32_t n_plan, uint32_t paths) {
+  RenderArgs A = args();
+  const BounceHdr *hh = ws_.h_hdr.as<BounceHdr>();
+  hip_check(hipMemcpyAsync(ws_.hdr.p, hh, sizeof(BounceHdr), hipMemcpyHostToDevice, st_), "header upload");
+  bool regen = graph || hh->n_new > 0;
+  if (regen) {
+    size_t plan_bytes = sizeof(RegenEntry) * (graph ? kMaxItems : n_plan);
+    if (plan_bytes)
+      hip_check(hipMemcpyAsync(ws_.plan.p, ws_.h_plan.p, plan_bytes, hipMemcpyHostToDevice, st_), "plan upload");
+    hip_check(launch_regen(A, qi, graph ? pool_ : hh->n_new, st_), "k_regen");
+  }
+  uint32_t g = kernel_grid(graph ? pool_ : paths, s_->ctx->grid);
   if (profile_) hip_check(hipEventRecord(ev0_, st_), "event");
-  hip_check(launch_extend(A, qi_, g, st_), "k_extend");
+  hip_check(launch_extend(A, qi, g, st_), "k_extend");
   if (profile_) hip_check(hipEventRecord(ev1_, st_), "event");
-  hip_check(launch_shade(A, qi_, g, st_), "k_shade");
+  hip_check(launch_shade(A, qi, g, st_), "k_shade");
   // the shadow queue holds at most one entry per shaded path: size the grid
   // for that bound, the kernel reads the real count
-  hip_check(launch_shadow(A, qi_ ^ 1, g, st_), "k_shadow");
+  hip_check(launch_shadow(A, qi ^ 1, g, st_), "k_shadow");
   hip_check(hipMemcpyAsync(ws_.h_counters.p, A.counters, 16, hipMemcpyDeviceToHost, st_), "counters D2H");
   hip_check(hipMemcpyAsync(ws_.h_done.p, A.item_done, sizeof(uint32_t) * kMaxItems, hipMemcpyDeviceToHost, st_),
             "item counts D2H");
-  launched_ = true;
-  return true;
+}
+
+// Captures enqueue_bounce(qi, graph form) on st_ and instantiates it.  Nothing
+// captured runs during the capture; a capture or instantiation the runtime
+// refuses leaves the engine eager (the bounce is then enqueued directly).
+void Engine::capture(int qi) {
+  hipGraph_t g = nullptr;
+  bool ok = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (ok) {
+    try {
+      enqueue_bounce(qi, true, kMaxItems, pool_);
+    } catch (const HkError &) {
+      ok = false;
+    }
+    hipGraph_t got = nullptr;
+    if (hipStreamEndCapture(st_, &got) != hipSuccess) ok = false;
+    g = got;
+  }
+  if (ok && hipGraphInstantiate(&gexec_[qi], g, nullptr, nullptr, 0) != hipSuccess) ok = false;
+  if (g) (void)hipGraphDestroy(g);
+  if (!ok) {
+    (void)hipGetLastError();
+    if (gexec_[qi]) (void)hipGraphExecDestroy(gexec_[qi]);
+    gexec_[qi] = nullptr;
+    use_graph_ = false;
+  }
 }
 
 std::vector<RequestRec *> Engine::collect() {
@@ -404,7 +461,7 @@ void Engine::run() {
 
 void Engine::reset_stats() {
   hip_check(hipMemsetAsync(ws_.stats.p, 0, 64, st_), "stats clear");
-  paths_ = bounces_ = trace_launches_ = 0;
+  paths_ = bounces_ = trace_launches_ = graph_bounces_ = 0;
   trace_ms_ = 0;
   timing_ = false;
 }
@@ -426,6 +483,7 @@ void Engine::take_stats(hk_render_stats *st) {
   st->trace_launches += trace_launches_;
   st->node_visits += hs[2] + hs[4];
   st->triangle_tests += hs[3] + hs[5];
+  st->graph_bounces += graph_bounces_;
 }
 
 }  // namespace hk

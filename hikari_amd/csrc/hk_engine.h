@@ -90,6 +90,9 @@ class Engine {
   void resolve_ready(std::vector<RequestRec *> &finished);
   void finalize(RequestRec *r);
   RenderArgs args() const;
+  void enqueue_bounce(int qi, bool graph, uint32_t n_plan, uint32_t paths);
+  void capture(int qi);
+  static constexpr uint64_t kGraphAfter = 4;   // eager bounces before capturing
 
   hk_scene *s_;
   Workspace &ws_;
@@ -107,10 +110,12 @@ class Engine {
   int qi_ = 0;
   bool launched_ = false;
   // statistics
-  uint64_t paths_ = 0, bounces_ = 0, trace_launches_ = 0;
+  uint64_t paths_ = 0, bounces_ = 0, trace_launches_ = 0, graph_bounces_ = 0;
   double trace_ms_ = 0;
   hipEvent_t ev_start_ = nullptr, ev_end_ = nullptr;
   bool timing_ = false;
+  bool use_graph_ = true;
+  hipGraphExec_t gexec_[2] = {nullptr, nullptr};  // the bounce, captured per queue parity
 };
 
 }  // namespace hk

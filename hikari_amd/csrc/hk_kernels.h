@@ -6,6 +6,13 @@
 
 namespace hk {
 
+// Per-bounce parameters of k_regen, read from device memory so that one
+// captured graph serves every bounce: the new samples go to queue positions
+// q_base .. q_base + n_new - 1, described by the first n_plan plan entries.
+struct BounceHdr {
+  uint32_t q_base, n_new, n_plan, pad;
+};
+
 // Everything the bounce kernels touch (passed by value to every kernel).
 // Work arrives as chunks (WorkItem, one slot each); each bounce k_regen
 // starts the samples listed in `plan` behind the continuing paths of the
@@ -19,7 +26,7 @@ struct RenderArgs {
   const DevRequest *reqs;    // request table (by request slot)
   const WorkItem *items;     // item table (by item slot)
   const RegenEntry *plan;    // this bounce's new samples
-  uint32_t n_plan;
+  const BounceHdr *hdr;      // this bounce's k_regen parameters
   uint32_t n_paths;          // queue capacity (paths in flight)
   uint32_t *item_done;       // finished samples per item slot
   float4 *sample_L;          // final radiance per sample (rgb, unused)
@@ -40,7 +47,8 @@ struct RenderArgs {
   unsigned long long *stats;     // [0] extension rays, [1] shadow rays, [2..5] traversal counters
 };
 
-hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t n_new, hipStream_t st);
+// grid_items: the most samples the launch may have to start (its grid)
+hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t grid_items, hipStream_t st);
 hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st);
 hipError_t launch_shade(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st);
 hipError_t launch_shadow(const RenderArgs &A, int qn, uint32_t grid, hipStream_t st);

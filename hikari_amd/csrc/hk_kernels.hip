@@ -560,14 +560,16 @@ __device__ __forceinline__ uint32_t tile_rank(uint32_t x, uint32_t y, uint32_t W
   return y0 * W + x0 * hb + (y - y0) * wb + (x - x0);
 }
 
-__global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi, uint32_t q_base, uint32_t n_new) {
+__global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi) {
+  const BounceHdr h = *A.hdr;
+  const uint32_t q_base = h.q_base, n_new = h.n_new;
   uint32_t stride = gridDim.x * blockDim.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     A.counters[qi] = q_base + n_new;
     A.counters[4] = 0u;  // k_extend's fetch counter (persistent build)
   }
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_new; j += stride) {
-    const RegenEntry e = A.plan[find_plan(A.plan, A.n_plan, j)];
+    const RegenEntry e = A.plan[find_plan(A.plan, h.n_plan, j)];
     const WorkItem it = A.items[e.slot];
     const DevRequest &rq = A.reqs[it.req];
     uint32_t local = e.local0 + (j - e.prefix);
@@ -1201,8 +1203,8 @@ hipError_t launch_bsdf_batch(const DevMaterial *mats, const float *rft, int idx,
   return hipGetLastError();
 }
 
-hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t q_base, uint32_t n_new, hipStream_t st) {
-  hipLaunchKernelGGL(k_regen, dim3(grid_for(n_new, 1u << 16)), dim3(kBlock), 0, st, A, qi, q_base, n_new);
+hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t grid_items, hipStream_t st) {
+  hipLaunchKernelGGL(k_regen, dim3(grid_for(grid_items, 1u << 16)), dim3(kBlock), 0, st, A, qi);
   return hipGetLastError();
 }
 // Blocks of a kernel that are resident at once on the current device (the

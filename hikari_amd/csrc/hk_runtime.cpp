@@ -455,6 +455,8 @@ int hk_set_option(hk_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "grid") {
     if (value < 0 || value > (1 << 20)) throw HkError(HK_ERR_INVALID, "grid out of range");
     ctx->grid = uint32_t(value);
+  } else if (n == "graph") {
+    ctx->graph = value != 0;
   } else if (n == "tile") {
     if (!(value == 0 || value == 1 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32))
       throw HkError(HK_ERR_INVALID, "tile: 1, 2, 4, 8, 16 or 32");

@@ -68,8 +68,9 @@ struct Workspace {
   DBuf q_o[2], q_d[2], q_beta[2], q_L[2], q_aux[2], q_meta[2], hit;
   DBuf s_ent, sh_o, sh_d, sh_c;
   DBuf counters, stats;
-  DBuf reqs, items, plan, item_done;               // request / item tables, regen plan, per-item counts
-  PinnedBuf h_reqs, h_items, h_plan, h_done, h_counters, h_zero;  // host mirrors / staging
+  DBuf reqs, items, plan, item_done, hdr;          // request / item tables, regen plan, per-item counts,
+                                                   // bounce header
+  PinnedBuf h_reqs, h_items, h_plan, h_done, h_counters, h_zero, h_hdr;  // host mirrors / staging
   void ensure(uint32_t paths, uint32_t emitters, uint32_t samples);
   uint64_t device_bytes() const;
   void release();
@@ -89,6 +90,7 @@ struct hk_ctx {
   int bvh_builder = 0;  // 0 host binned SAH, 1 device LBVH
   bool resolve_tiles = true;  // film resolve through LDS tiles (false: the plain gather)
   uint32_t tile = 8;          // camera-ray tile edge (1: scanlines)
+  bool graph = true;          // replay each bounce as a captured HIP graph
   bool profile = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   uint32_t *host_counters = nullptr;  // pinned

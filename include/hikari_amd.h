@@ -172,6 +172,7 @@ typedef struct hk_render_stats {
   double trace_ms;        /* device time inside the closest-hit kernel */
   uint64_t trace_launches;
   uint64_t node_visits, triangle_tests;  /* BVH work; non-zero only in builds with HK_TRAVERSAL_STATS */
+  uint64_t graph_bounces;                /* bounces replayed from a captured HIP graph */
 } hk_render_stats;
 
 /* Whole-image render with the gpuPathMIS integrator (the wavefront form of
@@ -252,6 +253,8 @@ int hk_server_destroy(hk_server *server);
  *                 floor(max_samples / pixels) samples per pixel, resolved
  *                 into the film when their segment ends
  *   "grid"        cap on blocks per queue-kernel launch (0: one lane per item)
+ *   "graph"       1 (default): after a few eager bounces a render replays
+ *                 each bounce as a captured HIP graph; 0: eager launches
  *   "tile"        edge of the pixel tiles camera rays start in (1, 2, 4, 8,
  *                 16 or 32; 1: scanlines; default 8)
  *   "profile"     1: time the closest-hit kernel with HIP events

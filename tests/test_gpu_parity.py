@@ -392,3 +392,36 @@ def test_autotune_keeps_results():
         assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
     finally:
         c.close()
+
+
+def test_graph_replays_equal_eager():
+    """Bounces replayed from the captured graphs (two per engine, one per queue
+    parity, replayed many times with eager admission, resolve and read-back
+    work between the replays) give the films of eager launches bit for bit,
+    for a single render, a batch and with profiling events inside the graph."""
+    sc = scenes.cbox(width=48, height=40, sphere_subdiv=2)
+    reqs = [hk.make_request(sc, spp=4 + 3 * i, seed=60 + i) for i in range(5)]
+    out = {}
+    for graph in (0, 1):
+        for profile in (0, 1):
+            c = hk.Context(0)
+            c.set_option("graph", graph)
+            c.set_option("profile", profile)
+            c.set_option("max_paths", 3000)  # many bounces per render
+            gs = c.scene(sc)
+            img, st = gs.render(24, seed=9)
+            imgs, bst = gs.render_batch(reqs)
+            print("graph", graph, "profile", profile, "bounces", st["bounces_launched"], "graph bounces",
+                  st["graph_bounces"], "trace_ms", st["trace_ms"])
+            if graph:
+                assert st["graph_bounces"] > 20 and bst["graph_bounces"] > 20
+            else:
+                assert st["graph_bounces"] == 0
+            if profile:
+                assert st["trace_ms"] > 0
+            out[(graph, profile)] = [img] + imgs
+            c.close()
+    ref = out[(0, 0)]
+    for key, ims in out.items():
+        for a, b in zip(ims, ref):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), key
