@@ -104,7 +104,9 @@ Engine::Engine(hk_scene *s, Workspace &ws, hipStream_t st, uint32_t pool_paths, 
   free_ranges_[0] = cap_samples_;
   hip_check(hipEventCreate(&ev_start_), "hipEventCreate");
   hip_check(hipEventCreate(&ev_end_), "hipEventCreate");
-  use_graph_ = s->ctx->graph;
+  // k_extend's profiling events need eager launches (an event recorded by a
+  // graph node carries no usable timestamp for hipEventElapsedTime)
+  use_graph_ = s->ctx->graph && !profile_;
   reset_stats();
 }
 

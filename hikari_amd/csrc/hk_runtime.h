@@ -43,7 +43,7 @@ struct DevBvh {
 // floats on the host.  Synchronizes `st`.
 void build_bvh4_device(hipStream_t st, const float *tri9, uint32_t n, DevBvh &out);
 
-constexpr uint32_t kDefaultMaxPaths = 1u << 24;     // path slots of the wavefront (~350 B of HBM each)
+constexpr uint32_t kDefaultMaxPaths = 1u << 25;     // path slots of the wavefront (~250-350 B of HBM each)
 constexpr uint32_t kDefaultMaxSamples = 1u << 28;   // samples per segment (16 B each in HBM)
 constexpr int kMaxEmitters = 32;
 

@@ -246,7 +246,7 @@ int hk_server_get_stats(hk_server *server, hk_server_stats *stats);
 int hk_server_destroy(hk_server *server);
 
 /* Tuning knobs (value 0 restores the default):
- *   "max_paths"   paths in flight (default 16M); at every bounce finished
+ *   "max_paths"   paths in flight (default 32M); at every bounce finished
  *                 paths are replaced by the next samples
  *   "max_samples" samples per segment (default 256M, 16 B of HBM each): a
  *                 request's samples are cut into chunks of
@@ -255,6 +255,7 @@ int hk_server_destroy(hk_server *server);
  *   "grid"        cap on blocks per queue-kernel launch (0: one lane per item)
  *   "graph"       1 (default): after a few eager bounces a render replays
  *                 each bounce as a captured HIP graph; 0: eager launches
+ *                 (profiled renders always launch eagerly)
  *   "tile"        edge of the pixel tiles camera rays start in (1, 2, 4, 8,
  *                 16 or 32; 1: scanlines; default 8)
  *   "profile"     1: time the closest-hit kernel with HIP events

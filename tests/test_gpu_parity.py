@@ -398,7 +398,7 @@ def test_graph_replays_equal_eager():
     """Bounces replayed from the captured graphs (two per engine, one per queue
     parity, replayed many times with eager admission, resolve and read-back
     work between the replays) give the films of eager launches bit for bit,
-    for a single render, a batch and with profiling events inside the graph."""
+    for a single render and a batch."""
     sc = scenes.cbox(width=48, height=40, sphere_subdiv=2)
     reqs = [hk.make_request(sc, spp=4 + 3 * i, seed=60 + i) for i in range(5)]
     out = {}
@@ -413,9 +413,9 @@ def test_graph_replays_equal_eager():
             imgs, bst = gs.render_batch(reqs)
             print("graph", graph, "profile", profile, "bounces", st["bounces_launched"], "graph bounces",
                   st["graph_bounces"], "trace_ms", st["trace_ms"])
-            if graph:
+            if graph and not profile:
                 assert st["graph_bounces"] > 20 and bst["graph_bounces"] > 20
-            else:
+            else:  # profiled renders launch eagerly (their events time k_extend)
                 assert st["graph_bounces"] == 0
             if profile:
                 assert st["trace_ms"] > 0
