@@ -425,3 +425,84 @@ def test_graph_replays_equal_eager():
     for key, ims in out.items():
         for a, b in zip(ims, ref):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), key
+
+
+def _degenerate_rays(rng, n):
+    """Rays with NaN / infinite / zero / subnormal components and empty,
+    inverted, negative or NaN [tmin, tmax] ranges, mixed with ordinary ones."""
+    r = box_rays(rng, n, -0.9, 0.9)
+    k = np.arange(n) % 12
+    nan, inf = np.float32(np.nan), np.float32(np.inf)
+    r[k == 0, 0] = nan
+    r[k == 1, 4] = nan
+    r[k == 2, 5] = inf
+    r[k == 3, 4:7] = 0.0
+    r[k == 4, 3], r[k == 4, 7] = 5.0, 1.0
+    r[k == 5, 3] = r[k == 5, 7]
+    r[k == 6, 7] = -1.0
+    r[k == 7, 3] = -inf
+    r[k == 8, 4:7] = [1e-38, 0.0, 0.0]
+    r[k == 9, 0] = 1e30
+    r[k == 10, 7] = nan
+    r[k == 11, 3] = nan
+    return r
+
+
+def test_degenerate_rays_match_brute_force(ctx):
+    """NaN, infinite, zero and subnormal ray components and empty or invalid
+    t ranges: the GPU terminates and agrees with the brute-force oracle bit for
+    bit (closest hit) and on hit / miss (any hit)."""
+    rng = np.random.default_rng(41)
+    sc = scenes.cbox()
+    v, f = scene_vf(sc)
+    acc = ctx.accel(v, f)
+    tris = scene_triangles(sc)
+    rays = _degenerate_rays(rng, 24000)
+    assert _hits_equal(acc.intersect(rays), O.intersect(tris, rays, use_bvh=False)).all()
+    g = acc.intersect(rays, any_hit=True)
+    o = O.intersect(tris, rays, any_hit=True, use_bvh=False)
+    assert np.array_equal(g[3] >= 0, o[3] >= 0)
+
+
+def test_degenerate_triangles_match_brute_force(ctx):
+    """Zero-area triangles (repeated vertices, a point, axis-aligned collinear
+    vertices: FP32 determinant exactly 0) inside a mesh of ordinary ones:
+    never hit, and every hit equals brute force bit for bit.  (Triangles that
+    are collinear only up to rounding can have a tiny non-zero determinant and
+    report Moller-Trumbore hits away from the triangle, which no bounding-box
+    hierarchy finds; the reference's own BVH misses them as well.)"""
+    rng = np.random.default_rng(42)
+    tris = rng.uniform(-1, 1, (3000, 9)).astype(np.float32)
+    tris[::7, 3:6] = tris[::7, 0:3]                                   # p1 == p0
+    col = tris[1::7]
+    col[:, 3:6] = col[:, 0:3] + np.array([0.5, 0, 0], np.float32)    # p0, p1, p2 on a line along x
+    col[:, 6:9] = col[:, 0:3] + np.array([1.0, 0, 0], np.float32)
+    tris[1::7] = col
+    tris[2::7, :] = np.tile(tris[2::7, 0:3], 3)                       # a point
+    v = tris.reshape(-1, 3)
+    f = np.arange(len(v), dtype=np.uint32).reshape(-1, 3)
+    acc = ctx.accel(v, f)
+    rays = np.concatenate([box_rays(rng, 40000, -1.2, 1.2), grazing_rays(rng, tris, 20000)])
+    o = O.intersect(tris, rays, use_bvh=False)
+    deg = np.zeros(len(tris), bool)
+    deg[0::7] = deg[1::7] = deg[2::7] = True
+    assert not deg[o[3][o[3] >= 0]].any()
+    assert _hits_equal(acc.intersect(rays), o).all()
+
+
+def test_invalid_samples_are_dropped_like_the_oracle(ctx):
+    """A sphere with NaN vertex normals makes the samples whose paths touch it
+    invalid (NaN radiance); ImageBlock::Put drops such samples
+    (src/core/Block.cpp:97-101), so pixels keep their finite samples only and
+    the GPU film equals the oracle's at equal streams."""
+    sc = scenes.cbox(width=32, height=32, sphere_subdiv=2)
+    sphere = sc.meshes[7]
+    sphere.n = np.full_like(np.asarray(sphere.v, np.float32), np.nan)
+    img, _ = ctx.scene(sc).render(8, seed=2)
+    ref, acc, _ = O.render(sc, req_seed=2, spp=(0, 8))
+    clean, _, _ = O.render(scenes.cbox(width=32, height=32, sphere_subdiv=2), req_seed=2, spp=(0, 8))
+    assert np.isfinite(img).all() and np.isfinite(ref).all()
+    assert not np.allclose(ref, clean) and lum(ref).mean() > 0.01  # samples were dropped, others kept
+    # pixels whose samples were all dropped are black on both sides
+    assert np.array_equal(img.sum(-1) == 0, ref.sum(-1) == 0) and (ref.sum(-1) == 0).any()
+    assert rel_mse(img, ref) < 1e-4
