@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--spp", type=int, default=256)
     ap.add_argument("--alpha", type=float, default=0.3)
     ap.add_argument("--max-paths", type=int, default=0)
+    ap.add_argument("--refill", type=int, default=0, help="k_extend refill threshold, 1..64 (0: library default)")
+    ap.add_argument("--refill-shadow", type=int, default=0, help="k_shadow refill threshold (0: library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="1: replay bounces from captured HIP graphs")
     ap.add_argument("--autotune", type=int, default=0,
@@ -223,6 +225,8 @@ def main():
         ctx.set_option("max_paths", a.max_paths)
     ctx.set_option("bvh_builder", 1 if a.bvh_builder == "device" else 0)
     ctx.set_option("graph", a.graph)
+    ctx.set_option("refill", a.refill)
+    ctx.set_option("refill_shadow", a.refill_shadow)
     t = time.time()
     gs = ctx.scene(scene)
     upload_s = time.time() - t
@@ -231,10 +235,10 @@ def main():
     tune = None
     if a.autotune and not a.max_paths:
         tune = gs.autotune(a.spp, seed=999)
-        print(f"# autotune: max_paths {tune['max_paths']}, tile {tune['tile']}, grid {tune['grid']} in "
-              f"{tune['total_ms']:.0f} ms: " + ", ".join(f"({t['max_paths']}, {t['tile']}, {t['grid']}) "
-                                                       f"{t['samples_per_s']:.3e}" for t in tune["tried"]),
-              file=sys.stderr, flush=True)
+        print(f"# autotune: max_paths {tune['max_paths']}, refill {tune['refill']}/{tune['refill_shadow']}, "
+              f"tile {tune['tile']}, grid {tune['grid']} in {tune['total_ms']:.0f} ms: "
+              + ", ".join(f"({t['max_paths']}, {t['refill']}/{t['refill_shadow']}, {t['tile']}, {t['grid']}) "
+                          f"{t['samples_per_s']:.3e}" for t in tune["tried"]), file=sys.stderr, flush=True)
     for i in range(a.warmup):
         _, st = gs.render(a.spp, seed=1000 + i)
         print(f"# warmup {i}: {st['gpu_ms']:.1f} ms", file=sys.stderr, flush=True)
@@ -296,7 +300,8 @@ def main():
                    "spp": a.spp, "width": scene.width, "height": scene.height, "triangles": scene.n_triangles,
                    "bvh_builder": a.bvh_builder},
         "scene_create_s": upload_s,
-        "autotune": None if tune is None else {k: tune[k] for k in ("max_paths", "grid", "tile", "total_ms")},
+        "autotune": None if tune is None else {k: tune[k] for k in ("max_paths", "refill", "refill_shadow", "grid",
+                                                                    "tile", "total_ms")},
         "bounces": tot["bounces_launched"], "graph_bounces": tot["graph_bounces"],
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "rays_per_s": rays / wall,

@@ -131,6 +131,8 @@ RenderArgs Engine::args() const {
   A.plan = ws_.plan.as<RegenEntry>();
   A.hdr = ws_.hdr.as<BounceHdr>();
   A.n_paths = pool_;
+  A.refill_extend = s_->ctx->refill_extend;
+  A.refill_shadow = s_->ctx->refill_shadow;
   A.item_done = ws_.item_done.as<uint32_t>();
   A.sample_L = ws_.sample_L.as<float4>();
   for (int i = 0; i < 2; i++) {

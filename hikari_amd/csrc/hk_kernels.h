@@ -28,6 +28,8 @@ struct RenderArgs {
   const RegenEntry *plan;    // this bounce's new samples
   const BounceHdr *hdr;      // this bounce's k_regen parameters
   uint32_t n_paths;          // queue capacity (paths in flight)
+  uint32_t refill_extend;    // persistent k_extend / k_shadow: idle lanes of a wave that
+  uint32_t refill_shadow;    // trigger a fetch of new queue entries (1..64)
   uint32_t *item_done;       // finished samples per item slot
   float4 *sample_L;          // final radiance per sample (rgb, unused)
   // per queue entry

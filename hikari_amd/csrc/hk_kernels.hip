@@ -603,16 +603,14 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi) {
 
 // ------------------------------------------------------------------ k_extend
 // Closest hit of every queued ray (Scene::RayIntersect, src/core/Scene.cpp:
-// 117-120).  Persistent lanes: a wave keeps tracing and, whenever at least kRefill of
-// its lanes have finished their rays, fetches that many new queue entries
-// with one atomic (counters[4], cleared by k_regen / k_shade before), so a
-// few long rays no longer hold a whole wave.  HK_PLAIN_TRACE builds trace
-// one ray per lane instead (~6% slower on the 1.3M-triangle sphere).
-#ifndef HK_REFILL
-#define HK_REFILL 16
-#endif
-constexpr int kRefill = HK_REFILL;
-static_assert(kRefill >= 1 && kRefill <= 64, "refill threshold must fit a wave");
+// 117-120).  Persistent lanes: a wave keeps tracing and, whenever at least
+// A.refill_extend of its lanes have finished their rays, fetches that many
+// new queue entries with one atomic (counters[4], cleared by k_regen /
+// k_shade before), so a few long rays no longer hold a whole wave.  The
+// threshold is a run-time option (1..64): fetching rarely keeps the wave's
+// lanes in step, fetching often keeps them busy; 48-64 is fastest on the
+// benchmark scenes (16: -20%).  HK_PLAIN_TRACE builds trace one ray per
+// lane instead.
 constexpr uint32_t kNoRay = 0xffffffffu;
 
 #ifndef HK_PLAIN_TRACE
@@ -638,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   for (;;) {
     unsigned long long idle = __ballot(rid == kNoRay);
     int n_idle = __popcll(idle);
-    if (!exhausted && n_idle >= kRefill) {
+    if (!exhausted && n_idle >= (int)A.refill_extend) {
       uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
       uint32_t base = 0;
       if (lane == leader) base = atomicAdd(&A.counters[4], (uint32_t)n_idle);
@@ -948,7 +946,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   for (;;) {
     unsigned long long idle = __ballot(ent == kNoRay);
     int n_idle = __popcll(idle);
-    if (!exhausted && n_idle >= kRefill) {
+    if (!exhausted && n_idle >= (int)A.refill_shadow) {
       uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
       uint32_t base = 0;
       if (lane == leader) base = atomicAdd(&A.counters[5], (uint32_t)n_idle);

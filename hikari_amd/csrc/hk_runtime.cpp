@@ -455,6 +455,10 @@ int hk_set_option(hk_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "grid") {
     if (value < 0 || value > (1 << 20)) throw HkError(HK_ERR_INVALID, "grid out of range");
     ctx->grid = uint32_t(value);
+  } else if (n == "refill" || n == "refill_shadow") {
+    if (value < 0 || value > 64) throw HkError(HK_ERR_INVALID, "%s: 1..64 (0: default)", name);
+    uint32_t v = value == 0 ? 56u : uint32_t(value);
+    (n == "refill" ? ctx->refill_extend : ctx->refill_shadow) = v;
   } else if (n == "graph") {
     ctx->graph = value != 0;
   } else if (n == "tile") {

@@ -302,8 +302,9 @@ int hk_render_batch(hk_scene *s, const hk_request *reqs, uint32_t n, float **out
 }
 
 // Coordinate search: wavefront size first (the largest effect), then the
-// camera-ray tile edge, then the queue-kernel grid cap, each with the best
-// setting found so far for the others.  Every calibration render goes
+// refill thresholds of k_extend and k_shadow, the camera-ray tile edge and
+// the queue-kernel grid cap, each with the best setting found so far for the
+// others.  Every calibration render goes
 // through hk_render_batch with the candidate in the context options.
 int hk_autotune(hk_scene *s, const hk_request *req, uint64_t calib_samples, hk_tune_result *out) {
   hk_ctx *ctx = s ? s->ctx : nullptr;
@@ -321,12 +322,15 @@ int hk_autotune(hk_scene *s, const hk_request *req, uint64_t calib_samples, hk_t
   std::vector<float> film(size_t(npix) * 3);
   float *films[1] = {film.data()};
   uint32_t best_paths = ctx->max_paths, best_grid = ctx->grid, best_tile = ctx->tile;
+  uint32_t best_re = ctx->refill_extend, best_rs = ctx->refill_shadow;
   bool saved_profile = ctx->profile;
   ctx->profile = false;
-  auto measure = [&](uint32_t paths, uint32_t grid, uint32_t tile) {
+  auto measure = [&](uint32_t paths, uint32_t grid, uint32_t tile, uint32_t re, uint32_t rs) {
     ctx->max_paths = paths;
     ctx->grid = grid;
     ctx->tile = tile;
+    ctx->refill_extend = re;
+    ctx->refill_shadow = rs;
     hk_render_stats st;
     int rc = hk_render_batch(s, &cal, 1, films, 0, &st);
     if (rc != HK_OK) throw HkError(rc, "calibration render: %s", ctx->err.c_str());
@@ -336,28 +340,40 @@ int hk_autotune(hk_scene *s, const hk_request *req, uint64_t calib_samples, hk_t
       out->tried_max_paths[k] = paths;
       out->tried_grid[k] = grid;
       out->tried_tile[k] = tile;
+      out->tried_refill[k] = re;
+      out->tried_refill_shadow[k] = rs;
       out->samples_per_s[k] = rate;
     }
     return rate;
   };
   try {
-    measure(best_paths, best_grid, best_tile);  // warm-up: workspace, code and caches
+    measure(best_paths, best_grid, best_tile, best_re, best_rs);  // warm-up: workspace, code and caches
     out->n_tried = 0;
     double best = 0.0;
     uint64_t total = npix * cal.spp;
     for (uint32_t p : {1u << 22, 1u << 23, 1u << 24, 1u << 25}) {
       if (p > total && p != (1u << 22)) break;  // a wavefront larger than the work is never filled
-      double r = measure(p, best_grid, best_tile);
+      double r = measure(p, best_grid, best_tile, best_re, best_rs);
       if (r > best) { best = r; best_paths = p; }
+    }
+    for (uint32_t re : {32u, 48u, 56u, 64u}) {
+      if (re == best_re) continue;
+      double r = measure(best_paths, best_grid, best_tile, re, best_rs);
+      if (r > best) { best = r; best_re = re; }
+    }
+    for (uint32_t rs : {32u, 48u, 56u, 64u}) {
+      if (rs == best_rs) continue;
+      double r = measure(best_paths, best_grid, best_tile, best_re, rs);
+      if (r > best) { best = r; best_rs = rs; }
     }
     for (uint32_t t : {1u, 4u, 8u, 16u}) {
       if (t == best_tile) continue;
-      double r = measure(best_paths, best_grid, t);
+      double r = measure(best_paths, best_grid, t, best_re, best_rs);
       if (r > best) { best = r; best_tile = t; }
     }
     for (uint32_t g : {0u, uint32_t(ctx->n_cu) * 8u, uint32_t(ctx->n_cu) * 32u}) {
       if (g == best_grid) continue;
-      double r = measure(best_paths, g, best_tile);
+      double r = measure(best_paths, g, best_tile, best_re, best_rs);
       if (r > best) { best = r; best_grid = g; }
     }
   } catch (...) {
@@ -368,10 +384,14 @@ int hk_autotune(hk_scene *s, const hk_request *req, uint64_t calib_samples, hk_t
   ctx->max_paths = best_paths;
   ctx->grid = best_grid;
   ctx->tile = best_tile;
+  ctx->refill_extend = best_re;
+  ctx->refill_shadow = best_rs;
   ctx->ws.release();  // the next render sizes the workspace for the chosen wavefront
   out->max_paths = best_paths;
   out->grid = best_grid;
   out->tile = best_tile;
+  out->refill = best_re;
+  out->refill_shadow = best_rs;
   out->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return HK_OK;
   HK_API_END(ctx)

@@ -358,15 +358,19 @@ def test_image_vs_high_spp_fp64_oracle(ctx, which):
 
 
 def test_tuning_options_do_not_change_bits():
-    """Camera-ray tile edge, queue-kernel grid cap and wavefront size (the
-    autotuner's knobs) leave every film bit unchanged (ragged 70 x 45 film)."""
+    """Camera-ray tile edge, queue-kernel grid cap, wavefront size and refill
+    thresholds (the autotuner's knobs) leave every film bit unchanged (ragged
+    70 x 45 film)."""
     sc = scenes.cbox(width=70, height=45, sphere_subdiv=2)
     imgs = []
-    for tile, grid, max_paths in [(8, 0, 0), (1, 0, 0), (4, 64, 0), (16, 0, 5000), (32, 7, 999)]:
+    for tile, grid, max_paths, refill in [(8, 0, 0, 0), (1, 0, 0, 1), (4, 64, 0, 64), (16, 0, 5000, 17),
+                                          (32, 7, 999, 40)]:
         c = hk.Context(0)
         c.set_option("tile", tile)
         c.set_option("grid", grid)
         c.set_option("max_paths", max_paths)
+        c.set_option("refill", refill)
+        c.set_option("refill_shadow", 65 - refill if refill else 0)
         img, _ = c.scene(sc).render(5, seed=12)
         imgs.append(img)
         c.close()
@@ -375,9 +379,9 @@ def test_tuning_options_do_not_change_bits():
 
 
 def test_autotune_keeps_results():
-    """hk_autotune tries wavefront sizes, tile edges and grid caps on a
-    calibration slice, keeps the fastest in the context options, and renders
-    after tuning equal those before."""
+    """hk_autotune tries wavefront sizes, refill thresholds, tile edges and
+    grid caps on a calibration slice, keeps the fastest in the context
+    options, and renders after tuning equal those before."""
     c = hk.Context(0)
     try:
         sc = scenes.cbox(width=96, height=64, sphere_subdiv=2)
@@ -386,7 +390,8 @@ def test_autotune_keeps_results():
         t = gs.autotune(16, calib_samples=96 * 64 * 16)
         print("autotune:", t)
         assert len(t["tried"]) >= 5
-        assert (t["max_paths"], t["grid"], t["tile"]) in {(x["max_paths"], x["grid"], x["tile"]) for x in t["tried"]}
+        keys = ("max_paths", "grid", "tile", "refill", "refill_shadow")
+        assert tuple(t[k] for k in keys) in {tuple(x[k] for k in keys) for x in t["tried"]}
         assert all(x["samples_per_s"] > 0 for x in t["tried"])
         after, _ = gs.render(16, seed=3)
         assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
