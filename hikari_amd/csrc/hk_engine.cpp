@@ -1,8 +1,8 @@
 // hk_engine.cpp -- the wavefront scheduler (see hk_engine.h for the model).
 // Per bounce the host launches k_regen (new samples from the plan), k_extend,
-// k_shade and k_shadow on the engine's stream, then reads back three queue
-// counters and the per-chunk finished counts (one 16 KiB copy) to plan the
-// next bounce and to resolve finished chunks.
+// k_shade and k_shadow on the engine's stream, then reads back the queue
+// counters and the per-chunk finished counts (one 128 KiB copy: eight partial
+// counts per chunk) to plan the next bounce and to resolve finished chunks.
 #include "hk_engine.h"
 
 #include <algorithm>
@@ -31,18 +31,16 @@ void Workspace::ensure(uint32_t paths, uint32_t emitters, uint32_t samples) {
     reqs.alloc(sizeof(DevRequest) * kMaxRequests);
     items.alloc(sizeof(WorkItem) * kMaxItems);
     plan.alloc(sizeof(RegenEntry) * kMaxItems);
-    item_done.alloc(sizeof(uint32_t) * kMaxItems);
+    item_done.alloc(sizeof(uint32_t) * kMaxItems * kDoneWays);
     hdr.alloc(sizeof(BounceHdr));
-    counters.alloc(64);
+    counters.alloc(sizeof(uint32_t) * kCounterWords);
     stats.alloc(64);
     h_reqs.alloc(sizeof(DevRequest) * kMaxRequests);
     h_items.alloc(sizeof(WorkItem) * kMaxItems);
     h_plan.alloc(sizeof(RegenEntry) * kMaxItems);
-    h_done.alloc(sizeof(uint32_t) * kMaxItems);
-    h_counters.alloc(64);
-    h_zero.alloc(sizeof(uint32_t) * kMaxItems);
+    h_done.alloc(sizeof(uint32_t) * kMaxItems * kDoneWays);
+    h_counters.alloc(sizeof(uint32_t) * (ctr(1) + 1));
     h_hdr.alloc(sizeof(BounceHdr));
-    std::memset(h_zero.p, 0, h_zero.bytes);
     tables = true;
   }
   if (samples > cap_samples) {
@@ -78,7 +76,7 @@ void Workspace::release() {
                  &q_aux[0], &q_aux[1], &q_meta[0], &q_meta[1], &hit, &s_ent, &sh_o, &sh_d, &sh_c, &counters,
                  &stats, &reqs, &items, &plan, &item_done, &hdr};
   for (DBuf *b : all) b->release();
-  PinnedBuf *pin[] = {&h_reqs, &h_items, &h_plan, &h_done, &h_counters, &h_zero, &h_hdr};
+  PinnedBuf *pin[] = {&h_reqs, &h_items, &h_plan, &h_done, &h_counters, &h_hdr};
   for (PinnedBuf *b : pin) b->release();
   cap_paths = cap_emitters = cap_samples = 0;
   tables = false;
@@ -263,8 +261,7 @@ void Engine::admit() {
       hw->buf_base = base;
       hip_check(hipMemcpyAsync(ws_.items.as<WorkItem>() + slot, hw, sizeof(WorkItem), hipMemcpyHostToDevice, st_),
                 "item upload");
-      hip_check(hipMemcpyAsync(ws_.item_done.as<uint32_t>() + slot, ws_.h_zero.p, 4, hipMemcpyHostToDevice, st_),
-                "item counter clear");
+      hip_check(launch_clear_done(ws_.item_done.as<uint32_t>(), slot, st_), "item counter clear");
       r->chunk_item[r->next_admit] = int32_t(slot);
       if (r->t_admit < 0) r->t_admit = now_ms();
       r->next_admit++;
@@ -349,8 +346,10 @@ void Engine::enqueue_bounce(int qi, bool graph, uint32_t n_plan, uint32_t paths)
   // the shadow queue holds at most one entry per shaded path: size the grid
   // for that bound, the kernel reads the real count
   hip_check(launch_shadow(A, qi ^ 1, g, st_), "k_shadow");
-  hip_check(hipMemcpyAsync(ws_.h_counters.p, A.counters, 16, hipMemcpyDeviceToHost, st_), "counters D2H");
-  hip_check(hipMemcpyAsync(ws_.h_done.p, A.item_done, sizeof(uint32_t) * kMaxItems, hipMemcpyDeviceToHost, st_),
+  hip_check(hipMemcpyAsync(ws_.h_counters.p, A.counters, sizeof(uint32_t) * (ctr(1) + 1), hipMemcpyDeviceToHost, st_),
+            "counters D2H");
+  hip_check(hipMemcpyAsync(ws_.h_done.p, A.item_done, sizeof(uint32_t) * kMaxItems * kDoneWays, hipMemcpyDeviceToHost,
+                           st_),
             "item counts D2H");
 }
 
@@ -393,11 +392,14 @@ std::vector<RequestRec *> Engine::collect() {
   }
   bounces_++;
   qi_ ^= 1;
-  in_flight_ = ws_.h_counters.as<uint32_t>()[qi_];
+  in_flight_ = ws_.h_counters.as<uint32_t>()[ctr(uint32_t(qi_))];
   const uint32_t *done = ws_.h_done.as<uint32_t>();
   for (uint32_t slot : active_) {
     ChunkRec &c = slots_[slot];
-    if (!c.complete && done[slot] == c.size) c.complete = true;
+    if (c.complete) continue;
+    uint32_t n = 0;
+    for (uint32_t w = 0; w < kDoneWays; w++) n += done[w * kMaxItems + slot];
+    if (n == c.size) c.complete = true;
   }
   resolve_ready(finished);
   return finished;

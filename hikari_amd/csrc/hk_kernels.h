@@ -19,8 +19,8 @@ struct BounceHdr {
 // extension queue.  Path state lives in the extension queues (SoA,
 // ping-pong): k_shade moves a continuing path from queue qi to the compacted
 // queue qi^1; a finished path writes its radiance to the sample buffer and
-// counts itself in item_done[slot], so the host knows when a chunk is
-// complete and can be resolved into its film.
+// counts itself in item_done, so the host knows when a chunk is complete and
+// can be resolved into its film.
 struct RenderArgs {
   DevScene sc;
   const DevRequest *reqs;    // request table (by request slot)
@@ -30,7 +30,8 @@ struct RenderArgs {
   uint32_t n_paths;          // queue capacity (paths in flight)
   uint32_t refill_extend;    // persistent k_extend / k_shadow: idle lanes of a wave that
   uint32_t refill_shadow;    // trigger a fetch of new queue entries (1..64)
-  uint32_t *item_done;       // finished samples per item slot
+  uint32_t *item_done;       // finished samples per item slot, in kDoneWays partial counts:
+                             // slot s, way w at w * kMaxItems + s (the host sums the ways)
   float4 *sample_L;          // final radiance per sample (rgb, unused)
   // per queue entry
   float4 *q_o[2];      // ray origin, tmin
@@ -45,9 +46,22 @@ struct RenderArgs {
   // per (emitter, source index) rays
   uint4 *s_ent;
   float4 *sh_o, *sh_d, *sh_c;
-  uint32_t *counters;            // [0],[1] extension queues, [2] shadow queue, [4],[5] fetch counters
+  uint32_t *counters;            // at ctr(k): [0],[1] extension queues, [2] shadow queue, [4],[5] fetch
+                                 // counters
   unsigned long long *stats;     // [0] extension rays, [1] shadow rays, [2..5] traversal counters
 };
+
+// Counter k of the bounce kernels lives at word ctr(k), 256 B from the
+// next: atomics to one memory line serialize at the memory-side atomic unit
+// even when they target different words, and k_shade bumps several counters
+// per block.
+constexpr uint32_t kCtrStride = 64;
+__host__ __device__ constexpr uint32_t ctr(uint32_t k) { return k * kCtrStride; }
+constexpr uint32_t kCounterWords = ctr(8);
+// A chunk's finished-sample count is split over kDoneWays words 16 KiB
+// apart (a block adds to way blockIdx & (kDoneWays - 1)), so that the waves
+// of a large chunk do not all serialize on one memory line.
+constexpr uint32_t kDoneWays = 8;
 
 // grid_items: the most samples the launch may have to start (its grid)
 hipError_t launch_regen(const RenderArgs &A, int qi, uint32_t grid_items, hipStream_t st);
@@ -60,6 +74,13 @@ hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, const DevRequest &
                           hipStream_t st);
 hipError_t launch_finalize(const void *accum, float *rgb, uint32_t n, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st);
+// zeroes every way of item slot `slot`'s finished count
+hipError_t launch_clear_done(uint32_t *item_done, uint32_t slot, hipStream_t st);
+// Shading records of nt triangles from the global vertex arrays (N / UV may
+// be null), face indices and per-triangle mesh ids.
+hipError_t launch_shade_tris(const float *V, const float *N, const float *UV, const uint32_t *F,
+                             const uint32_t *prim_mesh, const DevMesh *meshes, ShadeTri *out, uint32_t nt,
+                             hipStream_t st);
 
 hipError_t launch_intersect(const TraceNode *nodes, const TraceTri *tris, int32_t root, const void *rays, void *hits,
                             uint32_t n, int any, hipStream_t st);

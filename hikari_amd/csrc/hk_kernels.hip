@@ -349,6 +349,42 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
   return pred ? base + (uint32_t)__popcll(below) : 0xffffffffu;
 }
 
+// Block-aggregated append of two predicates to two counters: one atomic per
+// block and counter (the waves' counts are summed in LDS); slots in lane
+// order within a wave and wave order within the block.  Every thread of the
+// block must call it.  Returns the slots (0xffffffff where a predicate is
+// false).
+constexpr uint32_t kWaves = kBlock / 64;
+struct BlockAppend {
+  uint32_t cnt[2][kWaves];
+  uint32_t base[2];
+};
+__device__ __forceinline__ void block_append2(BlockAppend &sh, uint32_t *c0, bool p0, uint32_t *c1, bool p1,
+                                              uint32_t &s0, uint32_t &s1) {
+  unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
+  uint32_t w = threadIdx.x >> 6, lane = lane_id();
+  if (lane == 0) {
+    sh.cnt[0][w] = (uint32_t)__popcll(m0);
+    sh.cnt[1][w] = (uint32_t)__popcll(m1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint32_t k = threadIdx.x, tot = 0;
+    for (uint32_t v = 0; v < kWaves; v++) tot += sh.cnt[k][v];
+    sh.base[k] = tot ? atomicAdd(k == 0 ? c0 : c1, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t b0 = sh.base[0], b1 = sh.base[1];
+  for (uint32_t v = 0; v < w; v++) {
+    b0 += sh.cnt[0][v];
+    b1 += sh.cnt[1][v];
+  }
+  unsigned long long below = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
+  s0 = p0 ? b0 + (uint32_t)__popcll(m0 & below) : 0xffffffffu;
+  s1 = p1 ? b1 + (uint32_t)__popcll(m1 & below) : 0xffffffffu;
+  __syncthreads();  // sh is reused by the next call
+}
+
 // Adds a lane's traversal counters to stats[0] (node visits), stats[1]
 // (triangle tests): one atomic per wave.
 __device__ __forceinline__ void count_flush(unsigned long long *stats, TraceCount tc) {
@@ -374,10 +410,13 @@ __device__ __forceinline__ uint32_t find_plan(const RegenEntry *plan, uint32_t n
   return lo;
 }
 
-// Adds one to item_done[slot] for every lane with pred set: one atomic per
-// distinct slot in the wave.  Every lane of the wave must call it.
+// Adds one to slot's finished count for every lane with pred set: one atomic
+// per distinct slot in the wave, into the block's way of the count.  Every
+// lane of the wave must call it.
 __device__ __forceinline__ void count_done(uint32_t *item_done, bool pred, uint32_t slot) {
   unsigned long long m = __ballot(pred);
+  if (m == 0ull) return;
+  item_done += (blockIdx.x & (kDoneWays - 1u)) * kMaxItems;
   uint32_t lane = lane_id();
   while (m) {
     int leader = __ffsll((long long)m) - 1;
@@ -428,11 +467,11 @@ __device__ __forceinline__ vec3 area_sample(const DevScene &sc, const DevMesh &m
   int tri = cdf_find(sc.cdf + m.cdf_off, (int)m.nt, s1);
   float sq = sqrtf(1.0f - sx);
   float al = 1.0f - sq, be = sy * sq, ga = 1.0f - al - be;
-  const uint32_t *f = sc.F + 3 * size_t(m.first_prim + tri);
-  vec3 p0 = ld3(sc.V + 3 * size_t(f[0])), p1 = ld3(sc.V + 3 * size_t(f[1])), p2 = ld3(sc.V + 3 * size_t(f[2]));
+  const ShadeTri &st = sc.shade[m.first_prim + tri];
+  vec3 p0 = ld3(st.p), p1 = ld3(st.p + 3), p2 = ld3(st.p + 6);
   er.p = (ga * p0 + al * p1) + be * p2;
   if (m.has_n) {
-    er.n = (ga * ld3(sc.N + 3 * size_t(f[0])) + al * ld3(sc.N + 3 * size_t(f[1]))) + be * ld3(sc.N + 3 * size_t(f[2]));
+    er.n = (ga * ld3(st.n) + al * ld3(st.n + 3)) + be * ld3(st.n + 6);
   } else {
     er.n = normalized(cross(p1 - p0, p2 - p0));
   }
@@ -488,24 +527,25 @@ __device__ __forceinline__ vec3 env_eval(const DevScene &sc, vec3 wi) {
 // Triangle::PostIntersect (src/core/Mesh.cpp:67-172): position and shading frame.
 __device__ __forceinline__ void post_intersect(const DevScene &sc, int32_t prim, float u, float v, vec3 &P, Frame &sh,
                                                uint32_t &mesh_id) {
-  mesh_id = sc.prim_mesh[prim];
-  const DevMesh &m = sc.meshes[mesh_id];
+  // the whole 128-B record in seven independent 16-B loads (one line)
+  const float4 *rec = reinterpret_cast<const float4 *>(sc.shade + prim);
+  float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3], r4 = rec[4], r5 = rec[5], r6 = rec[6];
+  mesh_id = __float_as_uint(r6.x);
+  uint32_t attr = __float_as_uint(r6.z);
   float b0 = 1.0f - (u + v), b1 = u, b2 = v;
-  const uint32_t *f = sc.F + 3 * size_t(prim);
-  uint32_t i0 = f[0], i1 = f[1], i2 = f[2];
-  vec3 p0 = ld3(sc.V + 3 * size_t(i0)), p1 = ld3(sc.V + 3 * size_t(i1)), p2 = ld3(sc.V + 3 * size_t(i2));
+  vec3 p0 = v3(r0.x, r0.y, r0.z), p1 = v3(r0.w, r1.x, r1.y), p2 = v3(r1.z, r1.w, r2.x);
   P = (b0 * p0 + b1 * p1) + b2 * p2;
-  if (m.has_n) {
-    vec3 ns = (b0 * ld3(sc.N + 3 * size_t(i0)) + b1 * ld3(sc.N + 3 * size_t(i1))) + b2 * ld3(sc.N + 3 * size_t(i2));
-    if (m.has_uv) {
+  if (attr & kShadeHasN) {
+    vec3 n0 = v3(r2.y, r2.z, r2.w), n1 = v3(r3.x, r3.y, r3.z), n2 = v3(r3.w, r4.x, r4.y);
+    vec3 ns = (b0 * n0 + b1 * n1) + b2 * n2;
+    if (attr & kShadeHasUV) {
       vec3 dp1 = p1 - p0, dp2 = p2 - p0;
       if (norm(cross(dp1, dp2)) == 0.0f) {
         sh = frame_from_n(normalized(cross(p1 - p0, p2 - p0)));
         return;
       }
-      const float *uv = sc.UV;
-      float du1x = uv[2 * i1] - uv[2 * i0], du1y = uv[2 * i1 + 1] - uv[2 * i0 + 1];
-      float du2x = uv[2 * i2] - uv[2 * i0], du2y = uv[2 * i2 + 1] - uv[2 * i0 + 1];
+      float du1x = r5.x - r4.z, du1y = r5.y - r4.w;
+      float du2x = r5.z - r4.z, du2y = r5.w - r4.w;
       float det = du1x * du2y - du1y * du2x;
       if (det == 0.0f) {
         sh = frame_from_n(normalized(ns));
@@ -565,8 +605,8 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi) {
   const uint32_t q_base = h.q_base, n_new = h.n_new;
   uint32_t stride = gridDim.x * blockDim.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    A.counters[qi] = q_base + n_new;
-    A.counters[4] = 0u;  // k_extend's fetch counter (persistent build)
+    A.counters[ctr(qi)] = q_base + n_new;
+    A.counters[ctr(4)] = 0u;  // k_extend's fetch counter (persistent build)
   }
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_new; j += stride) {
     const RegenEntry e = A.plan[find_plan(A.plan, h.n_plan, j)];
@@ -616,11 +656,11 @@ constexpr uint32_t kNoRay = 0xffffffffu;
 #ifndef HK_PLAIN_TRACE
 __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
-  uint32_t count = A.counters[qi];
+  uint32_t count = A.counters[ctr(qi)];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    A.counters[qi ^ 1] = 0u;
-    A.counters[2] = 0u;
-    A.counters[5] = 0u;  // k_shadow's fetch counter
+    A.counters[ctr(qi ^ 1)] = 0u;
+    A.counters[ctr(2)] = 0u;
+    A.counters[ctr(5)] = 0u;  // k_shadow's fetch counter
     atomicAdd(&A.stats[0], (unsigned long long)count);
   }
   const float4 *qo = A.q_o[qi], *qd = A.q_d[qi];
@@ -639,7 +679,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
     if (!exhausted && n_idle >= (int)A.refill_extend) {
       uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
       uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(&A.counters[4], (uint32_t)n_idle);
+      if (lane == leader) base = atomicAdd(&A.counters[ctr(4)], (uint32_t)n_idle);
       base = __shfl(base, (int)leader);
       if (base + (uint32_t)n_idle >= count) exhausted = true;
       if (rid == kNoRay) {
@@ -671,11 +711,11 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
 #else
 __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
-  uint32_t count = A.counters[qi];
+  uint32_t count = A.counters[ctr(qi)];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    A.counters[qi ^ 1] = 0u;
-    A.counters[2] = 0u;
-    A.counters[5] = 0u;  // k_shadow's fetch counter (persistent build)
+    A.counters[ctr(qi ^ 1)] = 0u;
+    A.counters[ctr(2)] = 0u;
+    A.counters[ctr(5)] = 0u;  // k_shadow's fetch counter (persistent build)
     atomicAdd(&A.stats[0], (unsigned long long)count);
   }
   uint32_t stride = gridDim.x * blockDim.x;
@@ -707,8 +747,8 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
 __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
 #endif
   const DevScene &sc = A.sc;
-  uint32_t count = A.counters[qi];
-  if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[4] = 0u;  // k_extend's fetch counter
+  uint32_t count = A.counters[ctr(qi)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[ctr(4)] = 0u;  // k_extend's fetch counter
   int qn = qi ^ 1;
   uint32_t stride = gridDim.x * blockDim.x;
 #ifndef HK_NO_MATERIAL_SORT
@@ -719,12 +759,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
   __shared__ uint32_t s_cnt[kSortKeys];
   __shared__ uint32_t s_perm[kBlock];
 #endif
+  __shared__ BlockAppend s_app;
   for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
 #ifndef HK_NO_MATERIAL_SORT
     uint32_t key = kSortKeys - 1;
     if (base + threadIdx.x < count) {
       int32_t hp = __float_as_int(A.hit[base + threadIdx.x].w);
-      key = hp < 0 ? kSortKeys - 2 : min(sc.meshes[sc.prim_mesh[hp]].material, kSortKeys - 3);
+      key = hp < 0 ? kSortKeys - 2 : min(sc.shade[hp].material, kSortKeys - 3);
     }
     if (threadIdx.x < kSortKeys) s_cnt[threadIdx.x] = 0u;
     __syncthreads();
@@ -896,7 +937,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
       }
       meta_out = (depth & kDepthMask) | (flags << kFlagShift) | (islot << kSlotShift);
     }
-    uint32_t slot = wave_append(&A.counters[qn], cont);
+    uint32_t slot, ss;
+    block_append2(s_app, &A.counters[ctr(qn)], cont, &A.counters[ctr(2)], smask != 0u, slot, ss);
     if (cont) {  // the path moves to the compacted next queue
       A.q_o[qn][slot] = make_float4(P.x, P.y, P.z, 1e-4f);
       A.q_d[qn][slot] = make_float4(nd.x, nd.y, nd.z, kInfF);
@@ -912,7 +954,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
     uint32_t sbuf = A.items[islot].buf_base + local;
     if (fin) A.sample_L[sbuf] = make_float4(L.x, L.y, L.z, 0.0f);
     count_done(A.item_done, fin && smask == 0u, islot);
-    uint32_t ss = wave_append(&A.counters[2], smask != 0u);
     if (smask != 0u)
       A.s_ent[ss] = cont ? make_uint4(i, slot, smask, 0u) : make_uint4(i, sbuf, smask, islot | kShadowLast);
   }
@@ -929,7 +970,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
 #ifndef HK_PLAIN_TRACE
 __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
-  uint32_t count = A.counters[2];
+  uint32_t count = A.counters[ctr(2)];
   uint32_t traced = 0;
   TraceCount tc;
   RayTrav R;
@@ -949,7 +990,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
     if (!exhausted && n_idle >= (int)A.refill_shadow) {
       uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
       uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(&A.counters[5], (uint32_t)n_idle);
+      if (lane == leader) base = atomicAdd(&A.counters[ctr(5)], (uint32_t)n_idle);
       base = __shfl(base, (int)leader);
       if (base + (uint32_t)n_idle >= count) exhausted = true;
       if (ent == kNoRay) {
@@ -1006,10 +1047,11 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   count_flush(A.stats + 4, tc);
 #endif
 }
+
 #else
 __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
-  uint32_t count = A.counters[2];
+  uint32_t count = A.counters[ctr(2)];
   uint32_t stride = gridDim.x * blockDim.x;
   uint32_t traced = 0;
   TraceCount tc;
@@ -1170,6 +1212,39 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const float4 *accum, float 
   }
 }
 
+// One thread per triangle: gathers the face's vertex attributes into its
+// 128-B shading record, written as eight 16-B stores.
+__global__ __launch_bounds__(kBlock) void k_shade_tris(const float *__restrict__ V, const float *__restrict__ N,
+                                                       const float *__restrict__ UV, const uint32_t *__restrict__ F,
+                                                       const uint32_t *__restrict__ prim_mesh,
+                                                       const DevMesh *__restrict__ meshes, ShadeTri *__restrict__ out,
+                                                       uint32_t nt) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  uint32_t mi = prim_mesh[t];
+  const DevMesh &m = meshes[mi];
+  bool has_n = m.has_n && N, has_uv = m.has_uv && UV;
+  float r[32];
+  for (int k = 0; k < 3; k++) {
+    uint32_t i = F[3 * size_t(t) + k];
+    for (int a = 0; a < 3; a++) {
+      r[3 * k + a] = V[3 * size_t(i) + a];
+      r[9 + 3 * k + a] = has_n ? N[3 * size_t(i) + a] : 0.0f;
+    }
+    for (int a = 0; a < 2; a++) r[18 + 2 * k + a] = has_uv ? UV[2 * size_t(i) + a] : 0.0f;
+  }
+  r[24] = __uint_as_float(mi);
+  r[25] = __uint_as_float(m.material);
+  r[26] = __uint_as_float((has_n ? kShadeHasN : 0u) | (has_uv ? kShadeHasUV : 0u));
+  for (int k = 27; k < 32; k++) r[k] = 0.0f;
+  float4 *o = reinterpret_cast<float4 *>(out + t);
+  for (int k = 0; k < 8; k++) o[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+}
+
+__global__ void k_clear_done(uint32_t *item_done, uint32_t slot) {
+  if (threadIdx.x < kDoneWays) item_done[threadIdx.x * kMaxItems + slot] = 0u;
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1247,6 +1322,19 @@ hipError_t launch_finalize(const void *accum, float *rgb, uint32_t n, hipStream_
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_clear_done(uint32_t *item_done, uint32_t slot, hipStream_t st) {
+  if (slot >= kMaxItems) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(64), 0, st, item_done, slot);
+  return hipGetLastError();
+}
+hipError_t launch_shade_tris(const float *V, const float *N, const float *UV, const uint32_t *F,
+                             const uint32_t *prim_mesh, const DevMesh *meshes, ShadeTri *out, uint32_t nt,
+                             hipStream_t st) {
+  if (nt == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shade_tris, dim3((nt + kBlock - 1) / kBlock), dim3(kBlock), 0, st, V, N, UV, F, prim_mesh,
+                     meshes, out, nt);
   return hipGetLastError();
 }
 

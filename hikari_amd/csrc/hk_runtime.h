@@ -45,7 +45,6 @@ void build_bvh4_device(hipStream_t st, const float *tri9, uint32_t n, DevBvh &ou
 
 constexpr uint32_t kDefaultMaxPaths = 1u << 25;     // path slots of the wavefront (~250-350 B of HBM each)
 constexpr uint32_t kDefaultMaxSamples = 1u << 28;   // samples per segment (16 B each in HBM)
-constexpr int kMaxEmitters = 32;
 
 // Owned pinned host allocation (staging for small per-bounce copies).
 struct PinnedBuf {
@@ -70,7 +69,7 @@ struct Workspace {
   DBuf counters, stats;
   DBuf reqs, items, plan, item_done, hdr;          // request / item tables, regen plan, per-item counts,
                                                    // bounce header
-  PinnedBuf h_reqs, h_items, h_plan, h_done, h_counters, h_zero, h_hdr;  // host mirrors / staging
+  PinnedBuf h_reqs, h_items, h_plan, h_done, h_counters, h_hdr;  // host mirrors / staging
   void ensure(uint32_t paths, uint32_t emitters, uint32_t samples);
   uint64_t device_bytes() const;
   void release();
@@ -109,7 +108,7 @@ struct hk_scene {
   hk_ctx *ctx = nullptr;
   hk::DevScene dev;
   hk::DevBvh bvh;
-  hk::DBuf V, N, UV, F, prim_mesh, meshes, mats, rft, emitters, cdf;
+  hk::DBuf shade, meshes, mats, rft, emitters, cdf;
   hk::DBuf env_tex, env_row_cdf, env_row_func, env_row_int, env_marg_cdf, env_marg_func;
   hk_scene_desc desc;             // camera / film defaults (pointers not retained)
   uint32_t n_tris = 0;

@@ -219,12 +219,22 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
     throw HkError(HK_ERR_UNSUPPORTED, "%zu emitters (at most %d)", emitters.size(), kMaxEmitters);
 
   upload_bvh(ctx, tri9.data(), prim, s->bvh);
-  s->V.upload(V.data(), V.size() * 4, st);
-  if (any_n) s->N.upload(N.data(), N.size() * 4, st);
-  if (any_uv) s->UV.upload(UV.data(), UV.size() * 4, st);
-  s->F.upload(F.data(), F.size() * 4, st);
-  s->prim_mesh.upload(prim_mesh.data(), prim_mesh.size() * 4, st);
   s->meshes.upload(meshes.data(), meshes.size() * sizeof(DevMesh), st);
+  {
+    // shading records gathered on the device; the indexed arrays are only
+    // the gather's input
+    DBuf dV, dN, dUV, dF, dPM;
+    dV.upload(V.data(), V.size() * 4, st);
+    if (any_n) dN.upload(N.data(), N.size() * 4, st);
+    if (any_uv) dUV.upload(UV.data(), UV.size() * 4, st);
+    dF.upload(F.data(), F.size() * 4, st);
+    dPM.upload(prim_mesh.data(), prim_mesh.size() * 4, st);
+    s->shade.alloc(size_t(prim) * sizeof(ShadeTri));
+    hip_check(launch_shade_tris(dV.as<float>(), dN.as<float>(), dUV.as<float>(), dF.as<uint32_t>(),
+                                dPM.as<uint32_t>(), s->meshes.as<DevMesh>(), s->shade.as<ShadeTri>(), prim, st),
+              "shading records");
+    hip_check(hipStreamSynchronize(st), "shading records");
+  }
   s->mats.upload(mats.data(), mats.size() * sizeof(DevMaterial), st);
   s->rft.upload(pool.data(), pool.size() * 4, st);
   if (!emitters.empty()) s->emitters.upload(emitters.data(), emitters.size() * sizeof(DevEmitter), st);
@@ -232,11 +242,7 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
 
   ds.nodes = s->bvh.nodes.as<TraceNode>();
   ds.tris = s->bvh.tris.as<TraceTri>();
-  ds.V = s->V.as<float>();
-  ds.N = s->N.as<float>();
-  ds.UV = s->UV.as<float>();
-  ds.F = s->F.as<uint32_t>();
-  ds.prim_mesh = s->prim_mesh.as<uint32_t>();
+  ds.shade = s->shade.as<ShadeTri>();
   ds.meshes = s->meshes.as<DevMesh>();
   ds.mats = s->mats.as<DevMaterial>();
   ds.rft = s->rft.as<float>();
@@ -256,7 +262,7 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
   ds.force_background = d->force_background;
   ds.root_ref = 0;
   hip_check(hipStreamSynchronize(st), "scene upload");
-  DBuf *all[] = {&s->bvh.nodes, &s->bvh.tris, &s->V, &s->N, &s->UV, &s->F, &s->prim_mesh, &s->meshes, &s->mats,
+  DBuf *all[] = {&s->bvh.nodes, &s->bvh.tris, &s->shade, &s->meshes, &s->mats,
                  &s->rft, &s->emitters, &s->cdf, &s->env_tex, &s->env_row_cdf, &s->env_row_func, &s->env_row_int,
                  &s->env_marg_cdf, &s->env_marg_func};
   for (DBuf *b : all) s->bytes += b->bytes;

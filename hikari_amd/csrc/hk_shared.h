@@ -91,14 +91,27 @@ struct alignas(16) TraceTri {
 };
 static_assert(sizeof(TraceTri) == 48, "TraceTri layout");
 
+// Shading record of one triangle (128 B, one L2 line), in scene primitive
+// order: the three vertex positions, normals and texture coordinates (zero
+// where the mesh has none), its mesh, material and attribute flags.  A hit
+// reads this one line instead of the face's index triple followed by nine
+// scattered vertex attributes.
+constexpr int kMaxEmitters = 32;  // emitters of one scene (bits of a shadow entry's mask)
+
+constexpr uint32_t kShadeHasN = 1u, kShadeHasUV = 2u;
+struct alignas(128) ShadeTri {
+  float p[9];
+  float n[9];
+  float uv[6];
+  uint32_t mesh, material, flags;
+  uint32_t pad[5];
+};
+static_assert(sizeof(ShadeTri) == 128, "ShadeTri layout");
+
 struct DevScene {
   const TraceNode *nodes;
   const TraceTri *tris;
-  const float *V;          // nv x 3 (global vertex array)
-  const float *N;          // nv x 3 or null
-  const float *UV;         // nv x 2 or null
-  const uint32_t *F;       // nt x 3 global vertex indices
-  const uint32_t *prim_mesh;
+  const ShadeTri *shade;   // nt shading records
   const DevMesh *meshes;
   const DevMaterial *mats;
   const float *rft;        // reduced table pool
