@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--max-paths", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0, help="k_extend refill threshold, 1..64 (0: library default)")
     ap.add_argument("--refill-shadow", type=int, default=0, help="k_shadow refill threshold (0: library default)")
+    ap.add_argument("--fetch-batch", type=int, default=0,
+                    help="queue entries a persistent wave reserves per atomic, in 64s (0: library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="1: replay bounces from captured HIP graphs")
     ap.add_argument("--autotune", type=int, default=0,
@@ -227,6 +229,8 @@ def main():
     ctx.set_option("graph", a.graph)
     ctx.set_option("refill", a.refill)
     ctx.set_option("refill_shadow", a.refill_shadow)
+    if a.fetch_batch:
+        ctx.set_option("fetch_batch", a.fetch_batch)
     t = time.time()
     gs = ctx.scene(scene)
     upload_s = time.time() - t
@@ -235,10 +239,11 @@ def main():
     tune = None
     if a.autotune and not a.max_paths:
         tune = gs.autotune(a.spp, seed=999)
-        print(f"# autotune: max_paths {tune['max_paths']}, refill {tune['refill']}/{tune['refill_shadow']}, "
-              f"tile {tune['tile']}, grid {tune['grid']} in {tune['total_ms']:.0f} ms: "
-              + ", ".join(f"({t['max_paths']}, {t['refill']}/{t['refill_shadow']}, {t['tile']}, {t['grid']}) "
-                          f"{t['samples_per_s']:.3e}" for t in tune["tried"]), file=sys.stderr, flush=True)
+        print(f"# autotune: max_paths {tune['max_paths']}, fetch_batch {tune['fetch_batch']}, refill "
+              f"{tune['refill']}/{tune['refill_shadow']}, tile {tune['tile']}, grid {tune['grid']} in "
+              f"{tune['total_ms']:.0f} ms: "
+              + ", ".join(f"({t['max_paths']}, {t['fetch_batch']}, {t['refill']}/{t['refill_shadow']}, {t['tile']}, "
+                          f"{t['grid']}) {t['samples_per_s']:.3e}" for t in tune["tried"]), file=sys.stderr, flush=True)
     for i in range(a.warmup):
         _, st = gs.render(a.spp, seed=1000 + i)
         print(f"# warmup {i}: {st['gpu_ms']:.1f} ms", file=sys.stderr, flush=True)
@@ -300,8 +305,8 @@ def main():
                    "spp": a.spp, "width": scene.width, "height": scene.height, "triangles": scene.n_triangles,
                    "bvh_builder": a.bvh_builder},
         "scene_create_s": upload_s,
-        "autotune": None if tune is None else {k: tune[k] for k in ("max_paths", "refill", "refill_shadow", "grid",
-                                                                    "tile", "total_ms")},
+        "autotune": None if tune is None else {k: tune[k] for k in ("max_paths", "fetch_batch", "refill",
+                                                                    "refill_shadow", "grid", "tile", "total_ms")},
         "bounces": tot["bounces_launched"], "graph_bounces": tot["graph_bounces"],
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "rays_per_s": rays / wall,

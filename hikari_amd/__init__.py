@@ -222,11 +222,11 @@ TUNE_MAX = 32
 
 class TuneResult(C.Structure):
     _fields_ = [("max_paths", C.c_uint32), ("grid", C.c_uint32), ("tile", C.c_uint32), ("refill", C.c_uint32),
-                ("refill_shadow", C.c_uint32), ("n_tried", C.c_uint32),
+                ("refill_shadow", C.c_uint32), ("fetch_batch", C.c_uint32), ("n_tried", C.c_uint32),
                 ("tried_max_paths", C.c_uint32 * TUNE_MAX), ("tried_grid", C.c_uint32 * TUNE_MAX),
                 ("tried_tile", C.c_uint32 * TUNE_MAX), ("tried_refill", C.c_uint32 * TUNE_MAX),
-                ("tried_refill_shadow", C.c_uint32 * TUNE_MAX), ("samples_per_s", C.c_double * TUNE_MAX),
-                ("total_ms", C.c_double)]
+                ("tried_refill_shadow", C.c_uint32 * TUNE_MAX), ("tried_fetch_batch", C.c_uint32 * TUNE_MAX),
+                ("samples_per_s", C.c_double * TUNE_MAX), ("total_ms", C.c_double)]
 
 
 class GpuScene:
@@ -274,9 +274,11 @@ class GpuScene:
         self.ctx.check(lib().hk_autotune(self.h, C.byref(req), C.c_uint64(calib_samples), C.byref(t)))
         tried = [{"max_paths": t.tried_max_paths[i], "grid": t.tried_grid[i], "tile": t.tried_tile[i],
                   "refill": t.tried_refill[i], "refill_shadow": t.tried_refill_shadow[i],
-                  "samples_per_s": t.samples_per_s[i]} for i in range(t.n_tried)]
+                  "fetch_batch": t.tried_fetch_batch[i], "samples_per_s": t.samples_per_s[i]}
+                 for i in range(t.n_tried)]
         return {"max_paths": t.max_paths, "grid": t.grid, "tile": t.tile, "refill": t.refill,
-                "refill_shadow": t.refill_shadow, "total_ms": t.total_ms, "tried": tried}
+                "refill_shadow": t.refill_shadow, "fetch_batch": t.fetch_batch, "total_ms": t.total_ms,
+                "tried": tried}
 
     def pixel_samples(self, x, y, s0, n, seed=0):
         out = np.zeros((n, 5), np.float32)

@@ -131,6 +131,7 @@ RenderArgs Engine::args() const {
   A.n_paths = pool_;
   A.refill_extend = s_->ctx->refill_extend;
   A.refill_shadow = s_->ctx->refill_shadow;
+  A.fetch_batch = s_->ctx->fetch_batch;
   A.item_done = ws_.item_done.as<uint32_t>();
   A.sample_L = ws_.sample_L.as<float4>();
   for (int i = 0; i < 2; i++) {

@@ -30,6 +30,7 @@ struct RenderArgs {
   uint32_t n_paths;          // queue capacity (paths in flight)
   uint32_t refill_extend;    // persistent k_extend / k_shadow: idle lanes of a wave that
   uint32_t refill_shadow;    // trigger a fetch of new queue entries (1..64)
+  uint32_t fetch_batch;      // persistent kernels: entries a wave reserves per atomic, in 64s
   uint32_t *item_done;       // finished samples per item slot, in kDoneWays partial counts:
                              // slot s, way w at w * kMaxItems + s (the host sums the ways)
   float4 *sample_L;          // final radiance per sample (rgb, unused)

@@ -385,6 +385,65 @@ __device__ __forceinline__ void block_append2(BlockAppend &sh, uint32_t *c0, boo
   __syncthreads();  // sh is reused by the next call
 }
 
+// Per-wave reservation of queue entries for the persistent kernels.  A wave
+// takes batch x 64 entries per atomic on the shared fetch counter (fewer
+// atomics on its memory line) and hands them to its idle lanes over several
+// refills; when little of the queue is left it takes only what its idle
+// lanes need, so that no wave holds a backlog while others run dry.  Entries
+// are handed out exactly once whatever the batch: [next, end) is the wave's
+// reserved, not yet started range.
+struct WaveFetch {
+  uint32_t next, end;
+  uint32_t seen;  // the counter's value at the wave's last reservation
+  bool dry;       // the counter has passed the queue length
+};
+__device__ __forceinline__ void fetch_init(WaveFetch &f) {
+  f.next = f.end = f.seen = 0u;
+  f.dry = false;
+}
+__device__ __forceinline__ bool fetch_exhausted(const WaveFetch &f) { return f.dry && f.next == f.end; }
+// Gives every idle lane (bit set in `idle`, the same mask on all lanes) the
+// next entry, or 0xffffffff when there is none.  Every lane calls it.
+__device__ __forceinline__ uint32_t wave_fetch(WaveFetch &f, uint32_t *counter, uint32_t count, uint32_t batch,
+                                               unsigned long long idle) {
+  uint32_t lane = lane_id();
+  uint32_t n_idle = (uint32_t)__popcll(idle);
+  uint32_t avail = f.end - f.next;
+  uint32_t nlo = 0, nhi = 0;
+  bool fetched = avail < n_idle && !f.dry;
+  if (fetched) {
+    uint32_t want = n_idle - avail;
+    uint32_t left = count - min(f.seen, count);
+    if (batch > 1u && left > gridDim.x * kWaves * batch * 64u) want = max(want, batch * 64u);
+    uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
+    uint32_t b = 0;
+    if (lane == leader) b = atomicAdd(counter, want);
+    b = __shfl(b, (int)leader);
+    f.seen = b;
+    nlo = min(b, count);
+    nhi = min(b + want, count);
+    if (b + want >= count) f.dry = true;
+  }
+  uint32_t take_old = min(avail, n_idle);
+  uint32_t idx = 0xffffffffu;
+  if ((idle >> lane) & 1ull) {
+    uint32_t r = (uint32_t)__popcll(idle & (lane == 0 ? 0ull : ((1ull << lane) - 1ull)));
+    if (r < take_old) {
+      idx = f.next + r;
+    } else {
+      uint32_t j = nlo + (r - take_old);
+      if (j < nhi) idx = j;
+    }
+  }
+  if (fetched) {
+    f.next = min(nlo + (n_idle - take_old), nhi);
+    f.end = nhi;
+  } else {
+    f.next += take_old;
+  }
+  return idx;
+}
+
 // Adds a lane's traversal counters to stats[0] (node visits), stats[1]
 // (triangle tests): one atomic per wave.
 __device__ __forceinline__ void count_flush(unsigned long long *stats, TraceCount tc) {
@@ -671,30 +730,22 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   R.ref = kDone;
   int32_t pend = kDone;  // parked leaf
   uint32_t rid = kNoRay;
-  bool exhausted = false;
-  uint32_t lane = lane_id();
+  WaveFetch wf;
+  fetch_init(wf);
   for (;;) {
     unsigned long long idle = __ballot(rid == kNoRay);
     int n_idle = __popcll(idle);
-    if (!exhausted && n_idle >= (int)A.refill_extend) {
-      uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(&A.counters[ctr(4)], (uint32_t)n_idle);
-      base = __shfl(base, (int)leader);
-      if (base + (uint32_t)n_idle >= count) exhausted = true;
-      if (rid == kNoRay) {
-        unsigned long long below = lane == 0 ? 0ull : (idle & ((1ull << lane) - 1ull));
-        uint32_t idx = base + (uint32_t)__popcll(below);
-        if (idx < count) {
-          rid = idx;
-          float4 a = qo[idx], b = qd[idx];
-          trav_begin(R, stack, priv, A.sc.root_ref, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w);
-          pend = kDone;
-        }
+    if (!fetch_exhausted(wf) && n_idle >= (int)A.refill_extend) {
+      uint32_t idx = wave_fetch(wf, &A.counters[ctr(4)], count, A.fetch_batch, idle);
+      if (idx != kNoRay) {
+        rid = idx;
+        float4 a = qo[idx], b = qd[idx];
+        trav_begin(R, stack, priv, A.sc.root_ref, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w);
+        pend = kDone;
       }
     }
     if (__ballot(rid != kNoRay) == 0ull) {
-      if (exhausted) break;
+      if (fetch_exhausted(wf)) break;
       continue;  // every lane idle: the next round fetches a full wave
     }
     trav_round<false>(A.sc.nodes, A.sc.tris, R, pend, tc);
@@ -981,38 +1032,32 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   float4 *lp = nullptr;
   vec3 L = v3(0.0f), c = v3(0.0f);
   float lw = 0.0f;
-  bool exhausted = false, start = false, last = false;
+  bool start = false, last = false;
   uint32_t islot = 0;
   uint32_t lane = lane_id();
+  WaveFetch wf;
+  fetch_init(wf);
   for (;;) {
     unsigned long long idle = __ballot(ent == kNoRay);
     int n_idle = __popcll(idle);
-    if (!exhausted && n_idle >= (int)A.refill_shadow) {
-      uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(&A.counters[ctr(5)], (uint32_t)n_idle);
-      base = __shfl(base, (int)leader);
-      if (base + (uint32_t)n_idle >= count) exhausted = true;
-      if (ent == kNoRay) {
-        unsigned long long below = lane == 0 ? 0ull : (idle & ((1ull << lane) - 1ull));
-        uint32_t idx = base + (uint32_t)__popcll(below);
-        if (idx < count) {
-          ent = idx;
-          uint4 e4 = A.s_ent[idx];
-          src = e4.x;
-          mask = e4.z;
-          last = (e4.w & kShadowLast) != 0u;
-          islot = e4.w & ~kShadowLast;
-          lp = last ? &A.sample_L[e4.y] : &A.q_L[qn][e4.y];
-          float4 l4 = *lp;
-          L = v3(l4.x, l4.y, l4.z);
-          lw = l4.w;
-          start = true;
-        }
+    if (!fetch_exhausted(wf) && n_idle >= (int)A.refill_shadow) {
+      uint32_t idx = wave_fetch(wf, &A.counters[ctr(5)], count, A.fetch_batch, idle);
+      if (idx != kNoRay) {
+        ent = idx;
+        uint4 e4 = A.s_ent[idx];
+        src = e4.x;
+        mask = e4.z;
+        last = (e4.w & kShadowLast) != 0u;
+        islot = e4.w & ~kShadowLast;
+        lp = last ? &A.sample_L[e4.y] : &A.q_L[qn][e4.y];
+        float4 l4 = *lp;
+        L = v3(l4.x, l4.y, l4.z);
+        lw = l4.w;
+        start = true;
       }
     }
     if (__ballot(ent != kNoRay) == 0ull) {
-      if (exhausted) break;
+      if (fetch_exhausted(wf)) break;
       continue;
     }
     if (start) {  // the entry's next emitter ray

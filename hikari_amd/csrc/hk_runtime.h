@@ -90,8 +90,9 @@ struct hk_ctx {
   bool resolve_tiles = true;  // film resolve through LDS tiles (false: the plain gather)
   uint32_t tile = 8;          // camera-ray tile edge (1: scanlines)
   bool graph = true;          // replay each bounce as a captured HIP graph
-  uint32_t refill_extend = 56;  // idle lanes that make a persistent k_extend wave fetch rays
-  uint32_t refill_shadow = 56;  // the same for k_shadow
+  uint32_t refill_extend = 24;  // idle lanes that make a persistent k_extend wave fetch rays
+  uint32_t refill_shadow = 24;  // the same for k_shadow
+  uint32_t fetch_batch = 2;     // queue entries a persistent wave reserves per atomic, in 64s
   bool profile = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   uint32_t *host_counters = nullptr;  // pinned

@@ -308,9 +308,9 @@ int hk_render_batch(hk_scene *s, const hk_request *reqs, uint32_t n, float **out
 }
 
 // Coordinate search: wavefront size first (the largest effect), then the
-// refill thresholds of k_extend and k_shadow, the camera-ray tile edge and
-// the queue-kernel grid cap, each with the best setting found so far for the
-// others.  Every calibration render goes
+// fetch batch and refill thresholds of the persistent kernels, the camera-ray
+// tile edge and the queue-kernel grid cap, each with the best setting found
+// so far for the others.  Every calibration render goes
 // through hk_render_batch with the candidate in the context options.
 int hk_autotune(hk_scene *s, const hk_request *req, uint64_t calib_samples, hk_tune_result *out) {
   hk_ctx *ctx = s ? s->ctx : nullptr;
@@ -327,77 +327,101 @@ int hk_autotune(hk_scene *s, const hk_request *req, uint64_t calib_samples, hk_t
   cal.spp_offset = 0;
   std::vector<float> film(size_t(npix) * 3);
   float *films[1] = {film.data()};
-  uint32_t best_paths = ctx->max_paths, best_grid = ctx->grid, best_tile = ctx->tile;
-  uint32_t best_re = ctx->refill_extend, best_rs = ctx->refill_shadow;
+  struct Setting {
+    uint32_t paths, grid, tile, re, rs, fb;
+  };
+  Setting best{ctx->max_paths, ctx->grid, ctx->tile, ctx->refill_extend, ctx->refill_shadow, ctx->fetch_batch};
   bool saved_profile = ctx->profile;
   ctx->profile = false;
-  auto measure = [&](uint32_t paths, uint32_t grid, uint32_t tile, uint32_t re, uint32_t rs) {
-    ctx->max_paths = paths;
-    ctx->grid = grid;
-    ctx->tile = tile;
-    ctx->refill_extend = re;
-    ctx->refill_shadow = rs;
+  auto measure = [&](const Setting &t) {
+    ctx->max_paths = t.paths;
+    ctx->grid = t.grid;
+    ctx->tile = t.tile;
+    ctx->refill_extend = t.re;
+    ctx->refill_shadow = t.rs;
+    ctx->fetch_batch = t.fb;
     hk_render_stats st;
     int rc = hk_render_batch(s, &cal, 1, films, 0, &st);
     if (rc != HK_OK) throw HkError(rc, "calibration render: %s", ctx->err.c_str());
     double rate = st.gpu_ms > 0 ? double(st.paths) / (st.gpu_ms * 1e-3) : 0.0;
     if (out->n_tried < HK_TUNE_MAX) {
       uint32_t k = out->n_tried++;
-      out->tried_max_paths[k] = paths;
-      out->tried_grid[k] = grid;
-      out->tried_tile[k] = tile;
-      out->tried_refill[k] = re;
-      out->tried_refill_shadow[k] = rs;
+      out->tried_max_paths[k] = t.paths;
+      out->tried_grid[k] = t.grid;
+      out->tried_tile[k] = t.tile;
+      out->tried_refill[k] = t.re;
+      out->tried_refill_shadow[k] = t.rs;
+      out->tried_fetch_batch[k] = t.fb;
       out->samples_per_s[k] = rate;
     }
     return rate;
   };
   try {
-    measure(best_paths, best_grid, best_tile, best_re, best_rs);  // warm-up: workspace, code and caches
+    measure(best);  // warm-up: workspace, code and caches
     out->n_tried = 0;
-    double best = 0.0;
+    double best_rate = 0.0;
+    auto consider = [&](const Setting &t) {
+      double r = measure(t);
+      if (r > best_rate) {
+        best_rate = r;
+        best = t;
+      }
+    };
     uint64_t total = npix * cal.spp;
     for (uint32_t p : {1u << 22, 1u << 23, 1u << 24, 1u << 25}) {
       if (p > total && p != (1u << 22)) break;  // a wavefront larger than the work is never filled
-      double r = measure(p, best_grid, best_tile, best_re, best_rs);
-      if (r > best) { best = r; best_paths = p; }
+      Setting t = best;
+      t.paths = p;
+      consider(t);
     }
-    for (uint32_t re : {32u, 48u, 56u, 64u}) {
-      if (re == best_re) continue;
-      double r = measure(best_paths, best_grid, best_tile, re, best_rs);
-      if (r > best) { best = r; best_re = re; }
+    for (uint32_t fb : {1u, 2u, 4u}) {
+      if (fb == best.fb) continue;
+      Setting t = best;
+      t.fb = fb;
+      consider(t);
     }
-    for (uint32_t rs : {32u, 48u, 56u, 64u}) {
-      if (rs == best_rs) continue;
-      double r = measure(best_paths, best_grid, best_tile, best_re, rs);
-      if (r > best) { best = r; best_rs = rs; }
+    for (uint32_t re : {16u, 24u, 32u, 48u}) {
+      if (re == best.re) continue;
+      Setting t = best;
+      t.re = re;
+      consider(t);
     }
-    for (uint32_t t : {1u, 4u, 8u, 16u}) {
-      if (t == best_tile) continue;
-      double r = measure(best_paths, best_grid, t, best_re, best_rs);
-      if (r > best) { best = r; best_tile = t; }
+    for (uint32_t rs : {16u, 24u, 32u, 48u}) {
+      if (rs == best.rs) continue;
+      Setting t = best;
+      t.rs = rs;
+      consider(t);
+    }
+    for (uint32_t tile : {1u, 4u, 8u, 16u}) {
+      if (tile == best.tile) continue;
+      Setting t = best;
+      t.tile = tile;
+      consider(t);
     }
     for (uint32_t g : {0u, uint32_t(ctx->n_cu) * 8u, uint32_t(ctx->n_cu) * 32u}) {
-      if (g == best_grid) continue;
-      double r = measure(best_paths, g, best_tile, best_re, best_rs);
-      if (r > best) { best = r; best_grid = g; }
+      if (g == best.grid) continue;
+      Setting t = best;
+      t.grid = g;
+      consider(t);
     }
   } catch (...) {
     ctx->profile = saved_profile;
     throw;
   }
   ctx->profile = saved_profile;
-  ctx->max_paths = best_paths;
-  ctx->grid = best_grid;
-  ctx->tile = best_tile;
-  ctx->refill_extend = best_re;
-  ctx->refill_shadow = best_rs;
+  ctx->max_paths = best.paths;
+  ctx->grid = best.grid;
+  ctx->tile = best.tile;
+  ctx->refill_extend = best.re;
+  ctx->refill_shadow = best.rs;
+  ctx->fetch_batch = best.fb;
   ctx->ws.release();  // the next render sizes the workspace for the chosen wavefront
-  out->max_paths = best_paths;
-  out->grid = best_grid;
-  out->tile = best_tile;
-  out->refill = best_re;
-  out->refill_shadow = best_rs;
+  out->max_paths = best.paths;
+  out->grid = best.grid;
+  out->tile = best.tile;
+  out->refill = best.re;
+  out->refill_shadow = best.rs;
+  out->fetch_batch = best.fb;
   out->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return HK_OK;
   HK_API_END(ctx)

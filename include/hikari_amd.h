@@ -183,19 +183,19 @@ int hk_render(hk_scene *scene, const hk_request *req, float *out_rgb, hk_render_
 
 /* ------------------------------------------------------------ autotuning */
 /* No counterpart in the reference.  Picks the context's wavefront size
- * ("max_paths"), refill thresholds of the persistent traversal kernels
- * ("refill", "refill_shadow"), camera-ray tile edge ("tile") and queue-kernel
- * grid cap ("grid") for requests like `req` on `scene`: each candidate renders a
+ * ("max_paths"), refill thresholds and fetch batch of the persistent
+ * traversal kernels ("refill", "refill_shadow", "fetch_batch"), camera-ray
+ * tile edge ("tile") and queue-kernel grid cap ("grid") for requests like `req` on `scene`: each candidate renders a
  * calibration slice of req (about calib_samples path samples, 0: 64 M; at
  * least one sample per pixel), timed with HIP events; the fastest setting is
  * stored in the context options and reported.  No film depends on these
  * options (the same bits for any setting), so tuning changes only speed. */
 #define HK_TUNE_MAX 32
 typedef struct hk_tune_result {
-  uint32_t max_paths, grid, tile, refill, refill_shadow;   /* the chosen setting */
+  uint32_t max_paths, grid, tile, refill, refill_shadow, fetch_batch;   /* the chosen setting */
   uint32_t n_tried;
   uint32_t tried_max_paths[HK_TUNE_MAX], tried_grid[HK_TUNE_MAX], tried_tile[HK_TUNE_MAX];
-  uint32_t tried_refill[HK_TUNE_MAX], tried_refill_shadow[HK_TUNE_MAX];
+  uint32_t tried_refill[HK_TUNE_MAX], tried_refill_shadow[HK_TUNE_MAX], tried_fetch_batch[HK_TUNE_MAX];
   double samples_per_s[HK_TUNE_MAX];
   double total_ms;                  /* wall time of the search */
 } hk_tune_result;
@@ -255,9 +255,12 @@ int hk_server_destroy(hk_server *server);
  *                 floor(max_samples / pixels) samples per pixel, resolved
  *                 into the film when their segment ends
  *   "grid"        cap on blocks per queue-kernel launch (0: one lane per item)
- *   "refill"      idle lanes (1..64, default 56) that make a wave of the
+ *   "refill"      idle lanes (1..64, default 24) that make a wave of the
  *                 persistent closest-hit kernel fetch new rays
  *   "refill_shadow" the same for the shadow-ray kernel
+ *   "fetch_batch" queue entries (in 64s, 1..64, default 2) a wave of the
+ *                 persistent kernels reserves per atomic on the shared fetch
+ *                 counter
  *   "graph"       1 (default): after a few eager bounces a render replays
  *                 each bounce as a captured HIP graph; 0: eager launches
  *                 (profiled renders always launch eagerly)

@@ -358,19 +358,20 @@ def test_image_vs_high_spp_fp64_oracle(ctx, which):
 
 
 def test_tuning_options_do_not_change_bits():
-    """Camera-ray tile edge, queue-kernel grid cap, wavefront size and refill
-    thresholds (the autotuner's knobs) leave every film bit unchanged (ragged
-    70 x 45 film)."""
+    """Camera-ray tile edge, queue-kernel grid cap, wavefront size, refill
+    thresholds and fetch batch (the autotuner's knobs) leave every film bit
+    unchanged (ragged 70 x 45 film)."""
     sc = scenes.cbox(width=70, height=45, sphere_subdiv=2)
     imgs = []
-    for tile, grid, max_paths, refill in [(8, 0, 0, 0), (1, 0, 0, 1), (4, 64, 0, 64), (16, 0, 5000, 17),
-                                          (32, 7, 999, 40)]:
+    for tile, grid, max_paths, refill, batch in [(8, 0, 0, 0, 0), (1, 0, 0, 1, 3), (4, 64, 0, 64, 64),
+                                                 (16, 0, 5000, 17, 2), (32, 7, 999, 40, 8)]:
         c = hk.Context(0)
         c.set_option("tile", tile)
         c.set_option("grid", grid)
         c.set_option("max_paths", max_paths)
         c.set_option("refill", refill)
         c.set_option("refill_shadow", 65 - refill if refill else 0)
+        c.set_option("fetch_batch", batch)
         img, _ = c.scene(sc).render(5, seed=12)
         imgs.append(img)
         c.close()
@@ -390,7 +391,7 @@ def test_autotune_keeps_results():
         t = gs.autotune(16, calib_samples=96 * 64 * 16)
         print("autotune:", t)
         assert len(t["tried"]) >= 5
-        keys = ("max_paths", "grid", "tile", "refill", "refill_shadow")
+        keys = ("max_paths", "grid", "tile", "refill", "refill_shadow", "fetch_batch")
         assert tuple(t[k] for k in keys) in {tuple(x[k] for k in keys) for x in t["tried"]}
         assert all(x["samples_per_s"] > 0 for x in t["tried"])
         after, _ = gs.render(16, seed=3)
