@@ -20,6 +20,7 @@
 #include <cmath>
 
 #include "hk_bsdf.h"
+#include "hk_cdf.h"
 #include "hk_kernels.h"
 
 namespace hk {
@@ -498,20 +499,11 @@ struct ERec {
   float pdf, dist;
 };
 
-// DiscretePDF1D::SampleContinuous / SampleDiscrete (src/core/DiscretePDF.cpp:32-78)
-__device__ __forceinline__ int cdf_find(const float *cdf, int n, float s) {
-  int lo = 0, cnt = n + 1;  // lower_bound over n + 1 entries
-  while (cnt > 0) {
-    int step = cnt >> 1;
-    if (cdf[lo + step] < s) { lo += step + 1; cnt -= step + 1; }
-    else cnt = step;
-  }
-  int i = lo - 1;
-  return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
-}
-__device__ __forceinline__ float cdf_sample_cont(const float *cdf, const float *func, int n, float integral, float s,
-                                                 float &pdf, int &idx) {
-  int i = cdf_find(cdf, n, s);
+// DiscretePDF1D::SampleContinuous / SampleDiscrete (src/core/DiscretePDF.cpp:32-78);
+// the index search is cdf_find_guided (hk_cdf.h).
+__device__ __forceinline__ float cdf_sample_cont(const float *cdf, const uint32_t *guide, const float *func, int n,
+                                                 float integral, float s, float &pdf, int &idx) {
+  int i = cdf_find_guided(cdf, guide, n, s);
   idx = i;
   float du = s - cdf[i];
   if (cdf[i + 1] - cdf[i] > 0.0f) du /= (cdf[i + 1] - cdf[i]);
@@ -523,7 +515,7 @@ __device__ __forceinline__ float cdf_sample_cont(const float *cdf, const float *
 // src/core/Mesh.cpp:273-303)
 __device__ __forceinline__ vec3 area_sample(const DevScene &sc, const DevMesh &m, vec3 ref, float sx, float sy,
                                             float s1, ERec &er) {
-  int tri = cdf_find(sc.cdf + m.cdf_off, (int)m.nt, s1);
+  int tri = cdf_find_guided(sc.cdf + m.cdf_off, sc.cdf_guide + m.cdf_off, (int)m.nt, s1);
   float sq = sqrtf(1.0f - sx);
   float al = 1.0f - sq, be = sy * sq, ga = 1.0f - al - be;
   const ShadeTri &st = sc.shade[m.first_prim + tri];
@@ -551,9 +543,10 @@ constexpr float kPi2F = 9.8696044010893586f;  // float(pi * pi)
 __device__ __forceinline__ vec3 env_sample(const DevScene &sc, vec3 ref, float dist, float sx, float sy, ERec &er) {
   float py, px;
   int iy, ix;
-  float v = cdf_sample_cont(sc.env_marg_cdf, sc.env_marg_func, sc.env_h, sc.env_marg_int, sy, py, iy);
-  float u = cdf_sample_cont(sc.env_row_cdf + size_t(iy) * (sc.env_w + 1), sc.env_row_func + size_t(iy) * sc.env_w,
-                            sc.env_w, sc.env_row_int[iy], sx, px, ix);
+  float v = cdf_sample_cont(sc.env_marg_cdf, sc.env_marg_guide, sc.env_marg_func, sc.env_h, sc.env_marg_int, sy, py,
+                            iy);
+  float u = cdf_sample_cont(sc.env_row_cdf + size_t(iy) * (sc.env_w + 1), sc.env_row_guide + size_t(iy) * (sc.env_w + 1),
+                            sc.env_row_func + size_t(iy) * sc.env_w, sc.env_w, sc.env_row_int[iy], sx, px, ix);
   float map_pdf = px * py;
   float theta = v * kPiF;
   float st = sinf(theta);
@@ -790,13 +783,14 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
 // ------------------------------------------------------------------ k_shade
 constexpr uint32_t kSortKeys = 32;  // material keys of the block sort (last two: miss, idle lane)
 // One bounce of PathMISIntegrator::Li for every queued path.
-// HK_SHADE_MIN_WAVES (tuning builds): ask the register allocator for that
-// many waves per SIMD (it spills to scratch to get there).
-#ifdef HK_SHADE_MIN_WAVES
-__global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs A, int qi) {
-#else
-__global__ __launch_bounds__(kBlock) void k_shade(RenderArgs A, int qi) {
+// The register allocator is asked for 4 waves per SIMD (128 VGPRs, a few
+// bytes of spill per lane): with 3 waves (158 VGPRs, no spill) the kernel
+// waits on memory more than it gains from the spare registers.
+// HK_SHADE_MIN_WAVES overrides it in tuning builds.
+#ifndef HK_SHADE_MIN_WAVES
+#define HK_SHADE_MIN_WAVES 4
 #endif
+__global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs A, int qi) {
   const DevScene &sc = A.sc;
   uint32_t count = A.counters[ctr(qi)];
   if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[ctr(4)] = 0u;  // k_extend's fetch counter

@@ -292,18 +292,6 @@ void build_materials(const float *recs, int n, const char *rft_dir, std::vector<
   if (rft_pool.empty()) rft_pool.push_back(0.0f);
 }
 
-// DiscretePDF1D construction (src/core/DiscretePDF.cpp:5-26), in FP32.
-void build_cdf(const float *func, int n, float *cdf, float &integral) {
-  cdf[0] = 0.0f;
-  for (int i = 1; i < n + 1; i++) cdf[i] = cdf[i - 1] + func[i - 1] / n;
-  integral = cdf[n];
-  if (integral == 0.0f) {
-    for (int i = 1; i < n + 1; i++) cdf[i] = float(i) / float(n);
-  } else {
-    for (int i = 1; i < n + 1; i++) cdf[i] /= integral;
-  }
-}
-
 // PerspectiveCamera::Activate (src/camera/PerspectiveCamera.cpp:32-75):
 // sample space [0,1]^2 -> camera space on the near plane.
 void camera_setup(const float *c2w, float fov, int w, int h, float near_clip, float far_clip, DevRequest &r) {

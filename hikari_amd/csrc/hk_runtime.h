@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/hikari_amd.h"
+#include "hk_cdf.h"
 #include "hk_shared.h"
 
 namespace hk {
@@ -109,8 +110,9 @@ struct hk_scene {
   hk_ctx *ctx = nullptr;
   hk::DevScene dev;
   hk::DevBvh bvh;
-  hk::DBuf shade, meshes, mats, rft, emitters, cdf;
-  hk::DBuf env_tex, env_row_cdf, env_row_func, env_row_int, env_marg_cdf, env_marg_func;
+  hk::DBuf shade, meshes, mats, rft, emitters, cdf, cdf_guide;
+  hk::DBuf env_tex, env_row_cdf, env_row_func, env_row_int, env_marg_cdf, env_marg_func, env_row_guide,
+      env_marg_guide;
   hk_scene_desc desc;             // camera / film defaults (pointers not retained)
   uint32_t n_tris = 0;
   uint64_t bytes = 0;
@@ -168,7 +170,7 @@ struct Mat4 {
   }
 };
 
-void build_cdf(const float *func, int n, float *cdf, float &integral);
+
 void camera_setup(const float *c2w, float fov, int w, int h, float near_clip, float far_clip, DevRequest &r);
 void filter_setup(const hk_scene_desc &d, DevRequest &r);
 void upload_bvh(hk_ctx *ctx, const float *tri9, uint32_t nt, DevBvh &out);

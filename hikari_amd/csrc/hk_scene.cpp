@@ -179,6 +179,7 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
     int w = d->env_w, h = d->env_h;
     if (w <= 0 || h <= 0) throw HkError(HK_ERR_INVALID, "bad environment map size");
     std::vector<float> row_func(size_t(w) * h), row_cdf(size_t(w + 1) * h), row_int(h), marg_cdf(h + 1);
+    std::vector<uint32_t> row_guide(size_t(w + 1) * h), marg_guide(h + 1);
     for (int y = 0; y < h; y++) {
       float theta = float(y + 0.5f) / float(h) * float(M_PI);
       float sth = std::sin(theta);
@@ -187,21 +188,27 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
         row_func[size_t(y) * w + x] = (c[0] * 0.212671f + c[1] * 0.715160f + c[2] * 0.072169f) * sth;
       }
       build_cdf(&row_func[size_t(y) * w], w, &row_cdf[size_t(y) * (w + 1)], row_int[y]);
+      build_cdf_guide(&row_cdf[size_t(y) * (w + 1)], w, &row_guide[size_t(y) * (w + 1)]);
     }
     float marg_int;
     build_cdf(row_int.data(), h, marg_cdf.data(), marg_int);
+    build_cdf_guide(marg_cdf.data(), h, marg_guide.data());
     s->env_tex.upload(d->env, size_t(w) * h * 12, st);
     s->env_row_cdf.upload(row_cdf.data(), row_cdf.size() * 4, st);
     s->env_row_func.upload(row_func.data(), row_func.size() * 4, st);
     s->env_row_int.upload(row_int.data(), row_int.size() * 4, st);
     s->env_marg_cdf.upload(marg_cdf.data(), marg_cdf.size() * 4, st);
     s->env_marg_func.upload(row_int.data(), row_int.size() * 4, st);
+    s->env_row_guide.upload(row_guide.data(), row_guide.size() * 4, st);
+    s->env_marg_guide.upload(marg_guide.data(), marg_guide.size() * 4, st);
     ds.env_tex = s->env_tex.as<float>();
     ds.env_row_cdf = s->env_row_cdf.as<float>();
     ds.env_row_func = s->env_row_func.as<float>();
     ds.env_row_int = s->env_row_int.as<float>();
     ds.env_marg_cdf = s->env_marg_cdf.as<float>();
     ds.env_marg_func = s->env_marg_func.as<float>();
+    ds.env_row_guide = s->env_row_guide.as<uint32_t>();
+    ds.env_marg_guide = s->env_marg_guide.as<uint32_t>();
     ds.env_marg_int = marg_int;
     ds.env_w = w;
     ds.env_h = h;
@@ -239,6 +246,12 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
   s->rft.upload(pool.data(), pool.size() * 4, st);
   if (!emitters.empty()) s->emitters.upload(emitters.data(), emitters.size() * sizeof(DevEmitter), st);
   s->cdf.upload(cdf.data(), cdf.size() * 4, st);
+  {
+    std::vector<uint32_t> guide(cdf.size());
+    for (const DevMesh &dm : meshes) build_cdf_guide(cdf.data() + dm.cdf_off, int(dm.nt), guide.data() + dm.cdf_off);
+    s->cdf_guide.upload(guide.data(), guide.size() * 4, st);
+    hip_check(hipStreamSynchronize(st), "CDF guide upload");  // the host vector is freed at the end of the block
+  }
 
   ds.nodes = s->bvh.nodes.as<TraceNode>();
   ds.tris = s->bvh.tris.as<TraceTri>();
@@ -248,6 +261,7 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
   ds.rft = s->rft.as<float>();
   ds.emitters = s->emitters.as<DevEmitter>();
   ds.cdf = s->cdf.as<float>();
+  ds.cdf_guide = s->cdf_guide.as<uint32_t>();
   ds.n_emitters = int32_t(emitters.size());
   ds.n_meshes = d->n_meshes;
   ds.bbox_radius = bbox_radius(lo, hi);
@@ -263,8 +277,8 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
   ds.root_ref = 0;
   hip_check(hipStreamSynchronize(st), "scene upload");
   DBuf *all[] = {&s->bvh.nodes, &s->bvh.tris, &s->shade, &s->meshes, &s->mats,
-                 &s->rft, &s->emitters, &s->cdf, &s->env_tex, &s->env_row_cdf, &s->env_row_func, &s->env_row_int,
-                 &s->env_marg_cdf, &s->env_marg_func};
+                 &s->rft, &s->emitters, &s->cdf, &s->cdf_guide, &s->env_tex, &s->env_row_cdf, &s->env_row_func,
+                 &s->env_row_int, &s->env_marg_cdf, &s->env_marg_func, &s->env_row_guide, &s->env_marg_guide};
   for (DBuf *b : all) s->bytes += b->bytes;
   *out = s.release();
   return HK_OK;

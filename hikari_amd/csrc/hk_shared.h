@@ -117,12 +117,15 @@ struct DevScene {
   const float *rft;        // reduced table pool
   const DevEmitter *emitters;
   const float *cdf;        // area-light CDF pool (per mesh nt + 1 entries)
+  const uint32_t *cdf_guide;  // guide tables of the CDFs (same offsets)
   const float *env_tex;    // h x w x 3
   const float *env_row_cdf;   // h x (w + 1)
   const float *env_row_func;  // h x w
   const float *env_row_int;   // h
   const float *env_marg_cdf;  // h + 1
   const float *env_marg_func; // h
+  const uint32_t *env_row_guide;   // h x (w + 1), guide tables of the row CDFs
+  const uint32_t *env_marg_guide;  // h + 1
   float env_marg_int;
   int32_t env_w, env_h, has_env;
   float env_scale;
