@@ -112,7 +112,9 @@ __device__ __forceinline__ void trav_begin(RayTrav &R, uint64_t *lds_stack, uint
 }
 
 // Internal node R.ref: pushes the hit children far to near, continues with
-// the nearest one (or the next stack entry that can still hold a hit).
+// the nearest one (or the next stack entry that can still hold a hit).  Any-
+// hit rays keep the near-to-far order too: taking the children in node order
+// saves the sort but finds occluders later (2-3% slower overall).
 __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, RayTrav &R, TraceCount &cnt) {
 #ifdef HK_TRAVERSAL_STATS
   cnt.nodes++;
@@ -403,45 +405,36 @@ __device__ __forceinline__ void fetch_init(WaveFetch &f) {
   f.dry = false;
 }
 __device__ __forceinline__ bool fetch_exhausted(const WaveFetch &f) { return f.dry && f.next == f.end; }
-// Gives every idle lane (bit set in `idle`, the same mask on all lanes) the
-// next entry, or 0xffffffff when there is none.  Every lane calls it.
+// Gives idle lanes (bits set in `idle`, the same mask on all lanes) the next
+// entries of the wave's range, in lane order, reserving a new range first
+// when the current one is used up; lanes beyond the range's end stay idle
+// until the next call.  Returns the lane's entry or 0xffffffff.  Every lane
+// calls it.
 __device__ __forceinline__ uint32_t wave_fetch(WaveFetch &f, uint32_t *counter, uint32_t count, uint32_t batch,
                                                unsigned long long idle) {
   uint32_t lane = lane_id();
   uint32_t n_idle = (uint32_t)__popcll(idle);
-  uint32_t avail = f.end - f.next;
-  uint32_t nlo = 0, nhi = 0;
-  bool fetched = avail < n_idle && !f.dry;
-  if (fetched) {
-    uint32_t want = n_idle - avail;
+  if (f.next == f.end && !f.dry) {
+    uint32_t want = n_idle;
     uint32_t left = count - min(f.seen, count);
     if (batch > 1u && left > gridDim.x * kWaves * batch * 64u) want = max(want, batch * 64u);
     uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
     uint32_t b = 0;
     if (lane == leader) b = atomicAdd(counter, want);
-    b = __shfl(b, (int)leader);
+    // readlane (not a shuffle) keeps the reservation in scalar registers
+    b = (uint32_t)__builtin_amdgcn_readlane((int)b, (int)leader);
     f.seen = b;
-    nlo = min(b, count);
-    nhi = min(b + want, count);
+    f.next = min(b, count);
+    f.end = min(b + want, count);
     if (b + want >= count) f.dry = true;
   }
-  uint32_t take_old = min(avail, n_idle);
+  uint32_t take = min(f.end - f.next, n_idle);
   uint32_t idx = 0xffffffffu;
   if ((idle >> lane) & 1ull) {
     uint32_t r = (uint32_t)__popcll(idle & (lane == 0 ? 0ull : ((1ull << lane) - 1ull)));
-    if (r < take_old) {
-      idx = f.next + r;
-    } else {
-      uint32_t j = nlo + (r - take_old);
-      if (j < nhi) idx = j;
-    }
+    if (r < take) idx = f.next + r;
   }
-  if (fetched) {
-    f.next = min(nlo + (n_idle - take_old), nhi);
-    f.end = nhi;
-  } else {
-    f.next += take_old;
-  }
+  f.next += take;
   return idx;
 }
 
