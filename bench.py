@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--requests", type=int, default=0, help="batch/serve: number of requests (default 64 / 40)")
     ap.add_argument("--qps", type=float, default=1.0, help="serve: Poisson arrival rate")
     ap.add_argument("--budget-gb", type=float, default=96.0, help="serve: HBM budget of the server")
+    ap.add_argument("--schedule", type=int, default=1, help="serve / batch: 1 fewest samples left first "
+                    "(library default), 0 first come first served")
     return ap.parse_args()
 
 
@@ -94,6 +96,7 @@ def bench_batch(a):
     from hikari_amd import scenes, workloads
     scene = scenes.displaced_sphere(alpha=a.alpha)
     ctx = hk.Context(0)
+    ctx.set_option("schedule", a.schedule)
     gs = ctx.scene(scene)
     reqs = workloads.batch_requests(n=a.requests or 64)
     total = sum(r.width * r.height * r.spp for r in reqs)
@@ -118,7 +121,7 @@ def bench_batch(a):
         "dtype": "f32", "data": "synthetic (seeded procedural scene and request mix)",
         "config": {"workload": f"{len(reqs)} batched requests over the displaced icosphere-8 scene, res "
                                "{64..1024}^2, spp {16..1024}, camera on a radius-4 sphere",
-                   "requests": len(reqs), "path_samples": total},
+                   "requests": len(reqs), "path_samples": total, "schedule": ["fifo", "shortest_first"][a.schedule]},
         "plain_value": total / t_plain, "batch_over_plain": t_plain / t_batch,
         "identical_images": same,
         "latency_ms": {"batched_p50": t_batch * 1e3, "batched_p99": t_batch * 1e3,
@@ -135,6 +138,7 @@ def bench_serve(a):
     from hikari_amd import scenes, workloads
     box, sphere, field = scenes.cbox(), scenes.displaced_sphere(alpha=a.alpha), scenes.heightfield()
     ctx = hk.Context(0)
+    ctx.set_option("schedule", a.schedule)
     gss = [ctx.scene(box), ctx.scene(sphere), ctx.scene(field)]
     trace = workloads.serve_trace([box, None, field], n=a.requests or 40, qps=a.qps)
     total = sum(r.width * r.height * r.spp for _, _, r in trace)
@@ -188,7 +192,8 @@ def bench_serve(a):
         "dtype": "f32", "data": "synthetic (seeded procedural scenes, Poisson request trace)",
         "config": {"workload": f"serving {len(trace)} requests at {a.qps} req/s over 3 resident scenes "
                                "(box 5k tris, displaced sphere 1.3M, heightfield 2.1M); res {64..1024}^2, "
-                               "spp {16..1024}", "hbm_budget_gb": a.budget_gb, "path_samples": total},
+                               "spp {16..1024}", "hbm_budget_gb": a.budget_gb, "path_samples": total,
+                   "schedule": ["fifo", "shortest_first"][a.schedule]},
         "latency_ms": {"p50": _pct(lat, 50) * 1e3, "p99": _pct(lat, 99) * 1e3},
         "plain_fcfs": {"value": total / span_plain, "latency_p50_ms": _pct(lat_plain, 50) * 1e3,
                        "latency_p99_ms": _pct(lat_plain, 99) * 1e3, "busy_s": sum(service)},

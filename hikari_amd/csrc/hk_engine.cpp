@@ -95,6 +95,7 @@ Engine::Engine(hk_scene *s, Workspace &ws, hipStream_t st, uint32_t pool_paths, 
     : s_(s), ws_(ws), st_(st), pool_(std::max(1u, pool_paths)), cap_samples_(std::max(1u, buffer_samples)),
       chunk_cap_(std::max(1u, chunk_cap)), profile_(profile), ev0_(ev0), ev1_(ev1) {
   ws_.ensure(pool_, uint32_t(s->dev.n_emitters), cap_samples_);
+  shortest_first_ = s->ctx->schedule == 1;
   slots_.assign(kMaxItems, ChunkRec());
   reqs_by_slot_.assign(kMaxRequests, nullptr);
   for (uint32_t i = kMaxItems; i-- > 0;) free_items_.push_back(i);
@@ -235,16 +236,39 @@ void Engine::free_range(uint32_t base, uint32_t n) {
   }
 }
 
-// Admits chunks in submission order while an item slot and a sample-buffer
-// range are free; stops at the first chunk that does not fit (FIFO).
+// Admits chunks while an item slot and a sample-buffer range are free.  FIFO:
+// in submission order, stopping at the first chunk that does not fit.
+// Shortest first: requests with the fewest samples left go first, and a
+// request whose next chunk does not fit is passed over for smaller ones.
 void Engine::admit() {
+  if (shortest_first_ && pending_.size() > 1) {
+    std::vector<RequestRec *> order(pending_.begin(), pending_.end());
+    std::stable_sort(order.begin(), order.end(),
+                     [](const RequestRec *a, const RequestRec *b) { return a->left() < b->left(); });
+    for (RequestRec *r : order) {
+      if (free_items_.empty()) break;
+      admit_chunks(r);
+    }
+    pending_.erase(std::remove_if(pending_.begin(), pending_.end(),
+                                  [](const RequestRec *r) { return r->next_admit == r->n_chunks; }),
+                   pending_.end());
+    return;
+  }
   while (!pending_.empty()) {
     RequestRec *r = pending_.front();
+    if (!admit_chunks(r)) return;
+    pending_.pop_front();
+  }
+}
+
+// Admits r's next chunks while they fit; true when all are admitted.
+bool Engine::admit_chunks(RequestRec *r) {
+  {
     while (r->next_admit < r->n_chunks) {
       uint32_t k = std::min(r->chunk, r->spp - r->next_admit * r->chunk);
       uint32_t size = r->npix * k;
       uint32_t base = 0;
-      if (free_items_.empty() || !alloc_range(size, base)) return;
+      if (free_items_.empty() || !alloc_range(size, base)) return false;
       uint32_t slot = free_items_.back();
       free_items_.pop_back();
       ChunkRec &c = slots_[slot];
@@ -268,8 +292,8 @@ void Engine::admit() {
       r->next_admit++;
       active_.push_back(slot);
     }
-    pending_.pop_front();
   }
+  return true;
 }
 
 std::vector<RequestRec *> Engine::bounce() {
@@ -280,15 +304,30 @@ std::vector<RequestRec *> Engine::bounce() {
 bool Engine::launch() {
   admit();
   // plan: new samples for the free places of the queue, oldest chunks first
+  // (shortest first: chunks of the requests with the fewest samples left)
   uint32_t room = pool_ - in_flight_, n_new = 0, n_plan = 0;
   RegenEntry *plan = ws_.h_plan.as<RegenEntry>();
-  for (uint32_t slot : active_) {
+  const std::vector<uint32_t> *order = &active_;
+  std::vector<uint32_t> by_left;
+  if (shortest_first_ && active_.size() > 1) {
+    std::vector<std::pair<uint64_t, uint32_t>> keyed;
+    keyed.reserve(active_.size());
+    for (uint32_t slot : active_) keyed.push_back({slots_[slot].req->left(), slot});
+    std::stable_sort(keyed.begin(), keyed.end(),
+                     [](const std::pair<uint64_t, uint32_t> &a, const std::pair<uint64_t, uint32_t> &b) {
+                       return a.first < b.first;
+                     });
+    for (auto &k : keyed) by_left.push_back(k.second);
+    order = &by_left;
+  }
+  for (uint32_t slot : *order) {
     if (room == 0) break;
     ChunkRec &c = slots_[slot];
     if (c.started == c.size) continue;
     uint32_t take = std::min(room, c.size - c.started);
     plan[n_plan++] = RegenEntry{slot, c.started, take, n_new};
     c.started += take;
+    c.req->started += take;
     n_new += take;
     room -= take;
   }

@@ -37,6 +37,8 @@ struct RequestRec {
   bool keep_samples = false;       // pixel-sample mode: keep per-sample radiance
   std::vector<float> samples;      // keep_samples: n x 4 radiance (rgb, unused)
   uint32_t next_admit = 0, next_resolve = 0;
+  uint64_t started = 0;            // samples handed to k_regen so far
+  uint64_t left() const { return uint64_t(npix) * spp - started; }
   std::vector<int32_t> chunk_item;  // item slot per admitted, unresolved chunk (-1 otherwise)
   double t_submit = 0, t_admit = -1, t_done = -1;  // ms on the engine clock
   bool done = false;
@@ -85,6 +87,7 @@ class Engine {
 
  private:
   void admit();
+  bool admit_chunks(RequestRec *r);
   bool alloc_range(uint32_t n, uint32_t &base);
   void free_range(uint32_t base, uint32_t n);
   void resolve_ready(std::vector<RequestRec *> &finished);
@@ -100,7 +103,8 @@ class Engine {
   uint32_t pool_, cap_samples_, chunk_cap_;
   bool profile_;
   hipEvent_t ev0_, ev1_;
-  std::deque<RequestRec *> pending_;            // submitted, chunks left to admit (FIFO)
+  std::deque<RequestRec *> pending_;            // submitted, chunks left to admit (submission order)
+  bool shortest_first_ = false;                 // schedule: fewest samples left first (else FIFO)
   std::vector<ChunkRec> slots_;                 // by item slot
   std::vector<uint32_t> free_items_, free_reqs_;
   std::vector<uint32_t> active_;                // admitted item slots, admission order

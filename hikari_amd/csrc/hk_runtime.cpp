@@ -450,6 +450,9 @@ int hk_set_option(hk_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "fetch_batch") {
     if (value < 0 || value > 64) throw HkError(HK_ERR_INVALID, "fetch_batch: 1..64 (0: default)");
     ctx->fetch_batch = value == 0 ? 2u : uint32_t(value);
+  } else if (n == "schedule") {
+    if (value < 0 || value > 1) throw HkError(HK_ERR_INVALID, "schedule: 0 (FIFO) or 1 (shortest first)");
+    ctx->schedule = int(value);
   } else if (n == "graph") {
     ctx->graph = value != 0;
   } else if (n == "tile") {

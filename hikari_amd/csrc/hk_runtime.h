@@ -94,6 +94,7 @@ struct hk_ctx {
   uint32_t refill_extend = 24;  // idle lanes that make a persistent k_extend wave fetch rays
   uint32_t refill_shadow = 24;  // the same for k_shadow
   uint32_t fetch_batch = 2;     // queue entries a persistent wave reserves per atomic, in 64s
+  int schedule = 1;             // chunk admission / regeneration order: 0 FIFO, 1 shortest first
   bool profile = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   uint32_t *host_counters = nullptr;  // pinned

@@ -261,6 +261,11 @@ int hk_server_destroy(hk_server *server);
  *   "fetch_batch" queue entries (in 64s, 1..64, default 2) a wave of the
  *                 persistent kernels reserves per atomic on the shared fetch
  *                 counter
+ *   "schedule"    order in which chunks are admitted and their samples
+ *                 started: 1 (default) requests with the fewest samples
+ *                 left first (a chunk that does not fit is passed over for
+ *                 smaller ones); 0 first come first served.  Films do not
+ *                 depend on it
  *   "graph"       1 (default): after a few eager bounces a render replays
  *                 each bounce as a captured HIP graph; 0: eager launches
  *                 (profiled renders always launch eagerly)

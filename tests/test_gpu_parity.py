@@ -212,12 +212,14 @@ def test_batch_equals_alone(ctx):
             hk.make_request(cam_to_world=D.lookat((-0.5, 0.5, 0.9), (0, -0.2, -1), (0, 1, 0)), fov=50.0, width=13,
                             height=11, spp=17, seed=4)]
     alone = [gs.render(0, request=r)[0] for r in reqs]
-    for max_paths in (0, 500):
+    for max_paths, schedule in ((0, 0), (500, 0), (500, 1)):
+        ctx.set_option("schedule", schedule)
         batch, st = gs.render_batch(reqs, max_paths=max_paths)
         assert st["paths"] == sum(r.spp * (r.width * r.height) for r in reqs)
         for a, b in zip(alone, batch):
             assert a.shape == b.shape
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    ctx.set_option("schedule", 1)
 
 
 def _serve_requests(box, field):
@@ -231,16 +233,19 @@ def _serve_requests(box, field):
             (1, hk.make_request(field, spp=0, seed=16))]
 
 
-def test_server_results_equal_alone():
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_server_results_equal_alone(schedule):
     """Continuous batching over two resident scenes: requests submitted while
     others run (and split into chunks by a small sample capacity) return the
-    same films as hk_render of each request alone, bit for bit."""
+    same films as hk_render of each request alone, bit for bit, in first-come
+    and in shortest-first order."""
     import time
     box = scenes.cbox(width=32, height=24, sphere_subdiv=2)
     field = scenes.heightfield(width=28, height=20, n=24)
     c = hk.Context(0)
     c.set_option("max_paths", 2500)
     c.set_option("max_samples", 32 * 24 * 7)
+    c.set_option("schedule", schedule)
     g_box, g_field = c.scene(box), c.scene(field)
     reqs = _serve_requests(box, field)
     alone = [(g_box if sid == 0 else g_field).render(0, request=r)[0] for sid, r in reqs]
