@@ -1170,19 +1170,31 @@ __global__ __launch_bounds__(kBlock) void k_resolve_plain(RenderArgs A, uint32_t
 }
 
 // The same accumulation for whole images, a 16x16 pixel tile per block: for
-// each sample index the tile and its filter halo (jitter recomputed once per
-// halo pixel, radiance read once) are staged in LDS, then every pixel sums its
-// neighbours in the plain kernel's order, so the sums are bit-identical.
-constexpr int kResTile = 16, kResHalo = 4, kResEdge = kResTile + 2 * kResHalo;
+// each sample index the tile and its filter halo are staged in LDS once per
+// halo pixel -- radiance, and the sample's filter weights toward each column
+// and row its footprint can reach (jitter recomputed once, table lookups done
+// once instead of once per neighbour) -- then every pixel sums its
+// neighbours in the plain kernel's order with the same weights, so the sums
+// are bit-identical.
+constexpr int kResTile = 16, kResHalo = 4;
+constexpr float kResOut = -__builtin_huge_valf();  // weight slot outside the sample's footprint
+// Dynamic LDS, sized for the filter's halo rad: radiance (w: 1 valid sample,
+// 0 dropped or outside) per halo pixel, then its weights toward columns
+// x - qx + rad and rows y - qy + rad (res_lds_bytes).
+__host__ __device__ constexpr uint32_t res_lds_bytes(int rad) {
+  return uint32_t((kResTile + 2 * rad) * (kResTile + 2 * rad)) * (16u + 8u * uint32_t(2 * rad + 1));
+}
 __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t slot) {
-  __shared__ float4 s_l[kResEdge * kResEdge];   // radiance, w: 1 valid sample, 0 dropped or outside
-  __shared__ float2 s_c[kResEdge * kResEdge];   // sample position - 0.5
+  extern __shared__ float4 res_lds[];
   const WorkItem it = A.items[slot];
   const DevRequest &rq = A.reqs[it.req];
   int W = rq.width, H = rq.height;
   float R = rq.filter_radius;
-  int rad = (int)floorf(R + 0.5f);  // <= kResHalo (checked on the host)
-  int ew = kResTile + 2 * rad;
+  int rad = (int)floorf(R + 0.5f);  // <= kResHalo (checked on the host, which sized the LDS for it)
+  int ew = kResTile + 2 * rad, taps = 2 * rad + 1;
+  float4 *s_l = res_lds;
+  float *s_wx = reinterpret_cast<float *>(res_lds + ew * ew);
+  float *s_wy = s_wx + ew * ew * taps;
   int tiles_x = (W + kResTile - 1) / kResTile, tiles_y = (H + kResTile - 1) / kResTile;
   float4 *accum = reinterpret_cast<float4 *>(rq.accum);
   for (int tile = blockIdx.x; tile < tiles_x * tiles_y; tile += gridDim.x) {
@@ -1190,37 +1202,57 @@ __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t
     int x = tx0 + (int)threadIdx.x % kResTile, y = ty0 + (int)threadIdx.x / kResTile;
     bool own = x < W && y < H;
     float4 acc = own ? accum[y * W + x] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // the halo pixels this thread stages (at most kResPer), their sample
+    // ranks, and their radiance for the next sample, loaded one sample ahead
+    // so that the loads overlap the previous sample's sums
+    constexpr int kResPer = ((kResTile + 2 * kResHalo) * (kResTile + 2 * kResHalo) + kBlock - 1) / kBlock;
+    uint32_t rank[kResPer];
+    float4 next[kResPer];
+#pragma unroll
+    for (int j = 0; j < kResPer; j++) {
+      int e = (int)threadIdx.x + j * kBlock;
+      int qx = tx0 - rad + e % ew, qy = ty0 - rad + e / ew;
+      bool in = e < ew * ew && qx >= 0 && qx < W && qy >= 0 && qy < H;
+      rank[j] = in ? tile_rank((uint32_t)qx, (uint32_t)qy, W, H, rq.tile) : 0xffffffffu;
+      next[j] = (in && it.k > 0) ? A.sample_L[it.buf_base + rank[j]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     for (uint32_t s = 0; s < it.k; s++) {
       __syncthreads();  // the previous sample's tile has been read
-      for (int e = threadIdx.x; e < ew * ew; e += blockDim.x) {
+#pragma unroll
+      for (int j = 0; j < kResPer; j++) {
+        int e = (int)threadIdx.x + j * kBlock;
+        if (e >= ew * ew) break;
         int qx = tx0 - rad + e % ew, qy = ty0 - rad + e / ew;
         float4 l = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float2 c = make_float2(0.0f, 0.0f);
-        if (qx >= 0 && qx < W && qy >= 0 && qy < H) {
+        if (rank[j] != 0xffffffffu) {
           uint32_t q = (uint32_t)(qy * W + qx);
           Rng rng = rng_for_path(rq.seed, q, it.s0 + s);
           float jx = rng.next(), jy = rng.next();
-          c = make_float2(((float)qx + jx) - 0.5f, ((float)qy + jy) - 0.5f);
-          float4 l4 = A.sample_L[it.buf_base + s * it.npix + tile_rank((uint32_t)qx, (uint32_t)qy, W, H, rq.tile)];
+          float cx = ((float)qx + jx) - 0.5f, cy = ((float)qy + jy) - 0.5f;
+          float4 l4 = next[j];
+          if (s + 1 < it.k) next[j] = A.sample_L[it.buf_base + (s + 1) * it.npix + rank[j]];
           // invalid samples are dropped (Block.cpp:97-101)
           l = make_float4(l4.x, l4.y, l4.z, color_valid(v3(l4.x, l4.y, l4.z)) ? 1.0f : 0.0f);
+          int x0 = (int)ceilf(cx - R), x1 = (int)floorf(cx + R);
+          int y0 = (int)ceilf(cy - R), y1 = (int)floorf(cy + R);
+          for (int k = 0; k <= 2 * rad; k++) {
+            int tx = qx - rad + k, ty = qy - rad + k;
+            s_wx[e * taps + k] = (tx < x0 || tx > x1) ? kResOut : rq.filter[(int)(fabsf((float)tx - cx) * rq.lookup)];
+            s_wy[e * taps + k] = (ty < y0 || ty > y1) ? kResOut : rq.filter[(int)(fabsf((float)ty - cy) * rq.lookup)];
+          }
         }
         s_l[e] = l;
-        s_c[e] = c;
       }
       __syncthreads();
       if (!own) continue;
       for (int qy = max(0, y - rad); qy <= min(H - 1, y + rad); qy++) {
         for (int qx = max(0, x - rad); qx <= min(W - 1, x + rad); qx++) {
           int e = (qy - ty0 + rad) * ew + (qx - tx0 + rad);
-          float2 c = s_c[e];
-          float cx = c.x, cy = c.y;
-          if (x < (int)ceilf(cx - R) || x > (int)floorf(cx + R) || y < (int)ceilf(cy - R) || y > (int)floorf(cy + R))
-            continue;
+          float wx = s_wx[e * taps + (x - qx + rad)];
+          float wy = s_wy[e * taps + (y - qy + rad)];
+          if (wx == kResOut || wy == kResOut) continue;  // outside the sample's footprint
           float4 l4 = s_l[e];
           if (l4.w == 0.0f) continue;
-          float wx = rq.filter[(int)(fabsf((float)x - cx) * rq.lookup)];
-          float wy = rq.filter[(int)(fabsf((float)y - cy) * rq.lookup)];
           acc.x += l4.x * wx * wy;
           acc.y += l4.y * wx * wy;
           acc.z += l4.z * wx * wy;
@@ -1232,7 +1264,6 @@ __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t
   }
 }
 
-// Final ToBitmap (src/core/Block.cpp:61-72): rgb / w.
 __global__ __launch_bounds__(kBlock) void k_finalize(const float4 *accum, float *rgb, uint32_t n) {
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
@@ -1341,7 +1372,8 @@ hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, const DevRequest &
   int rad = (int)std::floor(rq.filter_radius + 0.5f);
   if (tiles && rq.tile && rad <= kResHalo && uint64_t(rq.width) * uint64_t(rq.height) == npix) {
     uint32_t tiles = uint32_t((rq.width + kResTile - 1) / kResTile) * uint32_t((rq.height + kResTile - 1) / kResTile);
-    hipLaunchKernelGGL(k_resolve_tiles, dim3(std::min(tiles, 1u << 16)), dim3(kBlock), 0, st, A, slot);
+    hipLaunchKernelGGL(k_resolve_tiles, dim3(std::min(tiles, 1u << 16)), dim3(kBlock), res_lds_bytes(rad), st, A,
+                       slot);
   } else {
     hipLaunchKernelGGL(k_resolve_plain, dim3(grid_for(npix, 1u << 16)), dim3(kBlock), 0, st, A, slot);
   }
