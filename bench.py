@@ -301,6 +301,17 @@ def main():
                      + tj["triangle_bytes"] * per["triangle_tests_per_ray"])
             eff = b_ray * pst["extension_rays"] / ext_launches / (ext_avg_ms * 1e-3) / 1e9
             effective = {"bytes_per_ray": b_ray, "achieved": eff, "frac": eff / HBM_PEAK_GBS}
+    # The bound k_extend actually meets is VALU issue (wave64 instructions on
+    # 16-lane SIMDs), measured by a committed rocprofv3 counter pass at the
+    # benchmark size; reported beside the HBM figure.
+    issue = None
+    vp = os.path.join(ROOT, "profiles", "valu_issue_c1_c2.json")
+    if os.path.exists(vp):
+        with open(vp) as fh:
+            vj = json.load(fh).get("c2" if a.workload == "c2" else "c1", {}).get("k_extend")
+        if vj:
+            issue = {"valu_issue_busy": vj["valu_issue_busy"], "lane_use": vj["lane_use"],
+                     "source": "profiles/valu_issue_c1_c2.json (rocprofv3 SQ counters, same workload)"}
     out = {
         "metric": "path samples/sec at matched relMSE vs plain GPU path; p50/p99 render latency",
         "value": value, "unit": "path samples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -320,7 +331,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "launch_ms": ext_avg_ms, "rays_per_launch": pst["extension_rays"] / ext_launches,
                      "timing": "HIP events around each k_extend launch of one extra (eager) render",
-                     "effective": effective},
+                     "effective": effective, "issue": issue},
         "render_roofline_frac": render_bytes / wall / (HBM_PEAK_GBS * 1e9),
     }
     if tot["node_visits"]:  # builds with traversal counters only
