@@ -263,35 +263,33 @@ void Engine::admit() {
 
 // Admits r's next chunks while they fit; true when all are admitted.
 bool Engine::admit_chunks(RequestRec *r) {
-  {
-    while (r->next_admit < r->n_chunks) {
-      uint32_t k = std::min(r->chunk, r->spp - r->next_admit * r->chunk);
-      uint32_t size = r->npix * k;
-      uint32_t base = 0;
-      if (free_items_.empty() || !alloc_range(size, base)) return false;
-      uint32_t slot = free_items_.back();
-      free_items_.pop_back();
-      ChunkRec &c = slots_[slot];
-      c = ChunkRec();
-      c.req = r;
-      c.index = r->next_admit;
-      c.size = size;
-      c.buf_base = base;
-      WorkItem *hw = ws_.h_items.as<WorkItem>() + slot;
-      std::memset(hw, 0, sizeof *hw);
-      hw->req = r->slot;
-      hw->npix = r->npix;
-      hw->s0 = r->spp_offset + r->next_admit * r->chunk;
-      hw->k = k;
-      hw->buf_base = base;
-      hip_check(hipMemcpyAsync(ws_.items.as<WorkItem>() + slot, hw, sizeof(WorkItem), hipMemcpyHostToDevice, st_),
-                "item upload");
-      hip_check(launch_clear_done(ws_.item_done.as<uint32_t>(), slot, st_), "item counter clear");
-      r->chunk_item[r->next_admit] = int32_t(slot);
-      if (r->t_admit < 0) r->t_admit = now_ms();
-      r->next_admit++;
-      active_.push_back(slot);
-    }
+  while (r->next_admit < r->n_chunks) {
+    uint32_t k = std::min(r->chunk, r->spp - r->next_admit * r->chunk);
+    uint32_t size = r->npix * k;
+    uint32_t base = 0;
+    if (free_items_.empty() || !alloc_range(size, base)) return false;
+    uint32_t slot = free_items_.back();
+    free_items_.pop_back();
+    ChunkRec &c = slots_[slot];
+    c = ChunkRec();
+    c.req = r;
+    c.index = r->next_admit;
+    c.size = size;
+    c.buf_base = base;
+    WorkItem *hw = ws_.h_items.as<WorkItem>() + slot;
+    std::memset(hw, 0, sizeof *hw);
+    hw->req = r->slot;
+    hw->npix = r->npix;
+    hw->s0 = r->spp_offset + r->next_admit * r->chunk;
+    hw->k = k;
+    hw->buf_base = base;
+    hip_check(hipMemcpyAsync(ws_.items.as<WorkItem>() + slot, hw, sizeof(WorkItem), hipMemcpyHostToDevice, st_),
+              "item upload");
+    hip_check(launch_clear_done(ws_.item_done.as<uint32_t>(), slot, st_), "item counter clear");
+    r->chunk_item[r->next_admit] = int32_t(slot);
+    if (r->t_admit < 0) r->t_admit = now_ms();
+    r->next_admit++;
+    active_.push_back(slot);
   }
   return true;
 }
