@@ -234,8 +234,10 @@ __device__ __forceinline__ void leaf_tris(const TraceTri *__restrict__ tris, Ray
 // One round of speculative while-while traversal for a wave (every lane
 // must call it).  Node phase: lanes descend; a lane that reaches a leaf
 // parks it in `pend` and keeps descending while any lane of the wave is
-// still looking for its first leaf.  Leaf phase: every parked leaf is
-// tested.  The lane's ray is finished when ref == kDone and pend == kDone.
+// still looking for its first leaf (closest-hit rays park up to two leaves;
+// any-hit rays one, which keeps k_shadow's registers at five waves).  Leaf
+// phase: every parked leaf is tested.  The lane's ray is finished when
+// ref == kDone and pend == kDone.
 // Visiting nodes before a parked leaf has lowered bt only costs extra
 // visits: the result is the minimum t (largest prim among equal t) in any
 // order.
@@ -246,10 +248,16 @@ __device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, 
   while (R.ref >= 0) node_step(nodes, R, cnt);
   if (R.ref != kDone) leaf_step<ANY>(tris, R, cnt);
 #else
+  int32_t pend2 = kDone;  // closest-hit rays park a second leaf rather than wait
   for (;;) {
-    if (R.ref < 0 && R.ref != kDone && pend == kDone) {
-      pend = R.ref;
-      R.ref = R.st.pop(R.bt * kWiden);
+    if (R.ref < 0 && R.ref != kDone) {
+      if (pend == kDone) {
+        pend = R.ref;
+        R.ref = R.st.pop(R.bt * kWiden);
+      } else if (!ANY && pend2 == kDone) {
+        pend2 = R.ref;
+        R.ref = R.st.pop(R.bt * kWiden);
+      }
     }
     if (__ballot(R.ref >= 0 && pend == kDone) == 0ull) break;
     if (R.ref >= 0) node_step(nodes, R, cnt);
@@ -257,6 +265,10 @@ __device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, 
   if (pend != kDone) {
     leaf_tris<ANY>(tris, R, pend, cnt);
     pend = kDone;
+  }
+  if (!ANY && pend2 != kDone) {
+    leaf_tris<ANY>(tris, R, pend2, cnt);
+    pend2 = kDone;
   }
 #endif
 }
