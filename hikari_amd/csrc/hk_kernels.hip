@@ -231,12 +231,13 @@ __device__ __forceinline__ void leaf_tris(const TraceTri *__restrict__ tris, Ray
   }
 }
 
+constexpr int kLeafEarly = 8;
 // One round of speculative while-while traversal for a wave (every lane
 // must call it).  Node phase: lanes descend; a lane that reaches a leaf
-// parks it in `pend` and keeps descending while any lane of the wave is
-// still looking for its first leaf (closest-hit rays park up to two leaves;
-// any-hit rays one, which keeps k_shadow's registers at five waves).  Leaf
-// phase: every parked leaf is tested.  The lane's ray is finished when
+// parks it in `pend` and keeps descending while enough lanes of the wave are
+// still looking for their first leaf (closest-hit rays park up to two
+// leaves; any-hit rays one, which keeps k_shadow's registers at five waves).
+// Leaf phase: every parked leaf is tested.  The lane's ray is finished when
 // ref == kDone and pend == kDone.
 // Visiting nodes before a parked leaf has lowered bt only costs extra
 // visits: the result is the minimum t (largest prim among equal t) in any
@@ -259,7 +260,12 @@ __device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, 
         R.ref = R.st.pop(R.bt * kWiden);
       }
     }
-    if (__ballot(R.ref >= 0 && pend == kDone) == 0ull) break;
+    // The leaf phase starts once fewer than kLeafEarly lanes still look for a
+    // first leaf and some lane holds one: a few searching lanes no longer
+    // keep the rest of the wave descending.  Every round makes progress
+    // (node steps, or at least one parked leaf tested), so the loop ends.
+    int searching = __popcll(__ballot(R.ref >= 0 && pend == kDone));
+    if (searching == 0 || (searching < kLeafEarly && __ballot(pend != kDone) != 0ull)) break;
     if (R.ref >= 0) node_step(nodes, R, cnt);
   }
   if (pend != kDone) {
