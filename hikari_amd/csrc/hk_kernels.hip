@@ -707,13 +707,12 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi) {
 // ------------------------------------------------------------------ k_extend
 // Closest hit of every queued ray (Scene::RayIntersect, src/core/Scene.cpp:
 // 117-120).  Persistent lanes: a wave keeps tracing and, whenever at least
-// A.refill_extend of its lanes have finished their rays, fetches that many
-// new queue entries with one atomic (counters[4], cleared by k_regen /
-// k_shade before), so a few long rays no longer hold a whole wave.  The
-// threshold is a run-time option (1..64): fetching rarely keeps the wave's
-// lanes in step, fetching often keeps them busy; 48-64 is fastest on the
-// benchmark scenes (16: -20%).  HK_PLAIN_TRACE builds trace one ray per
-// lane instead.
+// A.refill_extend of its lanes have finished their rays, gives them new queue
+// entries from its reserved range (wave_fetch: A.fetch_batch x 64 entries per
+// atomic on counters[4], cleared by k_regen / k_shade before), so a few long
+// rays no longer hold a whole wave.  Both are run-time options; 24 idle lanes
+// and 2 x 64 entries are fastest on the benchmark scenes.  HK_PLAIN_TRACE
+// builds trace one ray per lane instead.
 constexpr uint32_t kNoRay = 0xffffffffu;
 
 #ifndef HK_PLAIN_TRACE
