@@ -8,7 +8,7 @@
 //  * wavefront pathMIS (PathMISIntegrator::Li, src/integrator/
 //    PathMISIntegrator.cpp:16-157): [k_regen -> k_extend -> k_shade ->
 //    k_shadow]* -> k_resolve.  Path state lives in the extension queues
-//    (SoA, compacted with wave ballots, one atomic per wave) so every kernel
+//    (SoA, compacted with wave ballots, one atomic per block) so every kernel
 //    reads and writes it coalesced; k_regen tops the queue up with new
 //    samples, so every bounce launch stays full until the segment's samples
 //    run out.
@@ -356,19 +356,6 @@ __global__ __launch_bounds__(kBlock) void k_bsdf_batch(const DevMaterial *__rest
 
 // ------------------------------------------------------------------ wavefront helpers
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
-
-// Wave-aggregated append: one atomic per wave, slots in lane order.
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
-  unsigned long long m = __ballot(pred);
-  if (m == 0ull) return 0xffffffffu;
-  uint32_t lane = lane_id();
-  uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, (int)leader);
-  unsigned long long below = lane == 0 ? 0ull : (m & ((1ull << lane) - 1ull));
-  return pred ? base + (uint32_t)__popcll(below) : 0xffffffffu;
-}
 
 // Block-aggregated append of two predicates to two counters: one atomic per
 // block and counter (the waves' counts are summed in LDS); slots in lane
