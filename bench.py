@@ -59,7 +59,7 @@ def parse():
     ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
                     help="BVH construction: binned SAH on the host or linear BVH on the device")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["c1", "c2", "batch", "serve"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "batch", "serve", "accel"], default="c1")
     ap.add_argument("--requests", type=int, default=0, help="batch/serve: number of requests (default 64 / 40)")
     ap.add_argument("--qps", type=float, default=1.0, help="serve: Poisson arrival rate")
     ap.add_argument("--budget-gb", type=float, default=96.0, help="serve: HBM budget of the server")
@@ -202,6 +202,52 @@ def bench_serve(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_accel(a):
+    """Batched rayIntersect (hk_accel_intersect_device) over the displaced-sphere
+    mesh: 32 M rays leaving points just above the surface in uniform random
+    directions, resident in HBM; closest-hit and any-hit rays per second."""
+    import torch
+    import hikari_amd as hk
+    from hikari_amd import scenes
+    scene = scenes.displaced_sphere(alpha=a.alpha)
+    v = np.concatenate([m.v for m in scene.meshes]).astype(np.float32)
+    f, base = [], 0
+    for m in scene.meshes:
+        f.append(m.f.astype(np.int64) + base)
+        base += m.v.shape[0]
+    f = np.concatenate(f).astype(np.uint32)
+    ctx = hk.Context(0)
+    acc = ctx.accel(v, f)
+    n = 1 << 25
+    g = torch.Generator(device="cuda").manual_seed(5)
+    o = torch.nn.functional.normalize(torch.randn(n, 3, device="cuda", generator=g), dim=1) * 1.1
+    d = torch.nn.functional.normalize(torch.randn(n, 3, device="cuda", generator=g), dim=1)
+    rays = torch.cat([o, torch.full((n, 1), 1e-4, device="cuda"), d,
+                      torch.full((n, 1), float("inf"), device="cuda")], dim=1).contiguous()
+    hits = torch.empty((n, 4), device="cuda")
+    out = {}
+    for any_hit in (False, True):
+        acc.intersect_device(rays.data_ptr(), hits.data_ptr(), n, any_hit)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            acc.intersect_device(rays.data_ptr(), hits.data_ptr(), n, any_hit)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / a.steps
+        out["any_hit" if any_hit else "closest_hit"] = {"rays_per_s": n / dt, "ms": dt * 1e3,
+                                                        "hit_rate": float((hits[:, 3].view(torch.int32) >= 0)
+                                                                          .float().mean())}
+    line = {
+        "metric": "batched rayIntersect rays/s (hk_accel_intersect_device)", "value": out["closest_hit"]["rays_per_s"],
+        "unit": "rays/s", "n_gpus": 1, "steps": a.steps, "warmup": 1, "ms_per_step": out["closest_hit"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded rays from a sphere of radius 1.1 in uniform directions)",
+        "config": {"workload": f"{n} rays over the displaced icosphere-8 mesh ({f.shape[0]} tris), rays in HBM"},
+        "closest_hit": out["closest_hit"], "any_hit": out["any_hit"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -212,6 +258,8 @@ def main():
         return bench_batch(a)
     if a.workload == "serve":
         return bench_serve(a)
+    if a.workload == "accel":
+        return bench_accel(a)
     import hikari_amd as hk
     from hikari_amd import scenes
 

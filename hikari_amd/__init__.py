@@ -194,6 +194,12 @@ class Accel:
         prim = hits[:, 3].view(np.int32).copy()
         return hits[:, 0].copy(), hits[:, 1].copy(), hits[:, 2].copy(), prim
 
+    def intersect_device(self, rays_ptr, hits_ptr, n, any_hit=False):
+        """Device arrays (e.g. torch tensors' data_ptr()): rays n x 8 float32,
+        hits n x 4; runs on the context's stream without waiting for it."""
+        self.ctx.check(lib().hk_accel_intersect_device(self.h, C.c_void_p(rays_ptr), int(n), C.c_void_p(hits_ptr),
+                                                       int(any_hit)))
+
     def __del__(self):
         try:
             if self.h:

@@ -85,6 +85,11 @@ hipError_t launch_shade_tris(const float *V, const float *N, const float *UV, co
 
 hipError_t launch_intersect(const TraceNode *nodes, const TraceTri *tris, int32_t root, const void *rays, void *hits,
                             uint32_t n, int any, hipStream_t st);
+// The same queries on persistent waves (counter: one device word, zeroed by
+// the launch; refill / batch as the render's "refill" / "fetch_batch").
+hipError_t launch_intersect_persistent(const TraceNode *nodes, const TraceTri *tris, int32_t root, const void *rays,
+                                       void *hits, uint32_t n, int any, uint32_t *counter, uint32_t refill,
+                                       uint32_t batch, hipStream_t st);
 hipError_t launch_bsdf_batch(const DevMaterial *mats, const float *rft, int idx, uint32_t n, const float *wi,
                              const float *wo, const int32_t *meas, const float *u, float *out_eval, float *out_pdf,
                              float *out_f, float *out_wo, int32_t *out_meas, float *out_eta, float *out_spdf,

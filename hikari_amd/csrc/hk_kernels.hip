@@ -777,6 +777,52 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
 }
 #endif
 
+// ------------------------------------------------------------------ batched rayIntersect, persistent
+// Acceleration::RayIntersect over an array of rays with k_extend's machinery:
+// persistent waves that refill from reserved ranges of the ray array
+// (counter: one zeroed word) and speculative traversal rounds.  Closest hits
+// equal k_intersect's bit for bit (the closest hit does not depend on the
+// visit order); any-hit queries agree on whether a hit exists.
+template <bool ANY>
+__global__ __launch_bounds__(kBlock) void k_intersect_persistent(const TraceNode *__restrict__ nodes,
+                                                                 const TraceTri *__restrict__ tris, int32_t root,
+                                                                 const float4 *__restrict__ rays,
+                                                                 float4 *__restrict__ hits, uint32_t n,
+                                                                 uint32_t *counter, uint32_t refill, uint32_t batch) {
+  __shared__ uint64_t stack[kLdsStack * kBlock];
+  TraceCount tc;
+  RayTrav R;
+  uint64_t priv[kPrivStack];
+  R.ref = kDone;
+  int32_t pend = kDone;
+  uint32_t rid = kNoRay;
+  WaveFetch wf;
+  fetch_init(wf);
+  for (;;) {
+    unsigned long long idle = __ballot(rid == kNoRay);
+    int n_idle = __popcll(idle);
+    if (!fetch_exhausted(wf) && n_idle >= (int)refill) {
+      uint32_t idx = wave_fetch(wf, counter, n, batch, idle);
+      if (idx != kNoRay) {
+        rid = idx;
+        float4 a = rays[2 * size_t(idx)], b = rays[2 * size_t(idx) + 1];
+        trav_begin(R, stack, priv, root, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w);
+        pend = kDone;
+      }
+    }
+    if (__ballot(rid != kNoRay) == 0ull) {
+      if (fetch_exhausted(wf)) break;
+      continue;
+    }
+    trav_round<ANY>(nodes, tris, R, pend, tc);
+    if (rid != kNoRay && R.ref == kDone && pend == kDone) {
+      hits[rid] = R.bprim >= 0 ? make_float4(R.bt, R.bu, R.bv, __int_as_float(R.bprim))
+                               : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+      rid = kNoRay;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ k_shade
 constexpr uint32_t kSortKeys = 32;  // material keys of the block sort (last two: miss, idle lane)
 // One bounce of PathMISIntegrator::Li for every queued path.
@@ -1333,6 +1379,7 @@ hipError_t launch_intersect(const TraceNode *nodes, const TraceTri *tris, int32_
   return hipGetLastError();
 }
 
+
 hipError_t launch_bsdf_batch(const DevMaterial *mats, const float *rft, int idx, uint32_t n, const float *wi,
                              const float *wo, const int32_t *meas, const float *u, float *out_eval, float *out_pdf,
                              float *out_f, float *out_wo, int32_t *out_meas, float *out_eta, float *out_spdf,
@@ -1357,6 +1404,29 @@ static uint32_t resident_blocks(const void *kernel) {
   return uint32_t(std::max(1, n_cu * per_cu));
 }
 
+hipError_t launch_intersect_persistent(const TraceNode *nodes, const TraceTri *tris, int32_t root, const void *rays,
+                                       void *hits, uint32_t n, int any, uint32_t *counter, uint32_t refill,
+                                       uint32_t batch, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+#ifndef HK_PLAIN_TRACE
+  if (refill < 1 || refill > 64 || batch < 1) return hipErrorInvalidValue;
+  hipError_t e = launch_fill_u32(counter, 0u, 1u, st);
+  if (e != hipSuccess) return e;
+  uint32_t want = grid_for(n, 1u << 20);
+  if (any) {
+    static uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_intersect_persistent<true>));
+    hipLaunchKernelGGL(k_intersect_persistent<true>, dim3(std::min(want, cap)), dim3(kBlock), 0, st, nodes, tris,
+                       root, (const float4 *)rays, (float4 *)hits, n, counter, refill, batch);
+  } else {
+    static uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_intersect_persistent<false>));
+    hipLaunchKernelGGL(k_intersect_persistent<false>, dim3(std::min(want, cap)), dim3(kBlock), 0, st, nodes, tris,
+                       root, (const float4 *)rays, (float4 *)hits, n, counter, refill, batch);
+  }
+  return hipGetLastError();
+#else
+  return launch_intersect(nodes, tris, root, rays, hits, n, any, st);
+#endif
+}
 hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st) {
   static uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_extend));
   hipLaunchKernelGGL(k_extend, dim3(std::min(grid, cap)), dim3(kBlock), 0, st, A, qi);

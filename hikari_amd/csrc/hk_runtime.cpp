@@ -487,6 +487,7 @@ int hk_accel_build(hk_ctx *ctx, const float *v, uint32_t nv, const uint32_t *idx
   a->ctx = ctx;
   auto t0 = std::chrono::steady_clock::now();
   upload_bvh(ctx, tri9.data(), nt, a->bvh);
+  a->fetch.alloc(256);
   a->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   hip_check(hipStreamSynchronize(ctx->stream), "accel upload");
   *out = a.release();
@@ -512,8 +513,10 @@ int hk_accel_intersect_device(hk_accel *a, const hk_ray *d_rays, uint32_t n, hk_
   hk_ctx *ctx = a ? a->ctx : nullptr;
   HK_API_BEGIN
   if (!a || (n && (!d_rays || !d_hits))) throw HkError(HK_ERR_INVALID, "null argument");
-  hip_check(launch_intersect((const TraceNode *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, d_rays, d_hits,
-                             n, any_hit, ctx->stream),
+  hip_check(launch_intersect_persistent((const TraceNode *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, d_rays,
+                                        d_hits, n, any_hit, a->fetch.as<uint32_t>(),
+                                        any_hit ? ctx->refill_shadow : ctx->refill_extend, ctx->fetch_batch,
+                                        ctx->stream),
             "k_intersect launch");
   return HK_OK;
   HK_API_END(ctx)
@@ -527,8 +530,10 @@ int hk_accel_intersect(hk_accel *a, const hk_ray *rays, uint32_t n, hk_hit *hits
   DBuf dr, dh;
   dr.upload(rays, sizeof(hk_ray) * size_t(n), ctx->stream);
   dh.alloc(sizeof(hk_hit) * size_t(n));
-  hip_check(launch_intersect((const TraceNode *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, dr.p, dh.p, n,
-                             any_hit, ctx->stream),
+  hip_check(launch_intersect_persistent((const TraceNode *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, dr.p,
+                                        dh.p, n, any_hit, a->fetch.as<uint32_t>(),
+                                        any_hit ? ctx->refill_shadow : ctx->refill_extend, ctx->fetch_batch,
+                                        ctx->stream),
             "k_intersect launch");
   hip_check(hipMemcpyAsync(hits, dh.p, sizeof(hk_hit) * size_t(n), hipMemcpyDeviceToHost, ctx->stream), "D2H");
   hip_check(hipStreamSynchronize(ctx->stream), "hk_accel_intersect");

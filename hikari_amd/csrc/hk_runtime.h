@@ -104,6 +104,7 @@ struct hk_ctx {
 struct hk_accel {
   hk_ctx *ctx = nullptr;
   hk::DevBvh bvh;
+  hk::DBuf fetch;  // the persistent intersect kernel's fetch counter (one word)
   double build_ms = 0;
 };
 

@@ -51,6 +51,34 @@ def test_intersect_anyhit_cbox(ctx):
     assert np.array_equal(g[3] >= 0, o[3] >= 0)
 
 
+def test_intersect_persistent_batches_and_options(ctx):
+    """The persistent batched intersect: batches of 1 to 1000 rays, and refill /
+    fetch-batch settings from 1 to 64, give the brute-force closest hits bit
+    for bit, and the same occlusion answers."""
+    rng = np.random.default_rng(14)
+    sc = scenes.cbox()
+    v, f = scene_vf(sc)
+    tris = scene_triangles(sc)
+    rays = box_rays(rng, 5000, -1.2, 1.2)
+    o = O.intersect(tris, rays, use_bvh=False)
+    acc = ctx.accel(v, f)
+    for n in (1, 2, 63, 65, 1000, 5000):
+        g = acc.intersect(rays[:n])
+        assert _hits_equal(g, tuple(x[:n] for x in o)).all(), n
+    short = rays.copy()
+    short[:, 7] = rng.uniform(0.05, 2.0, len(rays))
+    oa = O.intersect(tris, short, any_hit=True, use_bvh=False)
+    for refill, batch in ((1, 1), (64, 8), (24, 64)):
+        c = hk.Context(0)
+        c.set_option("refill", refill)
+        c.set_option("refill_shadow", refill)
+        c.set_option("fetch_batch", batch)
+        a2 = c.accel(v, f)
+        assert _hits_equal(a2.intersect(rays), o).all(), (refill, batch)
+        assert np.array_equal(a2.intersect(short, any_hit=True)[3] >= 0, oa[3] >= 0), (refill, batch)
+        c.close()
+
+
 def test_intersect_sphere_bvh_bitexact(ctx):
     """~82k-triangle displaced icosphere: GPU BVH4 == oracle (brute-force-equivalent) closest hits."""
     rng = np.random.default_rng(13)
