@@ -1409,7 +1409,8 @@ hipError_t launch_intersect_persistent(const TraceNode *nodes, const TraceTri *t
                                        uint32_t batch, hipStream_t st) {
   if (n == 0) return hipSuccess;
 #ifndef HK_PLAIN_TRACE
-  if (refill < 1 || refill > 64 || batch < 1) return hipErrorInvalidValue;
+  // the fetch counter may run past n by up to one reservation per wave
+  if (refill < 1 || refill > 64 || batch < 1 || batch > 64 || n > (1u << 31)) return hipErrorInvalidValue;
   hipError_t e = launch_fill_u32(counter, 0u, 1u, st);
   if (e != hipSuccess) return e;
   uint32_t want = grid_for(n, 1u << 20);

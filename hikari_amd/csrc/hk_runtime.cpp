@@ -513,6 +513,7 @@ int hk_accel_intersect_device(hk_accel *a, const hk_ray *d_rays, uint32_t n, hk_
   hk_ctx *ctx = a ? a->ctx : nullptr;
   HK_API_BEGIN
   if (!a || (n && (!d_rays || !d_hits))) throw HkError(HK_ERR_INVALID, "null argument");
+  if (n > (1u << 31)) throw HkError(HK_ERR_INVALID, "at most 2^31 rays per call");
   hip_check(launch_intersect_persistent((const TraceNode *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, d_rays,
                                         d_hits, n, any_hit, a->fetch.as<uint32_t>(),
                                         any_hit ? ctx->refill_shadow : ctx->refill_extend, ctx->fetch_batch,
@@ -526,6 +527,7 @@ int hk_accel_intersect(hk_accel *a, const hk_ray *rays, uint32_t n, hk_hit *hits
   hk_ctx *ctx = a ? a->ctx : nullptr;
   HK_API_BEGIN
   if (!a || (n && (!rays || !hits))) throw HkError(HK_ERR_INVALID, "null argument");
+  if (n > (1u << 31)) throw HkError(HK_ERR_INVALID, "at most 2^31 rays per call");
   if (n == 0) return HK_OK;
   DBuf dr, dh;
   dr.upload(rays, sizeof(hk_ray) * size_t(n), ctx->stream);

@@ -85,7 +85,10 @@ int hk_accel_get_info(const hk_accel *accel, hk_accel_info *info);
  * Host buffers. */
 int hk_accel_intersect(hk_accel *accel, const hk_ray *rays, uint32_t n, hk_hit *hits, int any_hit);
 /* Same on device buffers (d_rays, d_hits in HBM) on the context stream;
- * asynchronous. */
+ * asynchronous.  Both take n <= 2^31 rays per call (HK_ERR_INVALID beyond) and
+ * trace them on persistent waves tuned by the context's "refill" (closest
+ * hit), "refill_shadow" (any hit) and "fetch_batch" options.  Calls on one
+ * accelerator must not overlap (they share its fetch counter). */
 int hk_accel_intersect_device(hk_accel *accel, const hk_ray *d_rays, uint32_t n, hk_hit *d_hits, int any_hit);
 
 /* ------------------------------------------------------------ BSDFs */
