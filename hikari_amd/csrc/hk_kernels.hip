@@ -27,7 +27,7 @@ namespace hk {
 
 constexpr int kBlock = 256;          // threads per block for every kernel here
 #ifndef HK_LDS_STACK
-#define HK_LDS_STACK 8
+#define HK_LDS_STACK 10
 #endif
 constexpr int kLdsStack = HK_LDS_STACK;  // stack entries per lane kept in LDS
 constexpr int kPrivStack = 96 - kLdsStack;  // further entries in private memory
