@@ -1225,12 +1225,15 @@ __global__ __launch_bounds__(kBlock) void k_resolve_plain(RenderArgs A, uint32_t
 // and row its footprint can reach (jitter recomputed once, table lookups done
 // once instead of once per neighbour) -- then every pixel sums its
 // neighbours in the plain kernel's order with the same weights, so the sums
-// are bit-identical.
+// are bit-identical.  Without branches: a dropped sample is staged as zero
+// radiance and weight flag, a column or row outside a sample's footprint as
+// weight 0, so what the plain kernel skips adds +0 here (products of finite
+// non-negative values; the sums start at +0 and never become -0).
 constexpr int kResTile = 16, kResHalo = 4;
-constexpr float kResOut = -__builtin_huge_valf();  // weight slot outside the sample's footprint
 // Dynamic LDS, sized for the filter's halo rad: radiance (w: 1 valid sample,
-// 0 dropped or outside) per halo pixel, then its weights toward columns
-// x - qx + rad and rows y - qy + rad (res_lds_bytes).
+// 0 dropped or outside; all zero unless valid) per halo pixel, then its
+// weights toward columns x - qx + rad and rows y - qy + rad (0 outside its
+// footprint) (res_lds_bytes).
 __host__ __device__ constexpr uint32_t res_lds_bytes(int rad) {
   return uint32_t((kResTile + 2 * rad) * (kResTile + 2 * rad)) * (16u + 8u * uint32_t(2 * rad + 1));
 }
@@ -1282,13 +1285,13 @@ __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t
           float4 l4 = next[j];
           if (s + 1 < it.k) next[j] = A.sample_L[it.buf_base + (s + 1) * it.npix + rank[j]];
           // invalid samples are dropped (Block.cpp:97-101)
-          l = make_float4(l4.x, l4.y, l4.z, color_valid(v3(l4.x, l4.y, l4.z)) ? 1.0f : 0.0f);
+          if (color_valid(v3(l4.x, l4.y, l4.z))) l = make_float4(l4.x, l4.y, l4.z, 1.0f);
           int x0 = (int)ceilf(cx - R), x1 = (int)floorf(cx + R);
           int y0 = (int)ceilf(cy - R), y1 = (int)floorf(cy + R);
           for (int k = 0; k <= 2 * rad; k++) {
             int tx = qx - rad + k, ty = qy - rad + k;
-            s_wx[e * taps + k] = (tx < x0 || tx > x1) ? kResOut : rq.filter[(int)(fabsf((float)tx - cx) * rq.lookup)];
-            s_wy[e * taps + k] = (ty < y0 || ty > y1) ? kResOut : rq.filter[(int)(fabsf((float)ty - cy) * rq.lookup)];
+            s_wx[e * taps + k] = (tx < x0 || tx > x1) ? 0.0f : rq.filter[(int)(fabsf((float)tx - cx) * rq.lookup)];
+            s_wy[e * taps + k] = (ty < y0 || ty > y1) ? 0.0f : rq.filter[(int)(fabsf((float)ty - cy) * rq.lookup)];
           }
         }
         s_l[e] = l;
@@ -1300,13 +1303,11 @@ __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t
           int e = (qy - ty0 + rad) * ew + (qx - tx0 + rad);
           float wx = s_wx[e * taps + (x - qx + rad)];
           float wy = s_wy[e * taps + (y - qy + rad)];
-          if (wx == kResOut || wy == kResOut) continue;  // outside the sample's footprint
           float4 l4 = s_l[e];
-          if (l4.w == 0.0f) continue;
           acc.x += l4.x * wx * wy;
           acc.y += l4.y * wx * wy;
           acc.z += l4.z * wx * wy;
-          acc.w += 1.0f * wx * wy;
+          acc.w += l4.w * wx * wy;
         }
       }
     }
