@@ -354,11 +354,14 @@ def test_heightfield_image_vs_oracle(ctx):
     assert abs(lum(img).mean() - lum(ref).mean()) / lum(ref).mean() < 0.01
 
 
-@pytest.mark.parametrize("filt", [D.FILTER_GAUSSIAN, D.FILTER_BOX, D.FILTER_MITCHELL])
-def test_tiled_resolve_equals_plain(ctx, filt):
+@pytest.mark.parametrize("filt,radius", [(D.FILTER_GAUSSIAN, 2.0), (D.FILTER_BOX, 0.5), (D.FILTER_MITCHELL, 2.0),
+                                          (D.FILTER_TENT, 1.0), (D.FILTER_GAUSSIAN, 3.2), (D.FILTER_GAUSSIAN, 0.4)])
+def test_tiled_resolve_equals_plain(ctx, filt, radius):
     """The LDS-tiled film resolve sums exactly what the plain per-pixel gather
-    sums (a ragged 70 x 45 film, so edge tiles are partial)."""
+    sums (a ragged 70 x 45 film, so edge tiles are partial), for filter halos
+    of 0 to 3 pixels."""
     sc = scenes.cbox(width=70, height=45, sphere_subdiv=2, filter=filt)
+    sc.filter_radius = radius
     a, _ = ctx.scene(sc).render(6, seed=8)
     ctx.set_option("resolve_tiles", 0)
     try:
