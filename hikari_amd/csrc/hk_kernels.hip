@@ -1245,11 +1245,15 @@ __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t
   float R = rq.filter_radius;
   int rad = (int)floorf(R + 0.5f);  // <= kResHalo (checked on the host, which sized the LDS for it)
   int ew = kResTile + 2 * rad, taps = 2 * rad + 1;
+  // the filter table, read ten times per halo pixel and sample
+  __shared__ float s_filt[33];
+  if (threadIdx.x < 33) s_filt[threadIdx.x] = rq.filter[threadIdx.x];
   float4 *s_l = res_lds;
   float *s_wx = reinterpret_cast<float *>(res_lds + ew * ew);
   float *s_wy = s_wx + ew * ew * taps;
   int tiles_x = (W + kResTile - 1) / kResTile, tiles_y = (H + kResTile - 1) / kResTile;
   float4 *accum = reinterpret_cast<float4 *>(rq.accum);
+  const float lookup = rq.lookup;
   for (int tile = blockIdx.x; tile < tiles_x * tiles_y; tile += gridDim.x) {
     int tx0 = (tile % tiles_x) * kResTile, ty0 = (tile / tiles_x) * kResTile;
     int x = tx0 + (int)threadIdx.x % kResTile, y = ty0 + (int)threadIdx.x / kResTile;
@@ -1290,8 +1294,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t
           int y0 = (int)ceilf(cy - R), y1 = (int)floorf(cy + R);
           for (int k = 0; k <= 2 * rad; k++) {
             int tx = qx - rad + k, ty = qy - rad + k;
-            s_wx[e * taps + k] = (tx < x0 || tx > x1) ? 0.0f : rq.filter[(int)(fabsf((float)tx - cx) * rq.lookup)];
-            s_wy[e * taps + k] = (ty < y0 || ty > y1) ? 0.0f : rq.filter[(int)(fabsf((float)ty - cy) * rq.lookup)];
+            s_wx[e * taps + k] = (tx < x0 || tx > x1) ? 0.0f : s_filt[(int)(fabsf((float)tx - cx) * lookup)];
+            s_wy[e * taps + k] = (ty < y0 || ty > y1) ? 0.0f : s_filt[(int)(fabsf((float)ty - cy) * lookup)];
           }
         }
         s_l[e] = l;
