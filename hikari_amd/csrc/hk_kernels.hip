@@ -833,8 +833,22 @@ constexpr uint32_t kSortKeys = 32;  // material keys of the block sort (last two
 #ifndef HK_SHADE_MIN_WAVES
 #define HK_SHADE_MIN_WAVES 4
 #endif
+// LDS_MATS: the scene's material records (at most kShadeMats) are copied to
+// LDS once per block; the BSDF code reads them there at each call instead of
+// from global memory.
+constexpr int kShadeMats = 16;
+template <bool LDS_MATS>
 __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs A, int qi) {
   const DevScene &sc = A.sc;
+  __shared__ DevMaterial s_mats[LDS_MATS ? kShadeMats : 1];
+  const DevMaterial *mats = sc.mats;
+  if constexpr (LDS_MATS) {
+    constexpr int kVec = int(sizeof(DevMaterial) / sizeof(float4));
+    for (int k = (int)threadIdx.x; k < sc.n_mats * kVec; k += kBlock)
+      reinterpret_cast<float4 *>(s_mats)[k] = reinterpret_cast<const float4 *>(sc.mats)[k];
+    __syncthreads();
+    mats = s_mats;
+  }
   uint32_t count = A.counters[ctr(qi)];
   if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[ctr(4)] = 0u;  // k_extend's fetch counter
   int qn = qi ^ 1;
@@ -914,7 +928,7 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
         post_intersect(sc, prim, h4.y, h4.z, P, fr, mesh_id);
         const DevMesh &mesh = sc.meshes[mesh_id];
         int mat = (int)mesh.material;
-        bool diffuse_mat = (sc.mats[mat].flags & kMatFlagDiffuse) != 0;
+        bool diffuse_mat = (mats[mat].flags & kMatFlagDiffuse) != 0;
         vec3 le = v3(0.0f);
         if (mesh.emitter >= 0 && mode != 0) {
           // PathMATSIntegrator.cpp:42-50 and WhittedIntegrator.cpp:54-62
@@ -967,8 +981,8 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
             if (!try_ems) continue;
             br.wo = to_local(fr, er.wi);
           }
-          float pb = (mode == 0 || self) ? bsdf_pdf(sc.mats, sc.rft, mat, br) : 0.0f;
-          vec3 fe = bsdf_eval(sc.mats, sc.rft, mat, br);
+          float pb = (mode == 0 || self) ? bsdf_pdf(mats, sc.rft, mat, br) : 0.0f;
+          vec3 fe = bsdf_eval(mats, sc.rft, mat, br);
           if (self) {
             if (pb != 0.0f) L = L + beta * (fe / pb * fabsf(br.wo.z) * le);
             continue;
@@ -998,7 +1012,7 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
           br.measure = kUnknownMeasure;
           float sx = rng.next(), sy = rng.next();
           U1 u1{&rng, nullptr, 0, 0};
-          vec3 f = bsdf_sample(sc.mats, sc.rft, mat, br, sx, sy, u1);
+          vec3 f = bsdf_sample(mats, sc.rft, mat, br, sx, sy, u1);
           nd = to_world(fr, br.wo);
           beta = beta * f;
           if (mode == 3) {
@@ -1007,7 +1021,7 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
             cont = !(beta.x == 0.0f && beta.y == 0.0f && beta.z == 0.0f);
             if (cont) depth++;
           } else if (!is_zero_fuzzy(beta)) {
-            pdf_b = bsdf_pdf(sc.mats, sc.rft, mat, br);
+            pdf_b = bsdf_pdf(mats, sc.rft, mat, br);
             flags = 0u;
             if (br.measure == kDiscrete) {
               w_mats = 1.0f;
@@ -1439,7 +1453,10 @@ hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t
   return hipGetLastError();
 }
 hipError_t launch_shade(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, st, A, qi);
+  if (A.sc.n_mats <= kShadeMats)
+    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kBlock), 0, st, A, qi);
+  else
+    hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kBlock), 0, st, A, qi);
   return hipGetLastError();
 }
 hipError_t launch_shadow(const RenderArgs &A, int qn, uint32_t grid, hipStream_t st) {

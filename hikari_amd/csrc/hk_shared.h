@@ -137,6 +137,7 @@ struct DevScene {
   int32_t force_background;
   int32_t root_ref;
   int32_t integrator;       // 0 pathMIS, 1 pathEMS, 2 pathMATS
+  int32_t n_mats;           // records in mats
 };
 
 // One render request as the kernels see it.

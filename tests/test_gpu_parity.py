@@ -354,6 +354,19 @@ def test_heightfield_image_vs_oracle(ctx):
     assert abs(lum(img).mean() - lum(ref).mean()) / lum(ref).mean() < 0.01
 
 
+def test_many_materials_equal_few(ctx):
+    """A scene with more material records than the shading kernel stages in
+    LDS (16) reads them from global memory; the unused extra records change no
+    bit of the film."""
+    sc = scenes.cbox(width=40, height=30, sphere_subdiv=2)
+    a, _ = ctx.scene(sc).render(4, seed=3)
+    for k in range(20):
+        sc.add_material(D.diffuse(0.1 + 0.02 * k))
+    b, _ = ctx.scene(sc).render(4, seed=3)
+    assert len(sc.materials) > 16
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 @pytest.mark.parametrize("filt,radius", [(D.FILTER_GAUSSIAN, 2.0), (D.FILTER_BOX, 0.5), (D.FILTER_MITCHELL, 2.0),
                                           (D.FILTER_TENT, 1.0), (D.FILTER_GAUSSIAN, 3.2), (D.FILTER_GAUSSIAN, 0.4)])
 def test_tiled_resolve_equals_plain(ctx, filt, radius):
