@@ -833,21 +833,26 @@ constexpr uint32_t kSortKeys = 32;  // material keys of the block sort (last two
 #ifndef HK_SHADE_MIN_WAVES
 #define HK_SHADE_MIN_WAVES 4
 #endif
-// LDS_MATS: the scene's material records (at most kShadeMats) are copied to
-// LDS once per block; the BSDF code reads them there at each call instead of
-// from global memory.
-constexpr int kShadeMats = 16;
-template <bool LDS_MATS>
+// LDS_TABLES: the scene's material records (at most kShadeMats) and mesh
+// records (at most kShadeMeshes) are copied to LDS once per block; the BSDF
+// code and the hit's emission read them there instead of from global memory.
+constexpr int kShadeMats = 16, kShadeMeshes = 32;
+template <bool LDS_TABLES>
 __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs A, int qi) {
   const DevScene &sc = A.sc;
-  __shared__ DevMaterial s_mats[LDS_MATS ? kShadeMats : 1];
+  __shared__ DevMaterial s_mats[LDS_TABLES ? kShadeMats : 1];
+  __shared__ DevMesh s_meshes[LDS_TABLES ? kShadeMeshes : 1];
   const DevMaterial *mats = sc.mats;
-  if constexpr (LDS_MATS) {
-    constexpr int kVec = int(sizeof(DevMaterial) / sizeof(float4));
-    for (int k = (int)threadIdx.x; k < sc.n_mats * kVec; k += kBlock)
+  const DevMesh *meshes = sc.meshes;
+  if constexpr (LDS_TABLES) {
+    constexpr int kVecM = int(sizeof(DevMaterial) / sizeof(float4)), kVecG = int(sizeof(DevMesh) / sizeof(float4));
+    for (int k = (int)threadIdx.x; k < sc.n_mats * kVecM; k += kBlock)
       reinterpret_cast<float4 *>(s_mats)[k] = reinterpret_cast<const float4 *>(sc.mats)[k];
+    for (int k = (int)threadIdx.x; k < sc.n_meshes * kVecG; k += kBlock)
+      reinterpret_cast<float4 *>(s_meshes)[k] = reinterpret_cast<const float4 *>(sc.meshes)[k];
     __syncthreads();
     mats = s_mats;
+    meshes = s_meshes;
   }
   uint32_t count = A.counters[ctr(qi)];
   if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[ctr(4)] = 0u;  // k_extend's fetch counter
@@ -926,7 +931,7 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
         Frame fr;
         uint32_t mesh_id;
         post_intersect(sc, prim, h4.y, h4.z, P, fr, mesh_id);
-        const DevMesh &mesh = sc.meshes[mesh_id];
+        const DevMesh &mesh = meshes[mesh_id];
         int mat = (int)mesh.material;
         bool diffuse_mat = (mats[mat].flags & kMatFlagDiffuse) != 0;
         vec3 le = v3(0.0f);
@@ -1453,7 +1458,7 @@ hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t
   return hipGetLastError();
 }
 hipError_t launch_shade(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st) {
-  if (A.sc.n_mats <= kShadeMats)
+  if (A.sc.n_mats <= kShadeMats && A.sc.n_meshes <= kShadeMeshes)
     hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kBlock), 0, st, A, qi);
   else
     hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kBlock), 0, st, A, qi);
