@@ -65,6 +65,7 @@ def parse():
     ap.add_argument("--budget-gb", type=float, default=96.0, help="serve: HBM budget of the server")
     ap.add_argument("--schedule", type=int, default=1, help="serve / batch: 1 fewest samples left first "
                     "(library default), 0 first come first served")
+    ap.add_argument("--detail", default="", help="batch / serve: write per-request timings (JSON) to this file")
     return ap.parse_args()
 
 
@@ -106,13 +107,19 @@ def bench_batch(a):
     imgs_b, st = gs.render_batch(reqs)
     t_batch = time.perf_counter() - t0
     print(f"# batched: {t_batch:.2f} s", file=sys.stderr, flush=True)
-    lat_plain, same = [], 0
+    lat_plain, same, rows = [], 0, []
     t0 = time.perf_counter()
     for r, ib in zip(reqs, imgs_b):
-        img, _ = gs.render(0, request=r)
+        t1 = time.perf_counter()
+        img, pst = gs.render(0, request=r)
         lat_plain.append(time.perf_counter() - t0)
+        rows.append({"pixels": r.width * r.height, "spp": r.spp, "alone_ms": (time.perf_counter() - t1) * 1e3,
+                     "gpu_ms": pst["gpu_ms"], "bounces": pst["bounces_launched"]})
         same += int(np.array_equal(img.view(np.uint32), ib.view(np.uint32)))
     t_plain = time.perf_counter() - t0
+    if a.detail:
+        with open(a.detail, "w") as fh:
+            json.dump(rows, fh)
     print(f"# one at a time: {t_plain:.2f} s, identical images {same}/{len(reqs)}", file=sys.stderr, flush=True)
     out = {
         "metric": "path samples/sec at matched relMSE vs plain GPU path; p50/p99 render latency",
@@ -174,13 +181,17 @@ def bench_serve(a):
 
     th = threading.Thread(target=feeder)
     th.start()
-    lat, t_last = [], 0.0
+    lat, t_last, rows = [], 0.0, []
     for i in range(len(trace)):
         while tickets[i] is None:
             time.sleep(0.001)
         _, res = srv.wait(tickets[i])
         lat.append(res["latency_ms"] * 1e-3)
         t_last = max(t_last, time.perf_counter() - t_start)
+        arr, sid, r = trace[i]
+        rows.append({"scene": sid, "pixels": r.width * r.height, "spp": r.spp, "arrival_s": arr,
+                     "alone_ms": service[i] * 1e3, "server_ms": res["latency_ms"], "queue_ms": res["queue_ms"],
+                     "submit_ms": res["submit_ms"], "plain_fcfs_ms": lat_plain[i] * 1e3})
     th.join()
     span = t_last - trace[0][0]
     stats = srv.stats()
@@ -197,8 +208,12 @@ def bench_serve(a):
         "latency_ms": {"p50": _pct(lat, 50) * 1e3, "p99": _pct(lat, 99) * 1e3},
         "plain_fcfs": {"value": total / span_plain, "latency_p50_ms": _pct(lat_plain, 50) * 1e3,
                        "latency_p99_ms": _pct(lat_plain, 99) * 1e3, "busy_s": sum(service)},
-        "server": {"bounces": stats["bounces"], "fixed_gb": stats["fixed_bytes"] / 2**30},
+        "server": {"bounces": stats["bounces"], "shared_bounces": stats["shared_bounces"],
+                   "max_in_flight": stats["max_in_flight"], "fixed_gb": stats["fixed_bytes"] / 2**30},
     }
+    if a.detail:
+        with open(a.detail, "w") as fh:
+            json.dump(rows, fh)
     print(json.dumps(out), flush=True)
 
 

@@ -61,12 +61,14 @@ class RenderStats(C.Structure):
 
 
 class ServerResult(C.Structure):
-    _fields_ = [("status", C.c_int32), ("queue_ms", C.c_double), ("latency_ms", C.c_double), ("paths", C.c_uint64)]
+    _fields_ = [("status", C.c_int32), ("queue_ms", C.c_double), ("latency_ms", C.c_double), ("paths", C.c_uint64),
+                ("submit_ms", C.c_double)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("bounces", C.c_uint64), ("paths", C.c_uint64),
-                ("fixed_bytes", C.c_uint64), ("film_bytes", C.c_uint64)]
+                ("fixed_bytes", C.c_uint64), ("film_bytes", C.c_uint64), ("shared_bounces", C.c_uint64),
+                ("max_in_flight", C.c_uint64)]
 
 
 _lib = None

@@ -145,59 +145,6 @@ struct Builder {
   }
 };
 
-// Quantize each node's child boxes against the node box (lower corner o,
-// step 2^e >= extent / 254 per axis).  The device decodes fmaf(q, step, o);
-// q is moved outward until that exact FP32 value bounds the child box.
-void quantize_nodes(const std::vector<Bvh4Node> &in, std::vector<Bvh4QNode> &out) {
-  out.resize(in.size());
-  for (size_t i = 0; i < in.size(); i++) {
-    const Bvh4Node &n = in[i];
-    Bvh4QNode &q = out[i];
-    std::memset(&q, 0, sizeof q);
-    const float *lo[3] = {n.lo_x, n.lo_y, n.lo_z}, *hi[3] = {n.hi_x, n.hi_y, n.hi_z};
-    float o[3], step[3];
-    uint32_t exps = 0;
-    for (int a = 0; a < 3; a++) {
-      float mn = INFINITY, mx = -INFINITY;
-      for (int c = 0; c < 4; c++)
-        if (n.child[c] != kEmptyRef) { mn = std::min(mn, lo[a][c]); mx = std::max(mx, hi[a][c]); }
-      o[a] = mn;
-      double ext = double(mx) - double(mn);
-      int e = 1;  // smallest normal step for degenerate axes
-      if (ext > 0) {
-        int ex;
-        std::frexp(ext / 254.0, &ex);  // ext/254 <= 2^ex
-        e = std::max(1, std::min(254, ex + 127));
-      }
-      exps |= uint32_t(e) << (8 * a);
-      uint32_t bits = uint32_t(e) << 23;
-      std::memcpy(&step[a], &bits, 4);
-    }
-    q.ox = o[0]; q.oy = o[1]; q.oz = o[2];
-    q.exps = exps;
-    for (int c = 0; c < 4; c++) {
-      q.child[c] = n.child[c];
-      for (int a = 0; a < 3; a++) {
-        int ql, qh;
-        if (n.child[c] == kEmptyRef) {
-          ql = 255; qh = 0;
-        } else {
-          ql = int(std::floor((double(lo[a][c]) - o[a]) / step[a]));
-          ql = std::max(0, std::min(255, ql));
-          while (ql > 0 && std::fma(float(ql), step[a], o[a]) > lo[a][c]) ql--;
-          qh = int(std::ceil((double(hi[a][c]) - o[a]) / step[a]));
-          qh = std::max(0, std::min(255, qh));
-          while (qh < 255 && std::fma(float(qh), step[a], o[a]) < hi[a][c]) qh++;
-          if (std::fma(float(ql), step[a], o[a]) > lo[a][c] || std::fma(float(qh), step[a], o[a]) < hi[a][c])
-            throw std::runtime_error("BVH node quantization failed");
-        }
-        q.q[8 * a + c] = uint8_t(ql);
-        q.q[8 * a + 4 + c] = uint8_t(qh);
-      }
-    }
-  }
-}
-
 }  // namespace
 
 void build_bvh4(const float *tri9, uint32_t nt, Bvh4Result &out) {
@@ -321,7 +268,6 @@ void build_bvh4(const float *tri9, uint32_t nt, Bvh4Result &out) {
   }
   out.max_depth = max_depth;
   out.n_leaves = leaves;
-  quantize_nodes(out.nodes, out.qnodes);
 }
 
 }  // namespace hk

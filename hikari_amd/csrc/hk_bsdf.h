@@ -201,7 +201,9 @@ HK_HD vec3 mf_beckmann_sample(float sx, float sy, float alpha) {  // Sampling.cp
   if (isnan(ln)) ln = 1.0f;
   float t2 = -alpha * alpha * ln;
   float c = 1.0f / sqrtf(1.0f + t2);
-  float st = sqrtf(t2) * c;  // sqrt(1 - c^2) without the cancellation
+  // sqrt(1 - c^2) without the cancellation; 1 where t2 is infinite (sx = 1),
+  // as the reference's expression gives
+  float st = isinf(t2) ? 1.0f : sqrtf(t2) * c;
   float phi = 2.0f * kPiF * sy;
   return v3(st * cosf(phi), st * sinf(phi), c);
 }

@@ -9,8 +9,7 @@
 namespace hk {
 
 struct Bvh4Result {
-  std::vector<Bvh4Node> nodes;       // full-precision boxes (build output)
-  std::vector<Bvh4QNode> qnodes;     // quantized boxes (what the kernels walk)
+  std::vector<Bvh4Node> nodes;       // FP32 boxes, what the kernels walk
   std::vector<TraceTri> tris;
   uint32_t max_depth = 0, n_leaves = 0;
 };

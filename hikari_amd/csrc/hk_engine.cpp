@@ -185,16 +185,18 @@ void Engine::submit(RequestRec *r) {
   if (r->n_chunks > 0 && uint64_t(r->npix) * r->chunk > cap_samples_)
     throw HkError(HK_ERR_INVALID, "a chunk of %u x %u samples exceeds the sample buffer (%u)", r->npix, r->chunk,
                   cap_samples_);
-  r->slot = free_reqs_.back();
-  free_reqs_.pop_back();
-  reqs_by_slot_[r->slot] = r;
+  // allocations first: a request that cannot get its memory leaves the
+  // engine as it was
   if (!r->keep_samples) {
     r->accum.alloc(size_t(r->npix) * 16);
-    hip_check(hipMemsetAsync(r->accum.p, 0, size_t(r->npix) * 16, st_), "accum clear");
     r->dev.accum = r->accum.as<float>();
   } else {
     r->samples.assign(size_t(r->spp) * r->npix * 4, 0.0f);
   }
+  r->slot = free_reqs_.back();
+  free_reqs_.pop_back();
+  reqs_by_slot_[r->slot] = r;
+  if (!r->keep_samples) hip_check(hipMemsetAsync(r->accum.p, 0, size_t(r->npix) * 16, st_), "accum clear");
   DevRequest *hr = ws_.h_reqs.as<DevRequest>() + r->slot;
   *hr = r->dev;
   hip_check(hipMemcpyAsync(ws_.reqs.as<DevRequest>() + r->slot, hr, sizeof(DevRequest), hipMemcpyHostToDevice, st_),

@@ -67,8 +67,14 @@ class Engine {
          uint32_t chunk_cap, bool profile, hipEvent_t ev0, hipEvent_t ev1);
   ~Engine();
 
-  // Hands a prepared request to the scheduler (FIFO admission).
+  // Hands a prepared request to the scheduler (FIFO admission).  Throws
+  // without changing the engine when the request cannot be taken (no free
+  // request slot, a chunk larger than the sample buffer, no memory for its
+  // film).
   void submit(RequestRec *r);
+  // Request slots still free (submit needs one per request in flight).
+  uint32_t free_request_slots() const { return uint32_t(free_reqs_.size()); }
+  uint32_t requests_in_flight() const { return kMaxRequests - uint32_t(free_reqs_.size()); }
   bool busy() const { return !pending_.empty() || !active_.empty() || in_flight_ > 0; }
   // One bounce: admit, start new samples, extend, shade, shadow, collect
   // finished chunks, resolve them, finalize finished requests.  Returns the

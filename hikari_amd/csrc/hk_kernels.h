@@ -83,10 +83,9 @@ hipError_t launch_shade_tris(const float *V, const float *N, const float *UV, co
                              const uint32_t *prim_mesh, const DevMesh *meshes, ShadeTri *out, uint32_t nt,
                              hipStream_t st);
 
-hipError_t launch_intersect(const TraceNode *nodes, const TraceTri *tris, int32_t root, const void *rays, void *hits,
-                            uint32_t n, int any, hipStream_t st);
-// The same queries on persistent waves (counter: one device word, zeroed by
-// the launch; refill / batch as the render's "refill" / "fetch_batch").
+// Closest / any hits of n rays (hk_ray / hk_hit layout) on persistent waves
+// (counter: one device word, zeroed by the launch; refill / batch as the
+// render's "refill" / "fetch_batch").
 hipError_t launch_intersect_persistent(const TraceNode *nodes, const TraceTri *tris, int32_t root, const void *rays,
                                        void *hits, uint32_t n, int any, uint32_t *counter, uint32_t refill,
                                        uint32_t batch, hipStream_t st);

@@ -71,8 +71,6 @@ struct TraceStack {
   }
 };
 
-__device__ __forceinline__ float ubyte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
-
 struct TraceCount {
   uint32_t nodes = 0, tris = 0;
 };
@@ -81,8 +79,7 @@ struct TraceCount {
 // nodes are processed by node_step, leaves by leaf_step; a lane is done when
 // ref == kDone.  Slab tests are widened by a few ulp (tnear <= tfar *
 // (1 + 2^-21)), which covers the rounding of (lo - o) * (1/d); the host pads
-// triangle boxes for the inexactness of the triangle test and rounds
-// quantized boxes outward.
+// triangle boxes for the inexactness of the triangle test.
 constexpr float kWiden = 1.0000005f;
 struct RayTrav {
   vec3 o, d;
@@ -124,33 +121,12 @@ __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, R
   float tlim = R.bt * kWiden;
   float tn[4];
   int32_t r[4];
-#ifndef HK_NODE_QUANTIZED
   const float4 *np = reinterpret_cast<const float4 *>(nodes + R.ref);
   float4 lx4 = np[0], hx4 = np[1], ly4 = np[2], hy4 = np[3], lz4 = np[4], hz4 = np[5];
   int4 ch = *reinterpret_cast<const int4 *>(np + 6);
   float lxs[4] = {lx4.x, lx4.y, lx4.z, lx4.w}, hxs[4] = {hx4.x, hx4.y, hx4.z, hx4.w};
   float lys[4] = {ly4.x, ly4.y, ly4.z, ly4.w}, hys[4] = {hy4.x, hy4.y, hy4.z, hy4.w};
   float lzs[4] = {lz4.x, lz4.y, lz4.z, lz4.w}, hzs[4] = {hz4.x, hz4.y, hz4.z, hz4.w};
-#else
-  const Bvh4QNode *np = nodes + R.ref;
-  float4 h = *reinterpret_cast<const float4 *>(np);
-  int4 ch = *reinterpret_cast<const int4 *>(&np->child[0]);
-  uint4 qa = *reinterpret_cast<const uint4 *>(&np->q[0]);
-  uint2 qb = *reinterpret_cast<const uint2 *>(&np->q[16]);
-  uint32_t ex = __float_as_uint(h.w);
-  float sx = __uint_as_float((ex & 0xffu) << 23), sy = __uint_as_float(((ex >> 8) & 0xffu) << 23);
-  float sz = __uint_as_float(((ex >> 16) & 0xffu) << 23);
-  float lxs[4], hxs[4], lys[4], hys[4], lzs[4], hzs[4];
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    lxs[c] = fmaf(ubyte_f(qa.x, c), sx, h.x);
-    hxs[c] = fmaf(ubyte_f(qa.y, c), sx, h.x);
-    lys[c] = fmaf(ubyte_f(qa.z, c), sy, h.y);
-    hys[c] = fmaf(ubyte_f(qa.w, c), sy, h.y);
-    lzs[c] = fmaf(ubyte_f(qb.x, c), sz, h.z);
-    hzs[c] = fmaf(ubyte_f(qb.y, c), sz, h.z);
-  }
-#endif
   r[0] = ch.x; r[1] = ch.y; r[2] = ch.z; r[3] = ch.w;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
@@ -174,37 +150,9 @@ __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, R
   R.ref = tn[0] != kInfF ? r[0] : R.st.pop(R.bt * kWiden);
 }
 
-// Leaf R.ref: exact FP32 Moller-Trumbore on its triangles, then the next
-// stack entry.  Closest hit: among equal t the largest prim id wins (the
-// order of the reference's brute-force loop); any hit: done at the first.
-template <bool ANY>
-__device__ __forceinline__ void leaf_step(const TraceTri *__restrict__ tris, RayTrav &R, TraceCount &cnt) {
-  uint32_t code = uint32_t(~R.ref);
-  uint32_t first = code >> 3, ntri = (code & 7u) + 1u;
-#ifdef HK_TRAVERSAL_STATS
-  cnt.tris += ntri;
-#endif
-  for (uint32_t k = 0; k < ntri; k++) {
-    const float4 *tp = reinterpret_cast<const float4 *>(tris + first + k);
-    float4 a = tp[0], b = tp[1], c = tp[2];
-    float u, v, t;
-    if (mt_test(R.o, R.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(b.w, c.x, c.y), R.tmin, R.bt, u, v, t)) {
-      int32_t prim = __float_as_int(a.w);
-      if (ANY) {
-        R.bt = t; R.bu = u; R.bv = v; R.bprim = prim;
-        R.ref = kDone;
-        return;
-      }
-      if (t < R.bt || prim > R.bprim) {
-        R.bt = t; R.bu = u; R.bv = v; R.bprim = prim;
-      }
-    }
-  }
-  R.ref = R.st.pop(R.bt * kWiden);
-}
-
-// The triangles of leaf `leaf` (no stack pop; any hit: ref = kDone at the
-// first hit).
+// The triangles of leaf `leaf`: exact FP32 Moller-Trumbore (no stack pop).
+// Closest hit: among equal t the largest prim id wins (the order of the
+// reference's brute-force loop); any hit: ref = kDone at the first hit.
 template <bool ANY>
 __device__ __forceinline__ void leaf_tris(const TraceTri *__restrict__ tris, RayTrav &R, int32_t leaf,
                                           TraceCount &cnt) {
@@ -245,10 +193,6 @@ constexpr int kLeafEarly = 8;
 template <bool ANY>
 __device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, const TraceTri *__restrict__ tris,
                                            RayTrav &R, int32_t &pend, TraceCount &cnt) {
-#ifdef HK_NO_SPECULATION
-  while (R.ref >= 0) node_step(nodes, R, cnt);
-  if (R.ref != kDone) leaf_step<ANY>(tris, R, cnt);
-#else
   int32_t pend2 = kDone;  // closest-hit rays park a second leaf rather than wait
   for (;;) {
     if (R.ref < 0 && R.ref != kDone) {
@@ -275,49 +219,6 @@ __device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, 
   if (!ANY && pend2 != kDone) {
     leaf_tris<ANY>(tris, R, pend2, cnt);
     pend2 = kDone;
-  }
-#endif
-}
-
-// Closest (ANY = false) or any (ANY = true) hit of one ray.  The inner loop
-// descends internal nodes until the lane holds a leaf (or is done), so leaf
-// tests run with the wave converged ("while-while").
-template <bool ANY>
-__device__ __forceinline__ bool trace_ray(const TraceNode *__restrict__ nodes, const TraceTri *__restrict__ tris,
-                                          int32_t root, uint64_t *lds_stack, vec3 o, vec3 d, float tmin, float tmax,
-                                          float &bt, float &bu, float &bv, int32_t &bprim, TraceCount &cnt) {
-  RayTrav R;
-  uint64_t priv[kPrivStack];
-  trav_begin(R, lds_stack, priv, root, o, d, tmin, tmax);
-  for (;;) {
-    while (R.ref >= 0) node_step(nodes, R, cnt);
-    if (R.ref == kDone) break;
-    leaf_step<ANY>(tris, R, cnt);
-  }
-  bt = R.bt;
-  bu = R.bu;
-  bv = R.bv;
-  bprim = R.bprim;
-  return R.bprim >= 0;
-}
-
-// ------------------------------------------------------------------ batched rayIntersect
-__global__ __launch_bounds__(kBlock) void k_intersect(const TraceNode *__restrict__ nodes,
-                                                      const TraceTri *__restrict__ tris, int32_t root,
-                                                      const float4 *__restrict__ rays, float4 *__restrict__ hits,
-                                                      uint32_t n, int any) {
-  __shared__ uint64_t stack[kLdsStack * kBlock];
-  uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    float4 a = rays[2 * i], b = rays[2 * i + 1];
-    float t, u, v;
-    int32_t prim;
-    TraceCount tc;
-    bool h = any ? trace_ray<true>(nodes, tris, root, stack, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, t, u,
-                                   v, prim, tc)
-                 : trace_ray<false>(nodes, tris, root, stack, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, t,
-                                    u, v, prim, tc);
-    hits[i] = h ? make_float4(t, u, v, __int_as_float(prim)) : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
   }
 }
 
@@ -698,11 +599,9 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi) {
 // entries from its reserved range (wave_fetch: A.fetch_batch x 64 entries per
 // atomic on counters[4], cleared by k_regen / k_shade before), so a few long
 // rays no longer hold a whole wave.  Both are run-time options; 24 idle lanes
-// and 2 x 64 entries are fastest on the benchmark scenes.  HK_PLAIN_TRACE
-// builds trace one ray per lane instead.
+// and 2 x 64 entries are fastest on the benchmark scenes.
 constexpr uint32_t kNoRay = 0xffffffffu;
 
-#ifndef HK_PLAIN_TRACE
 __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
   uint32_t count = A.counters[ctr(qi)];
@@ -749,40 +648,13 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
   count_flush(A.stats + 2, tc);
 #endif
 }
-#else
-__global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
-  __shared__ uint64_t stack[kLdsStack * kBlock];
-  uint32_t count = A.counters[ctr(qi)];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    A.counters[ctr(qi ^ 1)] = 0u;
-    A.counters[ctr(2)] = 0u;
-    A.counters[ctr(5)] = 0u;  // k_shadow's fetch counter (persistent build)
-    atomicAdd(&A.stats[0], (unsigned long long)count);
-  }
-  uint32_t stride = gridDim.x * blockDim.x;
-  const float4 *qo = A.q_o[qi], *qd = A.q_d[qi];
-  float4 *qh = A.hit;
-  TraceCount tc;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-    float4 a = qo[i], b = qd[i];
-    float t, u, v;
-    int32_t prim;
-    bool h = trace_ray<false>(A.sc.nodes, A.sc.tris, A.sc.root_ref, stack, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z),
-                              a.w, b.w, t, u, v, prim, tc);
-    qh[i] = h ? make_float4(t, u, v, __int_as_float(prim)) : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
-  }
-#ifdef HK_TRAVERSAL_STATS
-  count_flush(A.stats + 2, tc);
-#endif
-}
-#endif
 
-// ------------------------------------------------------------------ batched rayIntersect, persistent
+// ------------------------------------------------------------------ batched rayIntersect
 // Acceleration::RayIntersect over an array of rays with k_extend's machinery:
 // persistent waves that refill from reserved ranges of the ray array
 // (counter: one zeroed word) and speculative traversal rounds.  Closest hits
-// equal k_intersect's bit for bit (the closest hit does not depend on the
-// visit order); any-hit queries agree on whether a hit exists.
+// equal brute force bit for bit (the closest hit does not depend on the visit
+// order); any-hit queries agree on whether a hit exists.
 template <bool ANY>
 __global__ __launch_bounds__(kBlock) void k_intersect_persistent(const TraceNode *__restrict__ nodes,
                                                                  const TraceTri *__restrict__ tris, int32_t root,
@@ -858,17 +730,14 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
   if (blockIdx.x == 0 && threadIdx.x == 0) A.counters[ctr(4)] = 0u;  // k_extend's fetch counter
   int qn = qi ^ 1;
   uint32_t stride = gridDim.x * blockDim.x;
-#ifndef HK_NO_MATERIAL_SORT
   // Block-local material sort: the block's 256 entries are counting-sorted by
   // the material they hit (misses and idle lanes last), and lane t shades
   // the entry of rank t, so waves see one material where they can.  Nothing
   // moves in memory; a path's result does not depend on its lane.
   __shared__ uint32_t s_cnt[kSortKeys];
   __shared__ uint32_t s_perm[kBlock];
-#endif
   __shared__ BlockAppend s_app;
   for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
-#ifndef HK_NO_MATERIAL_SORT
     uint32_t key = kSortKeys - 1;
     if (base + threadIdx.x < count) {
       int32_t hp = __float_as_int(A.hit[base + threadIdx.x].w);
@@ -890,9 +759,6 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
     s_perm[s_cnt[key] + rank] = threadIdx.x;
     __syncthreads();
     uint32_t i = base + s_perm[threadIdx.x];
-#else
-    uint32_t i = base + threadIdx.x;
-#endif
     bool active = i < count;
     bool cont = false;
     uint32_t smask = 0, pix = 0, local = 0, islot = 0, meta_out = 0;
@@ -1072,9 +938,7 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
 // entry's emitter rays in emitter order and adds the unoccluded
 // contributions to the path's radiance, so no atomics are needed and the
 // additions happen in the reference's order.  Lanes refill as in k_extend
-// (fetch counter counters[5], cleared by k_extend); HK_PLAIN_TRACE builds
-// give each lane one entry.
-#ifndef HK_PLAIN_TRACE
+// (fetch counter counters[5], cleared by k_extend).
 __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
   uint32_t count = A.counters[ctr(2)];
@@ -1149,53 +1013,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
 #endif
 }
 
-#else
-__global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
-  __shared__ uint64_t stack[kLdsStack * kBlock];
-  uint32_t count = A.counters[ctr(2)];
-  uint32_t stride = gridDim.x * blockDim.x;
-  uint32_t traced = 0;
-  TraceCount tc;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
-    uint32_t i = base + threadIdx.x;
-    bool last = false;
-    uint32_t islot = 0;
-    if (i < count) {
-      uint4 ent = A.s_ent[i];
-      uint32_t src = ent.x, mask = ent.z;
-      last = (ent.w & kShadowLast) != 0u;
-      islot = ent.w & ~kShadowLast;
-      float4 *lp = last ? &A.sample_L[ent.y] : &A.q_L[qn][ent.y];
-      float4 l4 = *lp;
-      vec3 L = v3(l4.x, l4.y, l4.z);
-      while (mask) {
-        int e = __ffs((int)mask) - 1;
-        mask &= mask - 1u;
-        size_t slot = size_t(e) * A.n_paths + src;
-        float4 o = A.sh_o[slot], d = A.sh_d[slot], c = A.sh_c[slot];
-        float t, u, v;
-        int32_t prim;
-        traced++;
-        bool occ = trace_ray<true>(A.sc.nodes, A.sc.tris, A.sc.root_ref, stack, v3(o.x, o.y, o.z),
-                                   v3(d.x, d.y, d.z), 1e-4f, 1.0f - 1e-4f, t, u, v, prim, tc);
-        if (!occ) L = L + v3(c.x, c.y, c.z);
-      }
-      *lp = make_float4(L.x, L.y, L.z, l4.w);
-    }
-    count_done(A.item_done, last, islot);
-  }
-  // one atomic per wave for the statistics
-  for (int off = 32; off > 0; off >>= 1) traced += __shfl_down(traced, off);
-  if (lane_id() == 0 && traced) atomicAdd(&A.stats[1], (unsigned long long)traced);
-#ifdef HK_TRAVERSAL_STATS
-  count_flush(A.stats + 4, tc);
-#endif
-}
-#endif
-
 // ------------------------------------------------------------------ k_resolve
-// ImageBlock::Put (src/core/Block.cpp:95-134) as a gather: each output pixel
-// owns its accumulator and sums the filtered samples of its neighbours.
 // ImageBlock::Put (src/core/Block.cpp:95-134) as a gather: each output pixel
 // accumulates, in (sample, neighbour row, neighbour column) order, every
 // sample of the chunk whose filter footprint covers it.  A sample's jitter is
@@ -1395,15 +1213,6 @@ static uint32_t grid_for(uint32_t n, uint32_t cap) {
   return g < 1 ? 1 : g;
 }
 
-hipError_t launch_intersect(const TraceNode *nodes, const TraceTri *tris, int32_t root, const void *rays, void *hits,
-                            uint32_t n, int any, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_intersect, dim3(grid_for(n, 1u << 20)), dim3(kBlock), 0, st, nodes, tris, root,
-                     (const float4 *)rays, (float4 *)hits, n, any);
-  return hipGetLastError();
-}
-
-
 hipError_t launch_bsdf_batch(const DevMaterial *mats, const float *rft, int idx, uint32_t n, const float *wi,
                              const float *wo, const int32_t *meas, const float *u, float *out_eval, float *out_pdf,
                              float *out_f, float *out_wo, int32_t *out_meas, float *out_eta, float *out_spdf,
@@ -1432,7 +1241,6 @@ hipError_t launch_intersect_persistent(const TraceNode *nodes, const TraceTri *t
                                        void *hits, uint32_t n, int any, uint32_t *counter, uint32_t refill,
                                        uint32_t batch, hipStream_t st) {
   if (n == 0) return hipSuccess;
-#ifndef HK_PLAIN_TRACE
   // the fetch counter may run past n by up to one reservation per wave
   if (refill < 1 || refill > 64 || batch < 1 || batch > 64 || n > (1u << 31)) return hipErrorInvalidValue;
   hipError_t e = launch_fill_u32(counter, 0u, 1u, st);
@@ -1448,9 +1256,6 @@ hipError_t launch_intersect_persistent(const TraceNode *nodes, const TraceTri *t
                        root, (const float4 *)rays, (float4 *)hits, n, counter, refill, batch);
   }
   return hipGetLastError();
-#else
-  return launch_intersect(nodes, tris, root, rays, hits, n, any, st);
-#endif
 }
 hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st) {
   static uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_extend));

@@ -309,7 +309,11 @@ HK_HD vec3 micro_sample(const Micro &m, float sx, float sy, float &pdf) {
   if (pdf < 1e-20f) pdf = 0.0f;
   // sin = sqrt(1 - cos^2) of the reference, evaluated as tan * cos: the same
   // value without the cancellation that costs FP32 digits near the normal
-  float sin_m = sqrtf(tan2) * cos_m;
+  // (a few ulp from the exact sine of the FP32 angle, where the reference's
+  // FP32 expression loses up to half its digits at cos -> 1).  At sx = 1
+  // tan^2 is infinite
+  // and the reference's expression gives sin = 1, cos = 0; so does this one.
+  float sin_m = isinf(tan2) ? 1.0f : sqrtf(tan2) * cos_m;
   return {sin_m * cos_phi, sin_m * sin_phi, cos_m};
 }
 

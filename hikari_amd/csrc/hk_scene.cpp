@@ -26,15 +26,11 @@ extern thread_local std::string g_create_error;
 
 void upload_bvh(hk_ctx *ctx, const float *tri9, uint32_t nt, DevBvh &out) {
   if (ctx->bvh_builder == 1 && nt >= 2) {
-#ifdef HK_NODE_QUANTIZED
-    throw HkError(HK_ERR_UNSUPPORTED, "the device BVH builder emits FP32 nodes only");
-#else
     build_bvh4_device(ctx->stream, tri9, nt, out);
     if (3 * out.max_depth + 1 > 96)
       throw HkError(HK_ERR_UNSUPPORTED, "device BVH depth %u exceeds the traversal stack (use the host builder)",
                     out.max_depth);
     return;
-#endif
   }
   Bvh4Result r;
   try {
@@ -45,11 +41,7 @@ void upload_bvh(hk_ctx *ctx, const float *tri9, uint32_t nt, DevBvh &out) {
   // the traversal stack holds at most 3 entries per level below the root
   if (3 * r.max_depth + 1 > 96)
     throw HkError(HK_ERR_UNSUPPORTED, "BVH depth %u exceeds the traversal stack", r.max_depth);
-#ifdef HK_NODE_QUANTIZED
-  out.nodes.upload(r.qnodes.data(), r.qnodes.size() * sizeof(Bvh4QNode), ctx->stream);
-#else
   out.nodes.upload(r.nodes.data(), r.nodes.size() * sizeof(Bvh4Node), ctx->stream);
-#endif
   out.tris.upload(r.tris.data(), r.tris.size() * sizeof(TraceTri), ctx->stream);
   out.n_nodes = uint32_t(r.nodes.size());
   out.n_tris = nt;

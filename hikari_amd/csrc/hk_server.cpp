@@ -72,6 +72,7 @@ struct hk_server {
   bool started = false, stop = false;
   std::string fatal;                      // error that stopped the worker
   hk_server_stats st;
+  double t0 = 0;                          // server clock origin (steady clock, ms)
 };
 
 static double clock_ms() {
@@ -87,21 +88,40 @@ static void finish_ticket(hk_server *srv, Ticket &t, int status, const std::stri
   t.admitted = false;
 }
 
+// A failure of a bounce (a HIP error, a lost device) ends the worker: every
+// unfinished ticket fails with it and later submits are refused.
+static void stop_worker(hk_server *srv, int code, const std::string &msg) {
+  std::lock_guard<std::mutex> lk(srv->mu);
+  srv->fatal = msg;
+  for (auto &kv : srv->tickets)
+    if (!kv.second.done) finish_ticket(srv, kv.second, code, msg);
+  srv->arrivals.clear();
+  srv->cv_done.notify_all();
+}
+
 static void worker_main(hk_server *srv) {
   (void)hipSetDevice(srv->ctx->device);
-  std::vector<uint64_t> busy_since;
   for (;;) {
     std::vector<std::pair<uint64_t, Ticket *>> admitted;
     {
       std::unique_lock<std::mutex> lk(srv->mu);
-      // HBM-budget admission, first come first served
-      while (!srv->arrivals.empty()) {
-        uint64_t id = srv->arrivals.front();
+      // HBM-budget admission, first come first served.  A request whose
+      // scene has no free request slot stays queued (later requests for
+      // other scenes may pass it); one whose film does not fit the budget
+      // waits, with everything behind it, until running requests free memory.
+      std::vector<uint32_t> slots(srv->scenes.size());
+      for (size_t k = 0; k < slots.size(); k++) slots[k] = srv->scenes[k]->eng->free_request_slots();
+      for (auto it = srv->arrivals.begin(); it != srv->arrivals.end();) {
+        uint64_t id = *it;
         Ticket &t = srv->tickets[id];
+        if (slots[size_t(t.scene)] == 0) {
+          ++it;
+          continue;
+        }
         uint64_t need = t.rec->film_bytes();
         bool fits = srv->budget == 0 || srv->fixed_bytes + srv->film_bytes + need <= srv->budget;
         if (!fits && srv->film_bytes > 0) break;  // wait for running requests to free memory
-        srv->arrivals.pop_front();
+        it = srv->arrivals.erase(it);
         if (!fits) {
           finish_ticket(srv, t, HK_ERR_BUDGET, "request film exceeds the HBM budget");
           t.rec.reset();
@@ -110,22 +130,48 @@ static void worker_main(hk_server *srv) {
         }
         srv->film_bytes += need;
         t.admitted = true;
+        slots[size_t(t.scene)]--;
         admitted.push_back({id, &t});
       }
       bool any = !admitted.empty();
       for (auto &sc : srv->scenes) any = any || sc->eng->busy();
       if (!any) {
+        // (nothing busy means no request is in flight, so every queued
+        // request would have been admitted or refused above)
         if (srv->stop) return;
         srv->cv_work.wait(lk);
         continue;
       }
     }
-    try {
-      for (auto &a : admitted) {
-        Ticket &t = *a.second;
-        ServerScene &sc = *srv->scenes[size_t(t.scene)];
+    // A request the engine refuses (e.g. no device memory for its film)
+    // fails alone; the engine is left as it was.  Only the worker finishes
+    // tickets, so the accepted ones stay valid until this round ends.
+    std::vector<Ticket *> accepted;
+    for (auto &a : admitted) {
+      Ticket &t = *a.second;
+      ServerScene &sc = *srv->scenes[size_t(t.scene)];
+      int code = HK_OK;
+      std::string msg;
+      try {
         sc.eng->submit(t.rec.get());
+      } catch (const HkError &e) {
+        code = e.code;
+        msg = e.msg;
+      } catch (const std::exception &e) {
+        code = HK_ERR_OOM;
+        msg = e.what();
       }
+      if (code == HK_OK) {
+        accepted.push_back(&t);
+        continue;
+      }
+      std::lock_guard<std::mutex> lk(srv->mu);
+      finish_ticket(srv, t, code, msg);
+      srv->cv_done.notify_all();
+    }
+    try {
+      uint64_t in_flight = 0;
+      for (auto &sc : srv->scenes) in_flight += sc->eng->requests_in_flight();
       std::vector<ServerScene *> launched;
       for (auto &sc : srv->scenes)
         if (sc->eng->busy() && sc->eng->launch()) launched.push_back(sc.get());
@@ -135,11 +181,13 @@ static void worker_main(hk_server *srv) {
         finished.insert(finished.end(), f.begin(), f.end());
       }
       // requests with no samples finish at submit
-      for (auto &a : admitted)
-        if (a.second->rec->done && std::find(finished.begin(), finished.end(), a.second->rec.get()) == finished.end())
-          finished.push_back(a.second->rec.get());
+      for (Ticket *t : accepted)
+        if (t->rec->done && std::find(finished.begin(), finished.end(), t->rec.get()) == finished.end())
+          finished.push_back(t->rec.get());
       std::lock_guard<std::mutex> lk(srv->mu);
       srv->st.bounces += launched.size();
+      if (in_flight >= 2) srv->st.shared_bounces += launched.size();
+      srv->st.max_in_flight = std::max<uint64_t>(srv->st.max_in_flight, in_flight);
       for (RequestRec *r : finished) {
         Ticket &t = srv->tickets[r->id];
         finish_ticket(srv, t, HK_OK, "");
@@ -148,12 +196,10 @@ static void worker_main(hk_server *srv) {
       }
       if (!finished.empty()) srv->cv_done.notify_all();
     } catch (const HkError &e) {
-      std::lock_guard<std::mutex> lk(srv->mu);
-      srv->fatal = e.msg;
-      for (auto &kv : srv->tickets)
-        if (!kv.second.done) finish_ticket(srv, kv.second, e.code, e.msg);
-      srv->arrivals.clear();
-      srv->cv_done.notify_all();
+      stop_worker(srv, e.code, e.msg);
+      return;
+    } catch (const std::exception &e) {
+      stop_worker(srv, HK_ERR_HIP, e.what());
       return;
     }
   }
@@ -182,6 +228,7 @@ int hk_server_create(hk_ctx *ctx, uint64_t hbm_budget, hk_server **out) {
   srv->ctx = ctx;
   srv->budget = hbm_budget;
   std::memset(&srv->st, 0, sizeof srv->st);
+  srv->t0 = clock_ms();
   *out = srv.release();
   return HK_OK;
   HK_API_END(ctx)
@@ -260,6 +307,7 @@ int hk_server_wait(hk_server *srv, uint64_t ticket, hk_server_result *res) {
     if (t.rec) {
       res->queue_ms = t.rec->t_admit >= 0 ? t.rec->t_admit - t.rec->t_submit : 0.0;
       res->latency_ms = t.rec->t_done - t.rec->t_submit;
+      res->submit_ms = t.rec->t_submit - srv->t0;
       res->paths = uint64_t(t.rec->npix) * t.rec->spp;
     }
   }

@@ -59,27 +59,8 @@ struct alignas(64) Bvh4Node {
 };
 static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node layout");
 
-// Quantized BVH4 node (64 B, one cache line): the node box's lower corner and
-// per-axis power-of-two steps; child boxes as 8-bit multiples of the step,
-// rounded outward on the host so that the decoded box, fmaf(q, step, lo),
-// contains the exact child box.  Same child encoding as Bvh4Node.
-struct alignas(64) Bvh4QNode {
-  float ox, oy, oz;
-  uint32_t exps;        // byte a: biased FP32 exponent of the step on axis a
-  int32_t child[4];
-  uint8_t q[24];        // lo_x[4] hi_x[4] lo_y[4] hi_y[4] lo_z[4] hi_z[4]
-  uint32_t pad[2];
-};
-static_assert(sizeof(Bvh4QNode) == 64, "Bvh4QNode layout");
-
-// Node format the kernels walk: 128-B FP32 nodes, or the 64-B quantized
-// nodes in builds with HK_NODE_QUANTIZED (half the memory, ~9% slower on the
-// 1.3M-triangle sphere: the decode costs more VALU time than the bytes save).
-#ifdef HK_NODE_QUANTIZED
-using TraceNode = Bvh4QNode;
-#else
+// Node format the kernels walk: the 128-B FP32 node.
 using TraceNode = Bvh4Node;
-#endif
 
 // Leaf-ordered triangle for traversal (48 B): p0 + prim id, e1 = p1 - p0,
 // e2 = p2 - p0 (edges computed in FP32 on the host exactly as the reference

@@ -233,10 +233,13 @@ typedef struct hk_server_result {
   double queue_ms;         /* submit -> first samples admitted */
   double latency_ms;       /* submit -> film in host memory */
   uint64_t paths;          /* path samples of the request */
+  double submit_ms;        /* submit time on the server clock (ms since hk_server_create) */
 } hk_server_result;
 typedef struct hk_server_stats {
   uint64_t submitted, completed, bounces, paths;
   uint64_t fixed_bytes, film_bytes;   /* HBM held: workspaces, admitted films */
+  uint64_t shared_bounces;            /* bounces launched while two or more requests were in flight */
+  uint64_t max_in_flight;             /* most requests in flight at once */
 } hk_server_stats;
 
 int hk_server_create(hk_ctx *ctx, uint64_t hbm_budget, hk_server **out);

@@ -326,6 +326,29 @@ def test_server_hbm_budget_admission():
     c.close()
 
 
+def test_server_more_requests_than_slots():
+    """4097 tiny requests (8 x 8, 1 spp) on one scene with no HBM budget: one
+    more than an engine holds at once.  The request beyond the engine's slots
+    waits in the queue instead of failing the server, and every request
+    completes with the film it has alone."""
+    box = scenes.cbox(width=8, height=8, sphere_subdiv=1)
+    c = hk.Context(0)
+    c.set_option("max_paths", 20000)
+    g = c.scene(box)
+    srv = hk.Server(c)
+    srv.add_scene(g)
+    reqs = [hk.make_request(box, spp=1, seed=3000 + i) for i in range(4097)]
+    tickets = [srv.submit(0, r) for r in reqs]
+    imgs = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    assert st["completed"] == 4097 and st["film_bytes"] == 0
+    assert all(res["status"] == 0 for _, res in imgs)
+    for k in (0, 1, 2048, 4095, 4096):
+        assert np.array_equal(imgs[k][0].view(np.uint32), g.render(0, request=reqs[k])[0].view(np.uint32)), k
+    srv.close()
+    c.close()
+
+
 @pytest.mark.parametrize("env_h", [16, 32])
 def test_environment_sampling_closed_form_gpu(ctx, env_h):
     """The GPU environment sampler + estimator on the constant-sky plane
