@@ -11,23 +11,38 @@
 // Random numbers are always produced as pcg32 floats (as in the reference) and
 // promoted to R.
 //
-// Pinning (what checks each part; tests/ hold the checks):
+// Pinning (what checks each part; tests/test_oracle.py and
+// tests/test_oracle_pins.py hold the checks, and each check of the second file
+// is shown to fail with a planted mistake, enum Slip below):
 //   pcg32               ext/pcg32/pcg32-demo.out known-answer text, plus the
 //                       reference's own pcg32-demo built under oracle/_ref.
 //   vector ops / M-T    bit-exact against Eigen 3.3.5 from the reference tree
 //                       (oracle/_ref/eigen_ops), closed-form ray/triangle cases,
 //                       BVH == brute force on random rays.
-//   Fresnel             closed forms (normal incidence, TIR, k=0 conductor).
-//   microfacet D / G1   normalisation by quadrature, known G1 limits.
-//   BSDF sampling       chi^2 of sampled directions against Pdf (the
-//                       reference's Chi2Test procedure), Sample == Eval/Pdf.
+//   Fresnel             closed forms (normal incidence, TIR, k=0 conductor);
+//                       the conductor against the complex-index equations.
+//   microfacet D / G1   normalisation by quadrature; the Smith identity
+//                       int G1 <v.m> D dm = cos(theta_v) (GGX exact, Beckmann
+//                       to its rational fit).
+//   BSDFs               chi^2 of sampled directions against Pdf (the
+//                       reference's Chi2Test procedure), Sample == Eval/Pdf,
+//                       and per class a closed form or limit: rough conductor
+//                       -> ks F (alpha -> 0) and albedo <= 1 (F = 1); rough
+//                       dielectric -> F / (1-F) eta^-2; plastic and rough
+//                       plastic -> ks F (kd = 0) and their diffuse albedo;
+//                       coating of index 1 == its nested BSDF, absorption
+//                       exp(-sigma d (1/cos_i + 1/cos_o)), albedo in closed
+//                       form; rough coating -> coating (alpha -> 0).
 //   RFT tables          knot values reproduce table entries; T(eta=4, a=0,
-//                       cos=1) = 1 - ((4-1)/(4+1))^2.
+//                       cos=1) = 1 - ((4-1)/(4+1))^2; T and EvalDiffuse at
+//                       alpha = 0 against 1 - F and 1 - the cosine-weighted
+//                       Fresnel average.
+//   film                one sample's splat against the tabulated Gaussian
+//                       with half-pixel centres.
 //   estimator           furnace 1/(1-a) (scene/pa5/tests/test-furnace.xml) and
-//                       direct-lighting references (test-direct.xml).
-// Parity unpinned: none of the above pins the env-map sampler against a
-// reference output (the reference needs OpenEXR to build it); it is checked
-// against its own closed-form pdf normalisation only.
+//                       direct-lighting references (test-direct.xml); the
+//                       environment sampler against the closed-form
+//                       expectation of a plane under a constant sky.
 #pragma once
 
 #include <algorithm>
@@ -40,6 +55,31 @@
 #include <vector>
 
 namespace hko {
+
+// Planted mistakes.  The pinning tests show that each of their checks fails
+// when the restatement carries a plausible slip: a test selects one through
+// ho_set_slip, runs the check, expects it to fail, and selects kSlipNone
+// again.  Every other use runs with kSlipNone.
+enum Slip {
+  kSlipNone = 0,
+  kSlipGgxG1,              // GGX G1 with (alpha tan)^2 where alpha tan belongs
+  kSlipBeckmannG1,         // Beckmann G1 fit without its a^2 numerator term
+  kSlipConductorFresnel,   // conductor Fresnel with cos_i in place of 2 cos_i
+  kSlipRoughConductorG,    // rough conductor Sample() without the shadowing term
+  kSlipRoughDielectricEta, // rough dielectric transmission without eta^-2
+  kSlipPlasticFresnel,     // plastic specular weight 1 - F in place of F
+  kSlipPlasticEta2,        // plastic diffuse weight without eta^-2
+  kSlipRoughPlasticEta,    // rough plastic Fresnel with 1/eta in place of eta
+  kSlipCoatingSigma,       // coating absorption over the entering path only
+  kSlipCoatingExit,        // coating exit Fresnel with eta in place of 1/eta
+  kSlipRoughCoatingT,      // rough coating nested term with T(cos_i) twice
+  kSlipRftDiffuse,         // EvalDiffuse from the table half of 1/eta
+  kSlipFilmCentre,         // film footprint without the half-pixel centre offset
+};
+inline int &slip() {
+  static int s = kSlipNone;
+  return s;
+}
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kInvPi = 0.31830988618379067154;
@@ -213,7 +253,7 @@ inline V3<R> fresnel_conductor(R cos_i, V3<R> eta, V3<R> eta_k) {
   V3<R> t1 = a2b2 + V3<R>(ci2);
   V3<R> a = R(0.5f) * (a2b2 + t0);
   a = V3<R>(std::sqrt(a.x), std::sqrt(a.y), std::sqrt(a.z));
-  V3<R> t2 = (R(2) * cos_i) * a;
+  V3<R> t2 = (R(slip() == kSlipConductorFresnel ? 1 : 2) * cos_i) * a;
   V3<R> rs = (t1 - t2) / (t1 + t2);
   V3<R> t3 = ci2 * a2b2 + V3<R>(si2 * si2);
   V3<R> t4 = t2 * si2;
@@ -339,9 +379,11 @@ struct Microfacet {
       R a = R(1) / (alpha * tt);
       if (a >= R(1.6f)) return R(1);
       R a2 = a * a;
+      if (slip() == kSlipBeckmannG1) return (R(3.535f) * a) / (R(1) + R(2.276f) * a + R(2.577f) * a2);
       return (R(3.535f) * a + R(2.181f) * a2) / (R(1) + R(2.276f) * a + R(2.577f) * a2);
     }
     R root = alpha * tt;
+    if (slip() == kSlipGgxG1) root = root * root;
     // Common.hpp:608-626 Hypot2(1, root)
     R h;
     if (R(1) > std::abs(root)) { R q = root / R(1); h = R(1) * std::sqrt(R(1) + q * q); }
@@ -497,7 +539,8 @@ struct Rft {
   void set_eta(R eta) {  // :170-232
     if (eta_fixed) return;
     const float *pt = t.data(), *pd = d.data();
-    if (eta < R(1)) { pt += size_t(eta_n) * alpha_n * theta_n; pd += size_t(eta_n) * alpha_n; eta = R(1) / eta; }
+    if ((eta < R(1)) != (slip() == kSlipRftDiffuse)) pd += size_t(eta_n) * alpha_n;
+    if (eta < R(1)) { pt += size_t(eta_n) * alpha_n * theta_n; eta = R(1) / eta; }
     if (eta < eta_min) eta = eta_min;
     R we = warp(eta, eta_min, eta_max);
     std::vector<float> nt(size_t(alpha_n) * theta_n), nd(alpha_n);
@@ -546,7 +589,8 @@ struct Rft {
     else if (eta_fixed) res = cubic1d(warp(alpha, alpha_min, alpha_max), d.data(), alpha_n, R(0), R(1));
     else {
       const float *p = d.data();
-      if (eta < R(1)) { p += size_t(eta_n) * alpha_n; eta = R(1) / eta; }
+      if ((eta < R(1)) != (slip() == kSlipRftDiffuse)) p += size_t(eta_n) * alpha_n;
+      if (eta < R(1)) eta = R(1) / eta;
       if (eta < eta_min) eta = eta_min;
       res = cubic2d(warp(alpha, alpha_min, alpha_max), warp(eta, eta_min, eta_max), p, alpha_n, eta_n);
     }
@@ -728,7 +772,7 @@ struct Material {
         if (sx < sp) {
           rec.measure = kDiscrete;
           rec.wo = reflect(rec.wi);
-          return a * fi / sp;
+          return a * (slip() == kSlipPlasticFresnel ? R(1) - fi : fi) / sp;
         }
         rec.measure = kSolidAngle;
         rec.wo = square_to_cosine_hemisphere((sx - sp) / (R(1) - sp), sy);
@@ -736,7 +780,7 @@ struct Material {
         V3<R> diff = b;
         if (nonlinear) diff = diff / (V3<R>(R(1)) - diff * fdr_int);
         else diff = diff / (R(1) - fdr_int);
-        return diff * (inv_eta2 * (R(1) - fi) * (R(1) - fo) / (R(1) - sp));
+        return diff * ((slip() == kSlipPlasticEta2 ? R(1) : inv_eta2) * (R(1) - fi) * (R(1) - fo) / (R(1) - sp));
       }
       case kRoughConductor: {  // RoughConductorBSDF.cpp:54-90
         rec.measure = kSolidAngle;
@@ -751,7 +795,7 @@ struct Material {
         if (cos_theta(rec.wo) < R(0)) return V3<R>(R(0));
         R mdwi = dot(m, rec.wi);
         V3<R> f = fresnel_conductor(mdwi, eta3, eta_k);
-        R gg = md.g(rec.wi, rec.wo, m);
+        R gg = slip() == kSlipRoughConductorG ? R(1) : md.g(rec.wi, rec.wo, m);
         return a * f * md.eval(m) * gg * mdwi / (pdf * ci);
       }
       case kRoughDielectric: return sample_rough_dielectric(rec, sx, sy, u1);
@@ -1059,6 +1103,7 @@ struct Material {
       R co = cos_theta(rec.wo);
       if (co * ci >= R(0)) return V3<R>(R(0));
       R factor = ct < R(0) ? inv_eta : eta;
+      if (slip() == kSlipRoughDielectricEta) factor = R(1);
       w = w * (b * (factor * factor));
     }
     R s = std::abs(md.eval(m) * md.g(rec.wi, rec.wo, m) * dot(rec.wi, m) / (pdf * ci));
@@ -1151,7 +1196,8 @@ struct Material {
     R ct;
     V3<R> h = normalized(rec.wi + rec.wo);
     R dd = md.eval(h);
-    R f = fresnel_dielectric(dot(rec.wi, h), eta, inv_eta, ct);
+    R f = slip() == kSlipRoughPlasticEta ? fresnel_dielectric(dot(rec.wi, h), inv_eta, eta, ct)
+                                         : fresnel_dielectric(dot(rec.wi, h), eta, inv_eta, ct);
     R gg = md.g(rec.wi, rec.wi, h);
     R value = f * dd * gg / (R(4) * ci);
     res = res + a * value;
@@ -1187,7 +1233,8 @@ struct Material {
   }
   V3<R> coat_refract_out(V3<R> wi, R &refl) const {
     R ct;
-    refl = fresnel_dielectric(std::abs(cos_theta(wi)), inv_eta, eta, ct);
+    refl = slip() == kSlipCoatingExit ? fresnel_dielectric(std::abs(cos_theta(wi)), eta, inv_eta, ct)
+                                      : fresnel_dielectric(std::abs(cos_theta(wi)), inv_eta, eta, ct);
     return {eta * wi.x, eta * wi.y, -signum(cos_theta(wi)) * ct};
   }
   // src/bsdf/CoatingBSDF.cpp:39-104
@@ -1213,7 +1260,8 @@ struct Material {
     V3<R> wop = rec.wo;
     V3<R> sig = a * thickness;
     if (!is_zero_fuzzy(sig))
-      res = res * vexp(-sig * (R(1) / std::abs(cos_theta(wip)) + R(1) / std::abs(cos_theta(wop))));
+      res = res * vexp(-sig * (R(1) / std::abs(cos_theta(wip)) +
+                               (slip() == kSlipCoatingSigma ? R(0) : R(1) / std::abs(cos_theta(wop)))));
     R r21;
     rec.wo = coat_refract_out(wop, r21);
     if (r21 == R(1)) return V3<R>(R(0));
@@ -1238,7 +1286,8 @@ struct Material {
     V3<R> res = nested->eval(ri) * (R(1) - r12) * (R(1) - r21);
     V3<R> sig = a * thickness;
     if (!is_zero_fuzzy(sig))
-      res = res * vexp(-sig * (R(1) / std::abs(cos_theta(ri.wi)) + R(1) / std::abs(cos_theta(ri.wo))));
+      res = res * vexp(-sig * (R(1) / std::abs(cos_theta(ri.wi)) +
+                               (slip() == kSlipCoatingSigma ? R(0) : R(1) / std::abs(cos_theta(ri.wo)))));
     if (rec.measure == kSolidAngle)
       res = res * (inv_eta * inv_eta * cos_theta(rec.wo) / cos_theta(ri.wo));
     return res;
@@ -1316,7 +1365,8 @@ struct Material {
     BRec<R> ri = rec;
     ri.wi = rc_refract(rec.wi, inv_eta);
     ri.wo = rc_refract(rec.wo, inv_eta);
-    V3<R> nv = nested->eval(ri) * ext->eval(std::abs(ci), md.au) * ext->eval(std::abs(co), md.au);
+    V3<R> nv = nested->eval(ri) * ext->eval(std::abs(ci), md.au) *
+               ext->eval(std::abs(slip() == kSlipRoughCoatingT ? ci : co), md.au);
     V3<R> sig = a * thickness;
     if (!is_zero_fuzzy(sig))
       nv = nv * vexp(-sig * (R(1) / std::abs(cos_theta(ri.wi)) + R(1) / std::abs(cos_theta(ri.wo))));

@@ -329,6 +329,35 @@ struct MeshIn {
   int is_emitter = 0;
   float radiance[3] = {0, 0, 0};
 };
+// The tabulated reconstruction filter of ImageBlock (src/core/Block.cpp:9-33:
+// 32 entries on [0, r) plus a closing zero) for the box, Gaussian, tent and
+// Mitchell-Netravali filters of src/filter/*.cpp.
+struct FilmFilter {
+  float table[33];
+  float radius = 0.5f, lookup = 64;
+  void setup(int ft, float r, float stddev, float b, float c) {
+    radius = ft == 0 ? 0.5f : (ft == 2 ? 1.0f : r);
+    for (int i = 0; i < 32; i++) {
+      float x = (radius * i) / 32;
+      float w;
+      if (ft == 0) w = 1.0f;
+      else if (ft == 2) w = std::max(0.0f, 1.0f - std::abs(x));
+      else if (ft == 3) {
+        float X = std::abs(2.0f * x / radius), X2 = X * X, X3 = X2 * X, B = b, C = c;
+        if (X < 1.0f) w = 1.0f / 6.0f * ((12.0f - 9.0f * B - 6.0f * C) * X3 + (-18.0f + 12.0f * B + 6.0f * C) * X2 + (6.0f - 2.0f * B));
+        else if (X < 2.0f) w = 1.0f / 6.0f * ((-B - 6 * C) * X3 + (6.0f * B + 30.0f * C) * X2 + (-12.0f * B - 48.0f * C) * X + (8.0f * B + 24.0f * C));
+        else w = 0.0f;
+      } else {
+        float al = -1.0f / (2.0f * stddev * stddev);
+        w = std::max(0.0f, std::exp(al * x * x) - std::exp(al * radius * radius));
+      }
+      table[i] = w;
+    }
+    table[32] = 0.0f;
+    lookup = 32 / radius;
+  }
+};
+
 struct SceneIn {
   std::vector<MeshIn> meshes;
   const float *materials = nullptr; int n_materials = 0;
@@ -377,9 +406,7 @@ struct Scene {
   M4 sample_to_camera, cam_to_world;
   int width = 1, height = 1;
   R inv_w = 1, inv_h = 1, near_clip = 0, far_clip = 0;
-  // film
-  float filter_table[33];
-  float filter_radius = 0.5f, lookup = 64;
+  FilmFilter film;
   int max_depth = 1000;
   int integrator = 0;  // 0 pathMIS, 1 pathEMS, 2 pathMATS, 3 whitted
   V3<R> background;
@@ -485,27 +512,7 @@ struct Scene {
       M4 T = M4::identity(); T.m[0][3] = -1; T.m[1][3] = -1.0 / aspect;
       sample_to_camera = (S * T * P).inverse();
     }
-    // film (src/core/Block.cpp:9-33 tabulation; src/filter/*.cpp)
-    int ft = in.filter;
-    filter_radius = ft == 0 ? 0.5f : (ft == 2 ? 1.0f : in.filter_radius);
-    for (int i = 0; i < 32; i++) {
-      float x = (filter_radius * i) / 32;
-      float w;
-      if (ft == 0) w = 1.0f;
-      else if (ft == 2) w = std::max(0.0f, 1.0f - std::abs(x));
-      else if (ft == 3) {
-        float X = std::abs(2.0f * x / filter_radius), X2 = X * X, X3 = X2 * X, B = in.filter_b, C = in.filter_c;
-        if (X < 1.0f) w = 1.0f / 6.0f * ((12.0f - 9.0f * B - 6.0f * C) * X3 + (-18.0f + 12.0f * B + 6.0f * C) * X2 + (6.0f - 2.0f * B));
-        else if (X < 2.0f) w = 1.0f / 6.0f * ((-B - 6 * C) * X3 + (6.0f * B + 30.0f * C) * X2 + (-12.0f * B - 48.0f * C) * X + (8.0f * B + 24.0f * C));
-        else w = 0.0f;
-      } else {
-        float al = -1.0f / (2.0f * in.filter_stddev * in.filter_stddev);
-        w = std::max(0.0f, std::exp(al * x * x) - std::exp(al * filter_radius * filter_radius));
-      }
-      filter_table[i] = w;
-    }
-    filter_table[32] = 0.0f;
-    lookup = 32 / filter_radius;
+    film.setup(in.filter, in.filter_radius, in.filter_stddev, in.filter_b, in.filter_c);
     // the depth property is read as uint32 (PathMISIntegrator.cpp:13,
     // WhittedIntegrator.cpp:13): a negative value means no limit; paths are
     // capped at 4095 vertices as on the device
@@ -956,15 +963,16 @@ struct Film {
   int w, h;
   std::vector<double> acc;  // rgbw
   Film(int W, int H) : w(W), h(H), acc(size_t(W) * H * 4, 0.0) {}
-  void put(const Scene<R> &sc, float px, float py, V3<R> val) {
+  void put(const FilmFilter &ff, float px, float py, V3<R> val) {
     if (!color_valid(val)) return;
-    float cx = px - 0.5f, cy = py - 0.5f, r = sc.filter_radius;
+    float half = slip() == kSlipFilmCentre ? 0.0f : 0.5f;
+    float cx = px - half, cy = py - half, r = ff.radius;
     int x0 = std::max(0, int(std::ceil(cx - r))), x1 = std::min(w - 1, int(std::floor(cx + r)));
     int y0 = std::max(0, int(std::ceil(cy - r))), y1 = std::min(h - 1, int(std::floor(cy + r)));
     for (int y = y0; y <= y1; y++) {
-      float wy = sc.filter_table[int(std::abs(float(y) - cy) * sc.lookup)];
+      float wy = ff.table[int(std::abs(float(y) - cy) * ff.lookup)];
       for (int x = x0; x <= x1; x++) {
-        float wx = sc.filter_table[int(std::abs(float(x) - cx) * sc.lookup)];
+        float wx = ff.table[int(std::abs(float(x) - cx) * ff.lookup)];
         double wgt = double(wx) * double(wy);
         double *a = &acc[(size_t(y) * w + x) * 4];
         a[0] += double(val.x) * wgt; a[1] += double(val.y) * wgt; a[2] += double(val.z) * wgt; a[3] += wgt;
@@ -1014,7 +1022,7 @@ void render(const Scene<R> &sc, uint32_t req_seed, int s0, int s1, int threads, 
           for (int s = s0; s < s1; s++) {
             float px, py;
             V3<R> v = render_sample(sc, req_seed, uint32_t(x), uint32_t(y), uint32_t(s), px, py, &sts[t]);
-            films[t].put(sc, px, py, v);
+            films[t].put(sc.film, px, py, v);
           }
     });
   }

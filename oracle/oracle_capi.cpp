@@ -200,6 +200,39 @@ double ho_rft_eval(const char *path, int stage, float eta, float alpha, float co
   return r.eval(cos_t, alpha, eta);
 }
 
+// RoughTransmittance::EvalDiffuse (src/core/RoughTransmitance.cpp:114-150) of
+// the full table (stage 0), after SetEta (1) or after SetAlpha (2).
+double ho_rft_eval_diffuse(const char *path, int stage, float eta, float alpha) {
+  RftFile f;
+  f.load(path);
+  Rft<double> r(f);
+  if (stage >= 1) r.set_eta(eta);
+  if (stage >= 2) r.set_alpha(alpha);
+  return r.eval_diffuse(alpha, eta);
+}
+
+// Selects a planted mistake of the restatement (hk_oracle.hpp, enum Slip);
+// 0 restores the faithful one.
+void ho_set_slip(int s) { slip() = s; }
+
+// ImageBlock::Put (src/core/Block.cpp:95-134) of n samples (pos: n x 2 film
+// coordinates, val: n x 3) into a W x H film with the tabulated filter of
+// type ft (0 box, 1 gaussian, 2 tent, 3 mitchell); out: W x H x 4 sums.
+int ho_film_splat(int ft, float radius, float stddev, float b, float c, int W, int H, int n, const float *pos,
+                  const float *val, double *out) {
+  if (W <= 0 || H <= 0 || n < 0) {
+    snprintf(g_err, sizeof g_err, "bad film %d x %d or sample count %d", W, H, n);
+    return -1;
+  }
+  FilmFilter ff;
+  ff.setup(ft, radius, stddev, b, c);
+  Film<float> film(W, H);
+  for (int i = 0; i < n; i++)
+    film.put(ff, pos[2 * i], pos[2 * i + 1], V3<float>(val[3 * i], val[3 * i + 1], val[3 * i + 2]));
+  std::memcpy(out, film.acc.data(), film.acc.size() * sizeof(double));
+  return 0;
+}
+
 double ho_fresnel_dielectric(double cos_i, double eta, double *cos_t) {
   return fresnel_dielectric<double>(cos_i, eta, 1.0 / eta, *cos_t);
 }

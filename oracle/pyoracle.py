@@ -43,6 +43,11 @@ def lib():
         L.ho_render.argtypes = [C.c_int, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_int, P, P, P]
         L.ho_pixel_samples.argtypes = [C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.c_uint32, P]
+        L.ho_rft_eval_diffuse.argtypes = [C.c_char_p, C.c_int, C.c_float, C.c_float]
+        L.ho_rft_eval_diffuse.restype = C.c_double
+        L.ho_set_slip.argtypes = [C.c_int]
+        L.ho_film_splat.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                    P, P, P]
         _lib = L
     return _lib
 
@@ -119,6 +124,31 @@ def rft_reduce(path, eta, alpha, stage):
 
 def rft_eval(path, stage, eta, alpha, cos_t):
     return lib().ho_rft_eval(path.encode(), stage, eta, alpha, cos_t)
+
+
+def rft_eval_diffuse(path, stage, eta, alpha):
+    return lib().ho_rft_eval_diffuse(path.encode(), stage, eta, alpha)
+
+
+# Planted mistakes of the restatement (oracle/hk_oracle.hpp, enum Slip), for
+# the tests that show a pinning check fails when the oracle is wrong.
+SLIPS = ["none", "ggx_g1", "beckmann_g1", "conductor_fresnel", "rough_conductor_g", "rough_dielectric_eta",
+         "plastic_fresnel", "plastic_eta2", "rough_plastic_eta", "coating_sigma", "coating_exit", "rough_coating_t",
+         "rft_diffuse", "film_centre"]
+
+
+def set_slip(name):
+    lib().ho_set_slip(SLIPS.index(name))
+
+
+def film_splat(filter_type, radius, stddev, width, height, pos, val, b=1.0 / 3, c=1.0 / 3):
+    """ImageBlock::Put of n samples (pos (n,2), val (n,3)) -> (H, W, 4) float64 sums."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    val = np.ascontiguousarray(val, np.float32)
+    out = np.zeros((height, width, 4), np.float64)
+    _check(lib().ho_film_splat(filter_type, radius, stddev, b, c, width, height, pos.shape[0], _p(pos), _p(val),
+                               _p(out)))
+    return out
 
 
 def fresnel_dielectric(cos_i, eta):

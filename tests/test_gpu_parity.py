@@ -105,63 +105,161 @@ def _bsdf_inputs(rng, n):
     return wi, wo, measure, u
 
 
-def _rel_err(g, r):
-    g = np.asarray(g, np.float64)
-    r = np.asarray(r, np.float64)
-    return np.abs(g - r) / np.maximum(np.abs(r), 1e-3)
+# Conditioning rule of the BSDF parity tests.  The GPU evaluates in FP32 and
+# the oracle in FP64 on the same FP32 inputs.  An input is "flagged" as
+# ill-conditioned when, at the input or one FP32 ulp away from it in one
+# direction component or uniform, either the FP64 result moves by more than
+# REL / FLAG, or the reference formulas evaluated in FP32 (the oracle's FP32
+# mode: the reference's own precision and operation order) land further than
+# REL / FLAG from FP64 (relative to max(|ref|, 1e-3)).  Flagged inputs are held
+# to COND times the larger of the two; every other input to REL, except a tail
+# of at most TAIL of the inputs, held to 10 REL: roundings of intermediates
+# that no input step exposes (a uniform just below the table-derived branch
+# probability it is then divided by; 1 - F at grazing incidence where F ~ 1).
+REL = 1e-5
+FLAG = 4.0
+COND = 8.0
+TAIL = 5e-4
+
+
+def _local_error(fn, keys, inputs):
+    """Per row: (FP64 result, its largest move under one step of one input,
+    the largest FP32-vs-FP64 gap of the reference formulas at the input or one
+    step away)."""
+    base64 = fn(fp64=True, **inputs)
+    base32 = fn(fp64=False, **inputs)
+    n = len(next(iter(inputs.values())))
+
+    def rows(x):
+        x = np.asarray(x, np.float64)
+        return x.reshape(n, -1)
+    sens = {k: np.zeros(n) for k in keys}
+    e32 = {k: np.abs(rows(base32[k]) - rows(base64[k])).max(axis=1) for k in keys}
+    for name, x in inputs.items():
+        for moved in (_uniform_steps(x) if name == "u" else _ulp_steps(x)):
+            var = {**inputs, name: moved}
+            r64, r32 = fn(fp64=True, **var), fn(fp64=False, **var)
+            for k in keys:
+                sens[k] = np.maximum(sens[k], np.abs(rows(r64[k]) - rows(base64[k])).max(axis=1))
+                e32[k] = np.maximum(e32[k], np.abs(rows(r32[k]) - rows(r64[k])).max(axis=1))
+    return base64, sens, e32
+
+
+def _ulp_steps(x):
+    """x moved one FP32 ulp up and down (each component of each row in turn)."""
+    out = []
+    for k in range(x.shape[1]):
+        for d in (np.inf, -np.inf):
+            y = x.copy()
+            y[:, k] = np.nextafter(x[:, k], np.float32(d))
+            out.append(y)
+    return out
+
+
+def _uniform_steps(u):
+    """The two Sample() uniforms and the rough dielectric's extra draw, each
+    moved one FP32 ulp (kept inside [0, 1))."""
+    out = []
+    top = np.nextafter(np.float32(1), np.float32(0))
+    for k in range(3):
+        for d in (np.float32(1), np.float32(0)):
+            y = u.copy()
+            y[:, k] = np.minimum(np.nextafter(u[:, k], d), top)
+            out.append(y)
+    return out
+
+
+def _judge(g, r, sens, e32, scale):
+    """-> (error, flagged, tolerance, flagged by FP64 sensitivity alone) per row."""
+    err = np.abs(np.asarray(g, np.float64) - np.asarray(r, np.float64)).reshape(len(scale), -1).max(axis=1)
+    local = np.maximum(sens, e32)
+    flagged = FLAG * local > REL * scale
+    tol = np.where(flagged, COND * local, REL * scale)
+    return err, flagged, tol, FLAG * sens > REL * scale
 
 
 @pytest.mark.parametrize("index", list(range(len(material_table()))))
 def test_bsdf_eval_pdf_vs_fp64(ctx, index):
-    """Eval / Pdf on identical inputs within 1e-5 relative of the FP64 oracle (floor 1e-3 on the scale)."""
+    """Eval / Pdf on identical inputs within 1e-5 relative of the FP64 oracle
+    (of max(|ref|, 1e-3)) on every input the conditioning rule does not flag,
+    but for a tail of at most 0.05% within 1e-4; flagged ones within 8x the
+    rule's local error; under 3% of the inputs are ill-conditioned in FP64
+    itself."""
     rng = np.random.default_rng(100 + index)
     mats = material_table()
     wi, wo, measure, u = _bsdf_inputs(rng, 20000)
     g = ctx.bsdf_batch(mats, index, wi, wo, measure, u)
-    r = O.bsdf_batch(mats, index, wi, wo, measure, u, fp64=True)
-    e_eval = _rel_err(g["eval"], r["eval"]).max(axis=1)
-    e_pdf = _rel_err(g["pdf"], r["pdf"])
-    print(f"mat {index}: eval max rel {e_eval.max():.2e} p99.9 {np.quantile(e_eval, 0.999):.2e}; "
-          f"pdf max rel {e_pdf.max():.2e}")
-    assert np.quantile(e_eval, 0.999) <= 1e-5 and np.quantile(e_pdf, 0.999) <= 1e-5
-    assert e_eval.max() <= 1e-3 and e_pdf.max() <= 1e-3
+
+    def ref(wi, wo, fp64):
+        return O.bsdf_batch(mats, index, wi, wo, measure, u, fp64=fp64)
+    r, sens, e32 = _local_error(ref, ("eval", "pdf"), {"wi": wi, "wo": wo})
+    report = []
+    for key in ("eval", "pdf"):
+        scale = np.maximum(np.abs(np.asarray(r[key], np.float64)).reshape(len(wi), -1).max(axis=1), 1e-3)
+        err, flagged, tol, ill = _judge(g[key], r[key], sens[key], e32[key], scale)
+        rel = err / scale
+        worst = int(np.argmax(err / tol))
+        report.append(f"{key}: max rel {rel[~flagged].max():.2e} on {int((~flagged).sum())} unflagged, "
+                      f"{rel[~ill].max():.2e} on {int((~ill).sum())} well-conditioned; flagged {int(flagged.sum())} "
+                      f"(ill-conditioned {int(ill.sum())}); worst err/tol {err[worst] / tol[worst]:.2f} (wi "
+                      f"{wi[worst]}, wo {wo[worst]}, measure {measure[worst]})")
+        tail = ~flagged & (rel > REL)
+        report[-1] += f"; tail {int(tail.sum())}"
+        assert tail.mean() <= TAIL and (rel[tail] <= 10 * REL).all(), report[-1]
+        assert (err[~tail] <= tol[~tail]).all(), report[-1]
+        assert ill.mean() < 0.03, report[-1]
+    print(f"mat {index}: " + "; ".join(report))
 
 
 @pytest.mark.parametrize("index", list(range(len(material_table()))))
 def test_bsdf_sample_vs_fp64(ctx, index):
-    """Sample() on identical uniforms: within 1e-5 relative of the FP64 oracle, or within the FP64
-    result's own change when a uniform moves by 2^-22 (the absolute resolution of FP32 values in
-    [0, 1) that the warps remap, e.g. (u - p) / (1 - p)), where the warp amplifies it."""
+    """Sample() on identical uniforms: the same measure, f within 1e-5
+    relative (of max(|f|, 1e-3)) and wo within 1e-5 of the FP64 oracle on
+    every input the conditioning rule does not flag (a step of an input that
+    changes the FP64 measure flags it too: it sits on a branch boundary), but
+    for a tail of at most 0.05% within 1e-4; flagged ones within 8x the
+    rule's local error; under 3% of the inputs are ill-conditioned in FP64
+    itself.  (The reference's FP32 sin = sqrt(1 -
+    cos^2) of the sampled microfacet loses digits near the normal, so more
+    inputs of the sharp lobes are flagged by the FP32 criterion; the GPU
+    computes tan * cos there, and the printed well-conditioned maxima show
+    how close it is.)"""
     rng = np.random.default_rng(200 + index)
     mats = material_table()
     wi, wo, measure, u = _bsdf_inputs(rng, 20000)
     g = ctx.bsdf_batch(mats, index, wi, wo, measure, u)
-    r = O.bsdf_batch(mats, index, wi, wo, measure, u, fp64=True)
-    # sensitivity of the FP64 mapping to the last bit of each uniform
-    sens_f = np.zeros(len(u))
-    sens_wo = np.zeros(len(u))
-    for k in range(2):
-        for direction in (np.inf, -np.inf):
-            up = u.copy()
-            step = np.float32(2.0 ** -22) * (1 if direction > 0 else -1)
-            up[:, k] = np.clip(u[:, k] + step, 0.0, np.nextafter(np.float32(1), np.float32(0)))
-            rp = O.bsdf_batch(mats, index, wi, wo, measure, up, fp64=True)
-            sens_f = np.maximum(sens_f, np.abs(rp["f"] - r["f"]).max(axis=1))
-            sens_wo = np.maximum(sens_wo, np.abs(rp["wo"] - r["wo"]).max(axis=1))
-    same = g["measure"] == r["measure"]
-    tol_f = np.maximum(1e-5 * np.maximum(np.abs(r["f"]).max(axis=1), 1e-3), 4 * sens_f)
-    tol_wo = np.maximum(1e-5, 4 * sens_wo)
-    ok_f = np.abs(g["f"] - r["f"]).max(axis=1) <= tol_f
-    ok_wo = np.abs(g["wo"] - r["wo"]).max(axis=1) <= tol_wo
-    ok = same & ok_f & ok_wo
-    print(f"mat {index}: measure mismatches {int((~same).sum())}, f outside tol {int((~ok_f).sum())}, "
-          f"wo outside tol {int((~ok_wo).sum())} of {len(ok)}")
-    if ok.mean() < 0.999:
+
+    def ref(wi, u, fp64):
+        return O.bsdf_batch(mats, index, wi, wo, measure, u, fp64=fp64)
+    r, sens, e32 = _local_error(ref, ("f", "wo", "measure"), {"wi": wi, "u": u})
+    flip = np.maximum(sens["measure"], e32["measure"]) > 0
+    scale = np.maximum(np.abs(r["f"]).max(axis=1), 1e-3)
+    err_f, fl_f, tol_f, ill_f = _judge(g["f"], r["f"], sens["f"], e32["f"], scale)
+    err_wo, fl_wo, tol_wo, ill_wo = _judge(g["wo"], r["wo"], sens["wo"], e32["wo"], np.ones(len(wi)))
+    fl_f |= flip
+    fl_wo |= flip
+    ill_f |= flip
+    ill_wo |= flip
+    same = (g["measure"] == r["measure"]) | flip
+    rel_f = err_f / scale
+    tail = (~fl_f & (rel_f > REL)) | (~fl_wo & (err_wo > REL))
+    ok = same & (((err_f <= tol_f) & (err_wo <= tol_wo)) |
+                 (tail & (rel_f <= np.maximum(10 * REL, tol_f)) & (err_wo <= np.maximum(10 * REL, tol_wo))))
+    worst = int(np.argmax(np.maximum(err_f / tol_f, err_wo / tol_wo)))
+    msg = (f"mat {index}: f max rel {rel_f[~fl_f].max():.2e} / wo max {err_wo[~fl_wo].max():.2e} on unflagged, "
+           f"{rel_f[~ill_f].max():.2e} / {err_wo[~ill_wo].max():.2e} on well-conditioned; flagged f {int(fl_f.sum())} "
+           f"wo {int(fl_wo.sum())} (ill-conditioned {int(ill_f.sum())} / {int(ill_wo.sum())}, branch "
+           f"{int(flip.sum())}); tail {int(tail.sum())}; measure mismatches {int((~same).sum())}; outside tol "
+           f"{int((~ok).sum())}; worst "
+           f"(wi {wi[worst]}, u {u[worst]}): f {g['f'][worst]} vs {r['f'][worst]}")
+    print(msg)
+    if not ok.all():
         import os
         os.makedirs("gpurun_out", exist_ok=True)
         np.savez(f"gpurun_out/bsdf_sample_fail_{index}.npz", wi=wi, wo=wo, measure=measure, u=u, ok=ok,
                  **{"g_" + k: v for k, v in g.items()}, **{"r_" + k: v for k, v in r.items()})
-    assert ok.mean() >= 0.999
+    assert ok.all() and tail.mean() <= TAIL, msg
+    assert ill_f.mean() < 0.03 and ill_wo.mean() < 0.03, msg
 
 
 def _furnace(albedo):
@@ -187,15 +285,22 @@ def test_furnace_gpu(ctx, albedo, expect):
 
 
 def test_gpu_paths_track_oracle_paths(ctx):
-    """Same (pixel, sample) keys drive the same random path on GPU and oracle."""
+    """Same (pixel, sample) keys drive the same random path on GPU and oracle:
+    the film positions agree exactly and the radiance of at least 99.5% of the
+    samples within 1e-3 (measured: 99.8%; the rest are paths that a last-bit
+    difference sends another way, e.g. a Russian-roulette draw compared with a
+    throughput, after which the two streams decorrelate)."""
     sc = scenes.cbox(width=32, height=32, sphere_subdiv=2)
     gs = ctx.scene(sc)
     g = gs.pixel_samples(16, 12, 0, 4000)
     o = O.pixel_samples(sc, 16, 12, 0, 4000)
     assert np.allclose(g[:, 3:], o[:, 3:], atol=0)
-    close = np.all(np.abs(g[:, :3] - o[:, :3]) <= 1e-3 * np.maximum(1.0, np.abs(o[:, :3])), axis=1)
-    print("fraction of identical paths:", close.mean())
-    assert close.mean() > 0.95
+    diff = np.abs(g[:, :3] - o[:, :3]).max(axis=1) / np.maximum(1.0, np.abs(o[:, :3]).max(axis=1))
+    close = diff <= 1e-3
+    worst = np.argsort(-diff)[:5]
+    print("fraction of identical paths:", close.mean(), "worst samples:",
+          [(int(k), g[k, :3].tolist(), o[k, :3].tolist()) for k in worst])
+    assert close.mean() >= 0.995
 
 
 def test_cbox_image_vs_oracle(ctx):

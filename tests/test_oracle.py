@@ -212,6 +212,30 @@ def test_sample_weight_is_eval_over_pdf(index, cos_in_eval):
     assert np.allclose(r["f"][good], expect, rtol=2e-3, atol=1e-6)
 
 
+@pytest.mark.parametrize("index", [6, 7])
+def test_rough_dielectric_eval_over_pdf_is_sample(index):
+    """RoughDielectricBSDF: for reflected samples Sample() returns Eval / Pdf of
+    the sampled direction (both D G |wo.h| / (D' cos_h |cos_i|), h = m).  For
+    transmitted ones the two radiance factors differ in the reference itself:
+    Sample() scales by (CosThetaT < 0 ? 1/eta : eta)^2
+    (RoughDielectricBSDF.cpp:138-139), Eval by (CosThetaI < 0 ? 1/eta : eta)^2
+    (:204-206), so Eval / Pdf = eta^4 Sample() entering and eta^-4 leaving.
+    The oracle keeps both as written; this pins Eval to the Sample() that the
+    smooth limit pins."""
+    rng = np.random.default_rng(index)
+    mats = material_table()
+    n = 5000
+    wi = random_dirs(rng, n)
+    r = O.bsdf_batch(mats, index, wi, wi, np.ones(n, np.int32), rng.uniform(size=(n, 4)).astype(np.float32))
+    q = O.bsdf_batch(mats, index, wi, r["wo"].astype(np.float32), r["measure"], np.zeros((n, 4), np.float32))
+    good = (r["spdf"] > 1e-3) & (np.abs(r["f"]).sum(1) > 0)
+    eta = float(mats[index][20] / mats[index][21])
+    refl = r["wo"][:, 2] * wi[:, 2] > 0
+    factor = np.where(refl, 1.0, np.where(wi[:, 2] > 0, eta ** 4, eta ** -4))
+    assert good.sum() > 0.5 * n and (good & ~refl & (wi[:, 2] > 0)).any() and (good & ~refl & (wi[:, 2] < 0)).any()
+    assert np.allclose(q["eval"][good] / q["pdf"][good, None], factor[good, None] * r["f"][good], rtol=2e-3)
+
+
 def test_mirror_closed_form():
     """MirrorBSDF: wo = (-x, -y, z), weight 1, discrete; Eval = Pdf = 1 only there and only for the
     discrete measure; nothing from below."""
