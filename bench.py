@@ -1,24 +1,28 @@
 #!/usr/bin/env python3
-"""Throughput of the gpuPathMIS renderer on one MI355X.
+"""Throughput and latency of the gpuPathMIS renderer on one MI355X.
 
-Workload (one step = one full render): displaced icosphere of subdivision 8
-(1,310,720 triangles, radii 1 + 0.08 fBm), GGX rough gold conductor
-(alpha 0.3), diffuse ground, synthetic 512 x 256 environment map sampled by
-its CDF, pinhole camera, 1024 x 768, 256 samples per pixel, Gaussian filter.
+Default run (one JSON line):
+  * value: the plain single-request path on the displaced-sphere workload
+    (one step = one full render): icosphere of subdivision 8 (1,310,720
+    triangles, radii 1 + 0.08 fBm), GGX rough gold conductor (alpha 0.3),
+    diffuse ground, synthetic 512 x 256 environment map sampled by its CDF,
+    pinhole camera, 1024 x 768, 256 samples per pixel, Gaussian filter; with
+    p50/p99 step latency, rays/s, the HBM roofline of the dominant kernel
+    (k_extend, timed with HIP events on the context stream) and the relMSE of
+    a reduced film of the same scene against the oracle at equal streams;
+  * batch: the 64-request mix over the same scene (res {64..1024}^2, spp
+    {16..1024}, seeded poses on a radius-4 sphere) in one hk_render_batch
+    against the same requests rendered one at a time with hk_render, images
+    compared bit for bit, with the plain path's rate per film size;
+  * serve: Poisson arrivals over three resident scenes (box, displaced
+    sphere, full heightfield) through hk_server against a first-come-first-
+    served queue of plain hk_render calls on the same trace: p50/p99 latency
+    and path samples/s of both;
+  * cpu_baseline: the oracle (the C++ restatement) on the host cores on a
+    bounded sample of the displaced-sphere workload, with its measured ratio
+    to the reference renderer on the box scene.
 
-Prints one JSON line: path samples/s (whole job), p50/p99 step latency,
-rays/s, the HBM roofline of the dominant kernel (k_extend, timed with HIP
-events on the context stream) and the oracle's CPU throughput on a bounded
-sample of the same workload.
-
-Other workloads (one JSON line each, not part of the default run):
-  --workload batch  64 requests over the same scene in one hk_render_batch,
-                    against the same requests rendered one at a time
-                    (hk_render); images are compared bit for bit.
-  --workload serve  Poisson arrivals over three resident scenes (box,
-                    displaced sphere, heightfield) through hk_server, against
-                    a first-come-first-served queue of plain hk_render calls
-                    on the same trace.
+--workload c2|batch|serve|accel runs one leg alone (its own JSON line).
 """
 import argparse
 import json
@@ -37,6 +41,16 @@ HBM_PEAK_GBS = 8000.0
 #   the extension kernel's own share of a segment is the ray read (32 B), the
 #   hit write (16 B) and the triangle fetch (40 B) = 88 B per ray.
 SEG_B, SHADOW_B, PATH_B, EXTEND_B = 296, 190, 128, 88
+METRIC = "path samples/sec at matched relMSE vs plain GPU path; p50/p99 render latency"
+
+# The oracle against the reference renderer, both on the 256 x 256 box
+# scene at 64 spp on the same 8 host cores: the reference's pathMIS (built
+# headless with its TBB loop replaced by a std::thread driver) measured
+# 3.35e5 path samples/s and 26.3 rays per path; the oracle measured
+# 3.74e5-3.86e5 and 24.3 rays per path.
+ANCHOR_C0 = {"scene": "256x256 box, 64 spp, 8 threads of one host", "reference_path_samples_per_s": 3.35e5,
+             "reference_rays_per_path": 26.3, "oracle_path_samples_per_s": 3.80e5, "oracle_rays_per_path": 24.3,
+             "oracle_over_reference": 3.80e5 / 3.35e5}
 
 
 def parse():
@@ -52,6 +66,8 @@ def parse():
     ap.add_argument("--fetch-batch", type=int, default=0,
                     help="queue entries a persistent wave reserves per atomic, in 64s (0: library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="default run: skip the batch leg")
+    ap.add_argument("--no-serve", action="store_true", help="default run: skip the serving leg")
     ap.add_argument("--graph", type=int, default=1, help="1: replay bounces from captured HIP graphs")
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: pick max_paths / grid / tile with hk_autotune before timing (results do not change); "
@@ -61,12 +77,16 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["c1", "c2", "batch", "serve", "accel"], default="c1")
     ap.add_argument("--requests", type=int, default=0, help="batch/serve: number of requests (default 64 / 40)")
-    ap.add_argument("--qps", type=float, default=1.0, help="serve: Poisson arrival rate")
+    ap.add_argument("--qps", type=float, default=10.0, help="serve: Poisson arrival rate")
     ap.add_argument("--budget-gb", type=float, default=96.0, help="serve: HBM budget of the server")
-    ap.add_argument("--schedule", type=int, default=1, help="serve / batch: 1 fewest samples left first "
-                    "(library default), 0 first come first served")
+    ap.add_argument("--schedule", type=int, default=1, help="batch: 1 fewest samples left first (library "
+                    "default), 0 first come first served")
     ap.add_argument("--detail", default="", help="batch / serve: write per-request timings (JSON) to this file")
     return ap.parse_args()
+
+
+def log(msg):
+    print("# " + msg, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(scene, target_s):
@@ -82,31 +102,30 @@ def cpu_baseline(scene, target_s):
         paths += int(st[2])
         secs += float(st[3]) * 1e-6
         spp_done += extra
-    return {"value": paths / secs, "unit": "path samples/s", "cores": threads, "kind": "port",
+    value = paths / secs
+    return {"value": value, "unit": "path samples/s", "cores": threads, "kind": "port",
             "sample": f"same scene, all {scene.width}x{scene.height} pixels at {spp_done} spp "
-                      f"({paths} paths, {secs:.1f} s render time, BVH build excluded)"}
+                      f"({paths} paths, {secs:.1f} s render time, BVH build excluded)",
+            "anchor_c0": ANCHOR_C0,
+            "reference_estimate": value / ANCHOR_C0["oracle_over_reference"]}
 
 
 def _pct(xs, q):
     return float(np.percentile(np.asarray(xs, np.float64), q))
 
 
-def bench_batch(a):
-    """64 requests over the displaced-sphere scene: one batch vs one at a time."""
-    import hikari_amd as hk
-    from hikari_amd import scenes, workloads
-    scene = scenes.displaced_sphere(alpha=a.alpha)
-    ctx = hk.Context(0)
-    ctx.set_option("schedule", a.schedule)
-    gs = ctx.scene(scene)
-    reqs = workloads.batch_requests(n=a.requests or 64)
+def batch_leg(gs, n_requests, schedule, detail=""):
+    """The 64-request mix in one hk_render_batch vs one hk_render each."""
+    import hikari_amd as hk  # noqa: F401
+    from hikari_amd import workloads
+    reqs = workloads.batch_requests(n=n_requests)
     total = sum(r.width * r.height * r.spp for r in reqs)
-    print(f"# {len(reqs)} requests, {total / 1e9:.2f} G path samples", file=sys.stderr, flush=True)
-    gs.render(4)  # warm-up (kernels loaded, workspace allocated)
+    log(f"batch: {len(reqs)} requests, {total / 1e9:.2f} G path samples")
+    gs.ctx.set_option("schedule", schedule)
     t0 = time.perf_counter()
     imgs_b, st = gs.render_batch(reqs)
     t_batch = time.perf_counter() - t0
-    print(f"# batched: {t_batch:.2f} s", file=sys.stderr, flush=True)
+    log(f"batch: batched {t_batch:.2f} s")
     lat_plain, same, rows = [], 0, []
     t0 = time.perf_counter()
     for r, ib in zip(reqs, imgs_b):
@@ -117,42 +136,33 @@ def bench_batch(a):
                      "gpu_ms": pst["gpu_ms"], "bounces": pst["bounces_launched"]})
         same += int(np.array_equal(img.view(np.uint32), ib.view(np.uint32)))
     t_plain = time.perf_counter() - t0
-    if a.detail:
-        with open(a.detail, "w") as fh:
+    gs.ctx.set_option("schedule", 1)
+    log(f"batch: one at a time {t_plain:.2f} s, identical images {same}/{len(reqs)}")
+    if detail:
+        with open(detail, "w") as fh:
             json.dump(rows, fh)
-    print(f"# one at a time: {t_plain:.2f} s, identical images {same}/{len(reqs)}", file=sys.stderr, flush=True)
-    out = {
-        "metric": "path samples/sec at matched relMSE vs plain GPU path; p50/p99 render latency",
-        "value": total / t_batch, "unit": "path samples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seeded procedural scene and request mix)",
-        "config": {"workload": f"{len(reqs)} batched requests over the displaced icosphere-8 scene, res "
-                               "{64..1024}^2, spp {16..1024}, camera on a radius-4 sphere",
-                   "requests": len(reqs), "path_samples": total, "schedule": ["fifo", "shortest_first"][a.schedule]},
-        "plain_value": total / t_plain, "batch_over_plain": t_plain / t_batch,
-        "identical_images": same,
-        "latency_ms": {"batched_p50": t_batch * 1e3, "batched_p99": t_batch * 1e3,
-                       "plain_p50": _pct(lat_plain, 50) * 1e3, "plain_p99": _pct(lat_plain, 99) * 1e3},
-        "rays_per_s": (st["extension_rays"] + st["shadow_rays"]) / t_batch,
-    }
-    print(json.dumps(out), flush=True)
+    # plain rate per film size, and the batch ratio it allows: a batch can at
+    # best run every sample at the largest films' plain rate
+    by = {}
+    for row in rows:
+        s = by.setdefault(row["pixels"], [0, 0.0])
+        s[0] += row["pixels"] * row["spp"]
+        s[1] += row["alone_ms"] * 1e-3
+    rate = {str(int(round(k ** 0.5))): v[0] / v[1] for k, v in sorted(by.items())}
+    best = max(rate.values())
+    return {"value": total / t_batch, "unit": "path samples/s", "requests": len(reqs), "path_samples": total,
+            "batched_s": t_batch, "plain_s": t_plain, "plain_value": total / t_plain,
+            "batch_over_plain": t_plain / t_batch, "identical_images": same,
+            "plain_rate_by_film_edge": rate, "bound_at_best_plain_rate": t_plain / (total / best),
+            "rays_per_s": (st["extension_rays"] + st["shadow_rays"]) / t_batch,
+            "schedule": ["fifo", "shortest_first"][schedule]}
 
 
-def bench_serve(a):
-    """Poisson arrivals over three resident scenes: hk_server vs FCFS plain renders."""
+def serve_leg(ctx, gss, trace, budget_gb, detail=""):
+    """Poisson arrivals through hk_server vs a FCFS queue of plain renders."""
     import threading
     import hikari_amd as hk
-    from hikari_amd import scenes, workloads
-    box, sphere, field = scenes.cbox(), scenes.displaced_sphere(alpha=a.alpha), scenes.heightfield()
-    ctx = hk.Context(0)
-    ctx.set_option("schedule", a.schedule)
-    gss = [ctx.scene(box), ctx.scene(sphere), ctx.scene(field)]
-    trace = workloads.serve_trace([box, None, field], n=a.requests or 40, qps=a.qps)
     total = sum(r.width * r.height * r.spp for _, _, r in trace)
-    print(f"# {len(trace)} requests over {trace[-1][0]:.1f} s, {total / 1e9:.2f} G path samples",
-          file=sys.stderr, flush=True)
-    for g in gss:
-        g.render(1)
     # plain: each request alone; first come first served on one queue
     service = []
     for _, sid, r in trace:
@@ -161,12 +171,11 @@ def bench_serve(a):
         service.append(time.perf_counter() - t0)
     t_free, lat_plain = 0.0, []
     for (arr, _, _), sv in zip(trace, service):
-        start = max(arr, t_free)
-        t_free = start + sv
+        t_free = max(arr, t_free) + sv
         lat_plain.append(t_free - arr)
     span_plain = t_free - trace[0][0]
-    print(f"# plain FCFS: busy {sum(service):.1f} s, span {span_plain:.1f} s", file=sys.stderr, flush=True)
-    srv = hk.Server(ctx, hbm_budget=int(a.budget_gb * 2**30))
+    log(f"serve: plain FCFS busy {sum(service):.2f} s, span {span_plain:.2f} s")
+    srv = hk.Server(ctx, hbm_budget=int(budget_gb * 2 ** 30))
     for g in gss:
         srv.add_scene(g)
     tickets = [None] * len(trace)
@@ -189,31 +198,68 @@ def bench_serve(a):
         lat.append(res["latency_ms"] * 1e-3)
         t_last = max(t_last, time.perf_counter() - t_start)
         arr, sid, r = trace[i]
-        rows.append({"scene": sid, "pixels": r.width * r.height, "spp": r.spp, "arrival_s": arr,
+        rows.append({"scene": sid, "pixels": r.width * r.height, "spp": r.spp, "arrival_s": arr, "lane": res["lane"],
                      "alone_ms": service[i] * 1e3, "server_ms": res["latency_ms"], "queue_ms": res["queue_ms"],
                      "submit_ms": res["submit_ms"], "plain_fcfs_ms": lat_plain[i] * 1e3})
     th.join()
     span = t_last - trace[0][0]
     stats = srv.stats()
     srv.close()
-    out = {
-        "metric": "path samples/sec at matched relMSE vs plain GPU path; p50/p99 render latency",
-        "value": total / span, "unit": "path samples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": span * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seeded procedural scenes, Poisson request trace)",
-        "config": {"workload": f"serving {len(trace)} requests at {a.qps} req/s over 3 resident scenes "
-                               "(box 5k tris, displaced sphere 1.3M, heightfield 2.1M); res {64..1024}^2, "
-                               "spp {16..1024}", "hbm_budget_gb": a.budget_gb, "path_samples": total,
-                   "schedule": ["fifo", "shortest_first"][a.schedule]},
-        "latency_ms": {"p50": _pct(lat, 50) * 1e3, "p99": _pct(lat, 99) * 1e3},
-        "plain_fcfs": {"value": total / span_plain, "latency_p50_ms": _pct(lat_plain, 50) * 1e3,
-                       "latency_p99_ms": _pct(lat_plain, 99) * 1e3, "busy_s": sum(service)},
-        "server": {"bounces": stats["bounces"], "shared_bounces": stats["shared_bounces"],
-                   "max_in_flight": stats["max_in_flight"], "fixed_gb": stats["fixed_bytes"] / 2**30},
-    }
-    if a.detail:
-        with open(a.detail, "w") as fh:
+    if detail:
+        with open(detail, "w") as fh:
             json.dump(rows, fh)
+    log(f"serve: server span {span:.2f} s, p50 {_pct(lat, 50) * 1e3:.0f} ms, p99 {_pct(lat, 99) * 1e3:.0f} ms "
+        f"(plain FCFS p50 {_pct(lat_plain, 50) * 1e3:.0f} ms, p99 {_pct(lat_plain, 99) * 1e3:.0f} ms)")
+    return {"value": total / span, "unit": "path samples/s", "requests": len(trace), "path_samples": total,
+            "qps": len(trace) / trace[-1][0], "span_s": span,
+            "latency_ms": {"p50": _pct(lat, 50) * 1e3, "p99": _pct(lat, 99) * 1e3},
+            "plain_fcfs": {"value": total / span_plain, "span_s": span_plain, "busy_s": sum(service),
+                           "latency_ms": {"p50": _pct(lat_plain, 50) * 1e3, "p99": _pct(lat_plain, 99) * 1e3}},
+            "served_over_plain": span_plain / span,
+            "express_requests": sum(1 for x in rows if x["lane"] == 1),
+            "server": {"bounces": stats["bounces"], "shared_bounces": stats["shared_bounces"],
+                       "max_in_flight": stats["max_in_flight"], "fixed_gb": stats["fixed_bytes"] / 2 ** 30,
+                       "hbm_budget_gb": budget_gb}}
+
+
+def serve_scenes(sphere):
+    from hikari_amd import scenes
+    return scenes.cbox(), sphere, scenes.heightfield()
+
+
+def bench_batch(a):
+    import hikari_amd as hk
+    from hikari_amd import scenes
+    ctx = hk.Context(0)
+    gs = ctx.scene(scenes.displaced_sphere(alpha=a.alpha))
+    gs.render(4)  # warm-up (kernels loaded, workspace allocated)
+    b = batch_leg(gs, a.requests or 64, a.schedule, a.detail)
+    out = {"metric": METRIC, "value": b["value"], "unit": "path samples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": b["batched_s"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (seeded procedural scene and request mix)",
+           "config": {"workload": f"{b['requests']} batched requests over the displaced icosphere-8 scene, res "
+                                  "{64..1024}^2, spp {16..1024}, camera on a radius-4 sphere"},
+           "batch": b}
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(a):
+    import hikari_amd as hk
+    from hikari_amd import scenes, workloads
+    box, sphere, field = serve_scenes(scenes.displaced_sphere(alpha=a.alpha))
+    ctx = hk.Context(0)
+    gss = [ctx.scene(box), ctx.scene(sphere), ctx.scene(field)]
+    for g in gss:
+        g.render(1)
+    trace = workloads.serve_trace([box, None, field], n=a.requests or 40, qps=a.qps)
+    s = serve_leg(ctx, gss, trace, a.budget_gb, a.detail)
+    out = {"metric": METRIC, "value": s["value"], "unit": "path samples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": s["span_s"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (seeded procedural scenes, Poisson request trace)",
+           "config": {"workload": f"serving {len(trace)} requests at {a.qps} req/s over 3 resident scenes "
+                                  "(box 5k tris, displaced sphere 1.3M, heightfield 2.1M); res {64..1024}^2, "
+                                  "spp {16..1024}"},
+           "serve": s}
     print(json.dumps(out), flush=True)
 
 
@@ -263,6 +309,24 @@ def bench_accel(a):
     print(json.dumps(line), flush=True)
 
 
+def relmse_leg(scene_fn, alpha):
+    """The workload's scene on a reduced film (64 x 48, 16 spp): GPU vs the
+    oracle at equal per-path streams (same random paths), so that the
+    benchmark's kernels and scene data are checked for matched quality on
+    every run."""
+    import hikari_amd as hk
+    from oracle import pyoracle as O
+    sc = scene_fn(alpha, 64, 48)
+    ctx = hk.Context(0)
+    img, _ = ctx.scene(sc).render(16, seed=77)
+    ctx.close()
+    ref, _, _ = O.render(sc, req_seed=77, spp=(0, 16))
+    lum = np.array([0.212671, 0.715160, 0.072169])
+    rm = float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 1e-2)))
+    return {"film": "64x48", "spp": 16, "relmse_vs_oracle": rm,
+            "mean_ratio": float((img @ lum).mean() / (ref @ lum).mean())}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -276,7 +340,7 @@ def main():
     if a.workload == "accel":
         return bench_accel(a)
     import hikari_amd as hk
-    from hikari_amd import scenes
+    from hikari_amd import scenes, workloads
 
     t = time.time()
     if a.workload == "c2":
@@ -285,11 +349,17 @@ def main():
             a.spp = 512
         workload = ("1024^2 heightfield (2,093,058 tris) in 8 patches of 5 materials (rough dielectric, rough "
                     f"plastic, coated rough conductor, dielectric, diffuse), 2 quad lights + env, 1280x720, {a.spp} spp")
+
+        def scene_fn(alpha, w, h):
+            return scenes.heightfield(width=w, height=h)
     else:
         scene = scenes.displaced_sphere(alpha=a.alpha)
         workload = ("displaced icosphere-8 (1,310,720 tris), GGX rough conductor alpha "
                     f"{a.alpha}, env map 512x256, 1024x768, {a.spp} spp, plain gpuPathMIS")
-    print(f"# scene built in {time.time() - t:.1f} s: {scene.n_triangles} triangles", file=sys.stderr, flush=True)
+
+        def scene_fn(alpha, w, h):
+            return scenes.displaced_sphere(alpha=alpha, width=w, height=h)
+    log(f"scene built in {time.time() - t:.1f} s: {scene.n_triangles} triangles")
     ctx = hk.Context(0)
     if a.max_paths:
         ctx.set_option("max_paths", a.max_paths)
@@ -302,19 +372,17 @@ def main():
     t = time.time()
     gs = ctx.scene(scene)
     upload_s = time.time() - t
-    print(f"# scene resident in HBM ({gs.device_bytes() / 2**20:.0f} MiB) after {upload_s:.2f} s "
-          f"({a.bvh_builder} BVH build)", file=sys.stderr, flush=True)
+    log(f"scene resident in HBM ({gs.device_bytes() / 2**20:.0f} MiB) after {upload_s:.2f} s "
+        f"({a.bvh_builder} BVH build)")
     tune = None
     if a.autotune and not a.max_paths:
         tune = gs.autotune(a.spp, seed=999)
-        print(f"# autotune: max_paths {tune['max_paths']}, fetch_batch {tune['fetch_batch']}, refill "
-              f"{tune['refill']}/{tune['refill_shadow']}, tile {tune['tile']}, grid {tune['grid']} in "
-              f"{tune['total_ms']:.0f} ms: "
-              + ", ".join(f"({t['max_paths']}, {t['fetch_batch']}, {t['refill']}/{t['refill_shadow']}, {t['tile']}, "
-                          f"{t['grid']}) {t['samples_per_s']:.3e}" for t in tune["tried"]), file=sys.stderr, flush=True)
+        log(f"autotune: max_paths {tune['max_paths']}, fetch_batch {tune['fetch_batch']}, refill "
+            f"{tune['refill']}/{tune['refill_shadow']}, tile {tune['tile']}, grid {tune['grid']} in "
+            f"{tune['total_ms']:.0f} ms")
     for i in range(a.warmup):
         _, st = gs.render(a.spp, seed=1000 + i)
-        print(f"# warmup {i}: {st['gpu_ms']:.1f} ms", file=sys.stderr, flush=True)
+        log(f"warmup {i}: {st['gpu_ms']:.1f} ms")
     lat, tot = [], {"paths": 0, "extension_rays": 0, "shadow_rays": 0, "trace_ms": 0.0, "trace_launches": 0,
                     "gpu_ms": 0.0, "node_visits": 0, "triangle_tests": 0, "bounces_launched": 0, "graph_bounces": 0}
     t_all = time.perf_counter()
@@ -324,8 +392,7 @@ def main():
         lat.append(time.perf_counter() - t0)
         for k in tot:
             tot[k] += st[k]
-        print(f"# step {i}: {lat[-1] * 1e3:.1f} ms, {st['paths'] / lat[-1]:.3e} paths/s", file=sys.stderr,
-              flush=True)
+        log(f"step {i}: {lat[-1] * 1e3:.1f} ms, {st['paths'] / lat[-1]:.3e} paths/s")
     wall = time.perf_counter() - t_all
     value = tot["paths"] / wall
     rays = tot["extension_rays"] + tot["shadow_rays"]
@@ -337,23 +404,26 @@ def main():
     ctx.set_option("profile", 0)
     ext_launches = max(1, pst["trace_launches"])
     ext_avg_ms = pst["trace_ms"] / ext_launches
-    ext_bytes_per_launch = EXTEND_B * pst["extension_rays"] / ext_launches
-    achieved = ext_bytes_per_launch / (ext_avg_ms * 1e-3) / 1e9 if ext_avg_ms > 0 else 0.0
+    rays_per_launch = pst["extension_rays"] / ext_launches
+    achieved = EXTEND_B * rays_per_launch / (ext_avg_ms * 1e-3) / 1e9 if ext_avg_ms > 0 else 0.0
     render_bytes = SEG_B * tot["extension_rays"] + SHADOW_B * tot["shadow_rays"] + PATH_B * tot["paths"]
-    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # measurement (bytes per ray, scaled by a plain copy's counter readings)
-    traffic = None
+    roofline = {"bound": "hbm", "kernel": "k_extend", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launch_ms": ext_avg_ms,
+                "rays_per_launch": rays_per_launch,
+                "timing": "HIP events around each k_extend launch of one extra (eager) render"}
+    # Figures that need rocprofv3 counter passes (HBM bytes, BVH work per
+    # ray, VALU issue) come from the committed summaries of those passes,
+    # labelled with the commit they were measured on; traffic is scaled to
+    # this run's rays per launch.
+    fp = {}
     tp = os.path.join(ROOT, "profiles", "k_extend_traffic.json")
     if os.path.exists(tp):
         with open(tp) as fh:
-            per_ray = json.load(fh).get("hbm_bytes_per_ray")
-        if per_ray:
-            traffic = per_ray * pst["extension_rays"] / ext_launches
-    # Secondary "effective" figure: the algorithmic bytes plus every BVH node
-    # and triangle record a ray touches (counted once by the traversal-stats
-    # build); L1 / L2 / Infinity-Cache hits serve most of these, so it may
-    # exceed the HBM peak and is reported only.
-    effective = None
+            tj = json.load(fh)
+        if tj.get("hbm_bytes_per_ray"):
+            roofline["traffic"] = tj["hbm_bytes_per_ray"] * rays_per_launch
+            fp["traffic"] = {"hbm_bytes_per_ray": tj["hbm_bytes_per_ray"], "commit": tj.get("commit"),
+                             "source": "profiles/k_extend_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE)"}
     tv = os.path.join(ROOT, "profiles", "k_extend_traversal.json")
     if os.path.exists(tv) and ext_avg_ms > 0:
         with open(tv) as fh:
@@ -362,27 +432,24 @@ def main():
         if per:
             b_ray = (EXTEND_B + tj["node_bytes"] * per["node_visits_per_ray"]
                      + tj["triangle_bytes"] * per["triangle_tests_per_ray"])
-            eff = b_ray * pst["extension_rays"] / ext_launches / (ext_avg_ms * 1e-3) / 1e9
-            effective = {"bytes_per_ray": b_ray, "achieved": eff, "frac": eff / HBM_PEAK_GBS}
-    # The bound k_extend actually meets is VALU issue (wave64 instructions on
-    # 16-lane SIMDs), measured by a committed rocprofv3 counter pass at the
-    # benchmark size; reported beside the HBM figure.
-    issue = None
+            eff = b_ray * rays_per_launch / (ext_avg_ms * 1e-3) / 1e9
+            fp["effective"] = {"bytes_per_ray": b_ray, "achieved": eff, "frac": eff / HBM_PEAK_GBS,
+                               "commit": tj.get("commit"), "source": "profiles/k_extend_traversal.json"}
     vp = os.path.join(ROOT, "profiles", "valu_issue_c1_c2.json")
     if os.path.exists(vp):
         with open(vp) as fh:
-            vj = json.load(fh).get("c2" if a.workload == "c2" else "c1", {}).get("k_extend")
-        if vj:
-            issue = {"valu_issue_busy": vj["valu_issue_busy"], "lane_use": vj["lane_use"],
-                     "source": "profiles/valu_issue_c1_c2.json (rocprofv3 SQ counters, same workload)"}
+            vj = json.load(fh)
+        ke = vj.get("c2" if a.workload == "c2" else "c1", {}).get("k_extend")
+        if ke:
+            fp["issue"] = {"valu_issue_busy": ke["valu_issue_busy"], "lane_use": ke["lane_use"],
+                           "commit": vj.get("commit"), "source": "profiles/valu_issue_c1_c2.json (rocprofv3 SQ)"}
+    roofline["from_profiles"] = fp or None
     out = {
-        "metric": "path samples/sec at matched relMSE vs plain GPU path; p50/p99 render latency",
-        "value": value, "unit": "path samples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": workload,
-                   "spp": a.spp, "width": scene.width, "height": scene.height, "triangles": scene.n_triangles,
-                   "bvh_builder": a.bvh_builder},
+        "metric": METRIC, "value": value, "unit": "path samples/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes and request mixes)",
+        "config": {"workload": workload, "spp": a.spp, "width": scene.width, "height": scene.height,
+                   "triangles": scene.n_triangles, "bvh_builder": a.bvh_builder},
         "scene_create_s": upload_s,
         "autotune": None if tune is None else {k: tune[k] for k in ("max_paths", "fetch_batch", "refill",
                                                                     "refill_shadow", "grid", "tile", "total_ms")},
@@ -390,20 +457,29 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "rays_per_s": rays / wall,
         "rays_per_path": rays / max(1, tot["paths"]),
-        "roofline": {"bound": "hbm", "kernel": "k_extend", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "launch_ms": ext_avg_ms, "rays_per_launch": pst["extension_rays"] / ext_launches,
-                     "timing": "HIP events around each k_extend launch of one extra (eager) render",
-                     "effective": effective, "issue": issue},
+        "roofline": roofline,
         "render_roofline_frac": render_bytes / wall / (HBM_PEAK_GBS * 1e9),
     }
     if tot["node_visits"]:  # builds with traversal counters only
         out["traversal"] = {"node_visits_per_ray": tot["node_visits"] / max(1, rays),
                             "triangle_tests_per_ray": tot["triangle_tests"] / max(1, rays)}
+    t = time.time()
+    out["quality"] = relmse_leg(scene_fn, a.alpha)
+    log(f"relMSE vs oracle on the reduced film: {out['quality']['relmse_vs_oracle']:.2e} "
+        f"({time.time() - t:.1f} s)")
+    if a.workload == "c1" and not a.no_batch:
+        out["batch"] = batch_leg(gs, a.requests or 64, a.schedule, a.detail)
+    if a.workload == "c1" and not a.no_serve:
+        box, sphere, field = serve_scenes(scene)
+        gss = [ctx.scene(box), gs, ctx.scene(field)]
+        for g in gss:
+            g.render(1)
+        trace = workloads.serve_trace([box, None, field], n=40, qps=a.qps)
+        out["serve"] = serve_leg(ctx, gss, trace, a.budget_gb)
     if not a.no_cpu_baseline:
         t = time.time()
         out["cpu_baseline"] = cpu_baseline(scene, a.cpu_seconds)
-        print(f"# cpu baseline in {time.time() - t:.1f} s", file=sys.stderr, flush=True)
+        log(f"cpu baseline in {time.time() - t:.1f} s")
     print(json.dumps(out), flush=True)
 
 

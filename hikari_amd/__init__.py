@@ -62,7 +62,7 @@ class RenderStats(C.Structure):
 
 class ServerResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("queue_ms", C.c_double), ("latency_ms", C.c_double), ("paths", C.c_uint64),
-                ("submit_ms", C.c_double)]
+                ("submit_ms", C.c_double), ("lane", C.c_int32)]
 
 
 class ServerStats(C.Structure):

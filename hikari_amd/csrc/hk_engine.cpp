@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 
 namespace hk {
@@ -503,6 +504,22 @@ void Engine::finalize(RequestRec *r) {
 
 void Engine::run() {
   while (busy()) bounce();
+}
+
+uint64_t Engine::unstarted() const {
+  uint64_t n = 0;
+  if (free_reqs_.size() == kMaxRequests) return 0;
+  for (const RequestRec *r : reqs_by_slot_)
+    if (r) n += r->left();
+  return n;
+}
+
+double Engine::oldest_submit() const {
+  double t = HUGE_VAL;
+  if (free_reqs_.size() == kMaxRequests) return t;
+  for (const RequestRec *r : reqs_by_slot_)
+    if (r) t = std::min(t, r->t_submit);
+  return t;
 }
 
 void Engine::reset_stats() {

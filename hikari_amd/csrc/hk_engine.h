@@ -75,6 +75,19 @@ class Engine {
   // Request slots still free (submit needs one per request in flight).
   uint32_t free_request_slots() const { return uint32_t(free_reqs_.size()); }
   uint32_t requests_in_flight() const { return kMaxRequests - uint32_t(free_reqs_.size()); }
+  // Non-blocking progress: a launched bounce whose work has finished on the
+  // device (collect() then returns without waiting).
+  bool launched() const { return launched_; }
+  bool ready() const { return launched_ && hipStreamQuery(st_) == hipSuccess; }
+  // Samples of submitted requests not yet handed to k_regen, and whether
+  // this engine's next bounces keep at least half its wavefront busy.
+  uint64_t unstarted() const;
+  bool saturating() const { return busy() && 2 * (uint64_t(in_flight_) + unstarted()) >= pool_; }
+  // Submit time (engine clock) of the oldest request in flight; +inf if none.
+  double oldest_submit() const;
+  // Admission / start order: true the fewest samples left first, false
+  // first come first served.
+  void set_shortest_first(bool v) { shortest_first_ = v; }
   bool busy() const { return !pending_.empty() || !active_.empty() || in_flight_ > 0; }
   // One bounce: admit, start new samples, extend, shade, shadow, collect
   // finished chunks, resolve them, finalize finished requests.  Returns the

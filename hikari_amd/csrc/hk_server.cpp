@@ -1,12 +1,28 @@
 // hk_server.cpp -- request loop with continuous batching over resident scenes.
 //
-// One worker thread owns one Engine per scene, each on its own HIP stream
-// (default priority).  Every round it admits arrived requests while the HBM
-// budget allows (FIFO), issues one bounce on every busy engine, then collects
-// them, so the scenes' kernels overlap on the device and a request that
-// arrives while others run joins their very next bounce.  Results equal the
-// same requests rendered alone with hk_render on a context with the same
-// max_paths / max_samples, bit for bit (see hk_engine.h).
+// Every resident scene has two lanes, each an Engine with its own workspace
+// and HIP stream (default priority):
+//   * the bulk lane: the context's wavefront (max_paths), a sample buffer of
+//     4 x max_samples holding up to four chunks of max_samples;
+//   * the express lane: a wavefront of max_paths / 16 and a buffer of
+//     max_samples / 8, for requests of at most half that many samples,
+//     fewest samples left first.
+// Bulk requests run one at a time, in arrival order over all scenes, as a
+// first-come-first-served queue of plain renders would run them.  A request
+// finishes only when its longest Russian-roulette path does; sharing a
+// wavefront (or the device) with the next request's full bounces would stretch
+// that tail from a few hundred tiny bounces into as many full ones, so large
+// requests never overlap each other.  Small requests pass them on the
+// express lanes, whose bounces run beside the bulk lane's.
+// One worker thread drives all lanes without a common barrier: it collects a
+// lane's bounce as soon as the device has finished it (hipStreamQuery) and
+// launches the lane's next one, so each lane advances at its own pace and the
+// lanes' kernels overlap on the device.  A request that arrives while others
+// run is admitted (HBM budget permitting); an express request's paths join
+// its lane's next bounce.
+// Results equal the same requests rendered alone with hk_render on a context
+// with the same options, bit for bit: a film depends neither on the wavefront
+// size nor on the chunking (see hk_engine.h).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,13 +48,16 @@ extern thread_local std::string g_create_error;
 
 namespace {
 
-struct ServerScene {
-  hk_scene *scene = nullptr;
+enum { kBulk = 0, kExpress = 1 };
+
+struct Lane {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   Workspace ws;
   std::unique_ptr<Engine> eng;
-  ~ServerScene() {
+  uint32_t chunk_cap = 0;      // samples per chunk at most (prepare_request)
+  uint64_t max_request = 0;    // express: largest request (samples) it takes
+  ~Lane() {
     eng.reset();
     ws.release();
     if (ev0) (void)hipEventDestroy(ev0);
@@ -47,14 +66,19 @@ struct ServerScene {
   }
 };
 
+struct ServerScene {
+  hk_scene *scene = nullptr;
+  std::unique_ptr<Lane> lane[2];
+};
+
 struct Ticket {
   std::unique_ptr<RequestRec> rec;
   int32_t scene = 0;
+  int lane = kBulk;
   int status = HK_OK;
   std::string err;
   bool done = false;
   bool admitted = false;  // its film counts against the budget
-  double t_arrive = 0;
 };
 
 }  // namespace
@@ -99,22 +123,41 @@ static void stop_worker(hk_server *srv, int code, const std::string &msg) {
   srv->cv_done.notify_all();
 }
 
+static bool any_lane_active(const hk_server *srv) {
+  for (auto &sc : srv->scenes)
+    for (auto &l : sc->lane)
+      if (l->eng->busy() || l->eng->launched()) return true;
+  return false;
+}
+
+static bool bulk_idle(const hk_server *srv) {
+  for (auto &sc : srv->scenes) {
+    const Engine &e = *sc->lane[kBulk]->eng;
+    if (e.busy() || e.launched()) return false;
+  }
+  return true;
+}
+
 static void worker_main(hk_server *srv) {
   (void)hipSetDevice(srv->ctx->device);
+  // bulk requests admitted (HBM budget) and waiting for their turn, in
+  // arrival order over all scenes; only the worker touches it
+  std::deque<Ticket *> bulk_wait;
   for (;;) {
-    std::vector<std::pair<uint64_t, Ticket *>> admitted;
+    std::vector<Ticket *> admitted;
     {
       std::unique_lock<std::mutex> lk(srv->mu);
       // HBM-budget admission, first come first served.  A request whose
-      // scene has no free request slot stays queued (later requests for
-      // other scenes may pass it); one whose film does not fit the budget
+      // lane has no free request slot stays queued (later requests for
+      // other lanes may pass it); one whose film does not fit the budget
       // waits, with everything behind it, until running requests free memory.
-      std::vector<uint32_t> slots(srv->scenes.size());
-      for (size_t k = 0; k < slots.size(); k++) slots[k] = srv->scenes[k]->eng->free_request_slots();
+      std::vector<uint32_t> slots(srv->scenes.size() * 2);
+      for (size_t k = 0; k < srv->scenes.size(); k++)
+        for (int l = 0; l < 2; l++) slots[2 * k + l] = srv->scenes[k]->lane[l]->eng->free_request_slots();
       for (auto it = srv->arrivals.begin(); it != srv->arrivals.end();) {
-        uint64_t id = *it;
-        Ticket &t = srv->tickets[id];
-        if (slots[size_t(t.scene)] == 0) {
+        Ticket &t = srv->tickets[*it];
+        uint32_t &free = slots[2 * size_t(t.scene) + t.lane];
+        if (free == 0) {
           ++it;
           continue;
         }
@@ -130,30 +173,38 @@ static void worker_main(hk_server *srv) {
         }
         srv->film_bytes += need;
         t.admitted = true;
-        slots[size_t(t.scene)]--;
-        admitted.push_back({id, &t});
+        free--;
+        admitted.push_back(&t);
       }
-      bool any = !admitted.empty();
-      for (auto &sc : srv->scenes) any = any || sc->eng->busy();
-      if (!any) {
-        // (nothing busy means no request is in flight, so every queued
+      if (admitted.empty() && bulk_wait.empty() && !any_lane_active(srv)) {
+        // (nothing active means no request is in flight, so every queued
         // request would have been admitted or refused above)
         if (srv->stop) return;
         srv->cv_work.wait(lk);
         continue;
       }
     }
-    // A request the engine refuses (e.g. no device memory for its film)
-    // fails alone; the engine is left as it was.  Only the worker finishes
-    // tickets, so the accepted ones stay valid until this round ends.
+    // Express requests go to their lane's engine at once; bulk requests one
+    // at a time, the next when no bulk lane has one in flight.  A request the
+    // engine refuses (e.g. no device memory for its film) fails alone; the
+    // engine is left as it was.  Only the worker finishes tickets, so the
+    // accepted ones stay valid until this round ends.
+    std::vector<Ticket *> to_submit;
+    for (Ticket *tp : admitted) {
+      if (tp->lane == kBulk) bulk_wait.push_back(tp);
+      else to_submit.push_back(tp);
+    }
+    if (!bulk_wait.empty() && bulk_idle(srv)) {
+      to_submit.push_back(bulk_wait.front());
+      bulk_wait.pop_front();
+    }
     std::vector<Ticket *> accepted;
-    for (auto &a : admitted) {
-      Ticket &t = *a.second;
-      ServerScene &sc = *srv->scenes[size_t(t.scene)];
+    for (Ticket *tp : to_submit) {
+      Ticket &t = *tp;
       int code = HK_OK;
       std::string msg;
       try {
-        sc.eng->submit(t.rec.get());
+        srv->scenes[size_t(t.scene)]->lane[t.lane]->eng->submit(t.rec.get());
       } catch (const HkError &e) {
         code = e.code;
         msg = e.msg;
@@ -170,31 +221,46 @@ static void worker_main(hk_server *srv) {
       srv->cv_done.notify_all();
     }
     try {
-      uint64_t in_flight = 0;
-      for (auto &sc : srv->scenes) in_flight += sc->eng->requests_in_flight();
-      std::vector<ServerScene *> launched;
-      for (auto &sc : srv->scenes)
-        if (sc->eng->busy() && sc->eng->launch()) launched.push_back(sc.get());
+      bool progressed = !to_submit.empty();
       std::vector<RequestRec *> finished;
-      for (ServerScene *sc : launched) {
-        std::vector<RequestRec *> f = sc->eng->collect();
-        finished.insert(finished.end(), f.begin(), f.end());
-      }
+      // bounces the device has finished
+      for (auto &sc : srv->scenes)
+        for (auto &l : sc->lane)
+          if (l->eng->ready()) {
+            std::vector<RequestRec *> f = l->eng->collect();
+            finished.insert(finished.end(), f.begin(), f.end());
+            progressed = true;
+          }
+      // next bounces of every lane with work
+      uint64_t in_flight = 0;
+      for (auto &sc : srv->scenes)
+        for (auto &l : sc->lane) in_flight += l->eng->requests_in_flight();
+      uint64_t launches = 0;
+      for (auto &sc : srv->scenes)
+        for (auto &l : sc->lane) {
+          Engine &e = *l->eng;
+          if (!e.launched() && e.busy() && e.launch()) launches++;
+        }
+      progressed = progressed || launches > 0;
       // requests with no samples finish at submit
       for (Ticket *t : accepted)
         if (t->rec->done && std::find(finished.begin(), finished.end(), t->rec.get()) == finished.end())
           finished.push_back(t->rec.get());
-      std::lock_guard<std::mutex> lk(srv->mu);
-      srv->st.bounces += launched.size();
-      if (in_flight >= 2) srv->st.shared_bounces += launched.size();
-      srv->st.max_in_flight = std::max<uint64_t>(srv->st.max_in_flight, in_flight);
-      for (RequestRec *r : finished) {
-        Ticket &t = srv->tickets[r->id];
-        finish_ticket(srv, t, HK_OK, "");
-        srv->st.completed++;
-        srv->st.paths += uint64_t(r->npix) * r->spp;
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->st.bounces += launches;
+        if (in_flight >= 2) srv->st.shared_bounces += launches;
+        srv->st.max_in_flight = std::max<uint64_t>(srv->st.max_in_flight, in_flight);
+        for (RequestRec *r : finished) {
+          Ticket &t = srv->tickets[r->id];
+          finish_ticket(srv, t, HK_OK, "");
+          srv->st.completed++;
+          srv->st.paths += uint64_t(r->npix) * r->spp;
+        }
+        if (!finished.empty()) srv->cv_done.notify_all();
       }
-      if (!finished.empty()) srv->cv_done.notify_all();
+      // every launched bounce is still running: wait a little for one
+      if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     } catch (const HkError &e) {
       stop_worker(srv, e.code, e.msg);
       return;
@@ -219,6 +285,19 @@ static void worker_main(hk_server *srv) {
     return HK_ERR_INVALID;               \
   }
 
+static std::unique_ptr<Lane> make_lane(hk_ctx *ctx, hk_scene *scene, uint32_t paths, uint32_t samples,
+                                       uint32_t chunk_cap, bool shortest_first) {
+  std::unique_ptr<Lane> l(new Lane());
+  hip_check(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreate(&l->ev0), "hipEventCreate");
+  hip_check(hipEventCreate(&l->ev1), "hipEventCreate");
+  l->chunk_cap = chunk_cap;
+  l->eng.reset(new Engine(scene, l->ws, l->stream, paths, samples, chunk_cap, false, l->ev0, l->ev1));
+  l->eng->set_shortest_first(shortest_first);
+  hip_check(hipStreamSynchronize(l->stream), "workspace");
+  return l;
+}
+
 extern "C" {
 
 int hk_server_create(hk_ctx *ctx, uint64_t hbm_budget, hk_server **out) {
@@ -241,16 +320,16 @@ int hk_server_add_scene(hk_server *srv, hk_scene *scene, int32_t *scene_id) {
   if (scene->ctx != srv->ctx) throw HkError(HK_ERR_INVALID, "scene belongs to another context");
   std::lock_guard<std::mutex> lk(srv->mu);
   if (srv->started) throw HkError(HK_ERR_INVALID, "scenes must be added before the first submit");
+  hip_check(hipSetDevice(ctx->device), "hipSetDevice");
   std::unique_ptr<ServerScene> sc(new ServerScene());
   sc->scene = scene;
-  hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-  hip_check(hipStreamCreateWithFlags(&sc->stream, hipStreamNonBlocking), "hipStreamCreate");
-  hip_check(hipEventCreate(&sc->ev0), "hipEventCreate");
-  hip_check(hipEventCreate(&sc->ev1), "hipEventCreate");
-  sc->eng.reset(new Engine(scene, sc->ws, sc->stream, ctx->max_paths, ctx->max_samples, ctx->max_samples, false,
-                           sc->ev0, sc->ev1));
-  hip_check(hipStreamSynchronize(sc->stream), "workspace");
-  uint64_t ws_bytes = sc->ws.device_bytes();
+  uint64_t bulk_samples = std::min<uint64_t>(uint64_t(ctx->max_samples) * 4, 0x7fffffffu);
+  sc->lane[kBulk] = make_lane(ctx, scene, ctx->max_paths, uint32_t(bulk_samples), ctx->max_samples, false);
+  uint32_t ex_samples = std::max<uint32_t>(1, ctx->max_samples / 8);
+  sc->lane[kExpress] = make_lane(ctx, scene, std::max<uint32_t>(1, ctx->max_paths / 16), ex_samples, ex_samples,
+                                 true);
+  sc->lane[kExpress]->max_request = ex_samples / 2;
+  uint64_t ws_bytes = sc->lane[kBulk]->ws.device_bytes() + sc->lane[kExpress]->ws.device_bytes();
   if (srv->budget && srv->fixed_bytes + ws_bytes > srv->budget)
     throw HkError(HK_ERR_BUDGET, "scene workspaces (%llu B) exceed the HBM budget (%llu B)",
                   (unsigned long long)(srv->fixed_bytes + ws_bytes), (unsigned long long)srv->budget);
@@ -268,15 +347,22 @@ int hk_server_submit(hk_server *srv, int32_t scene_id, const hk_request *req, fl
   std::unique_lock<std::mutex> lk(srv->mu);
   if (!srv->fatal.empty()) throw HkError(HK_ERR_HIP, "server stopped: %s", srv->fatal.c_str());
   if (scene_id < 0 || size_t(scene_id) >= srv->scenes.size()) throw HkError(HK_ERR_INVALID, "bad scene id %d", scene_id);
-  std::unique_ptr<RequestRec> r = prepare_request(srv->scenes[size_t(scene_id)]->scene, *req, out_rgb,
-                                                  ctx->max_samples);
+  ServerScene &sc = *srv->scenes[size_t(scene_id)];
+  // the lane: express for requests of at most its limit whose film fits its
+  // chunks, bulk otherwise
+  const hk_scene_desc &d = sc.scene->desc;
+  uint64_t w = uint64_t(req->use_scene_camera ? d.width : req->width);
+  uint64_t h = uint64_t(req->use_scene_camera ? d.height : req->height);
+  const Lane &ex = *sc.lane[kExpress];
+  int lane = (w * h * req->spp <= ex.max_request && w * h <= ex.chunk_cap) ? kExpress : kBulk;
+  std::unique_ptr<RequestRec> r = prepare_request(sc.scene, *req, out_rgb, sc.lane[lane]->chunk_cap);
   uint64_t id = srv->next_ticket++;
   r->id = id;
   r->t_submit = clock_ms();
   Ticket &t = srv->tickets[id];
   t.rec = std::move(r);
   t.scene = scene_id;
-  t.t_arrive = t.rec->t_submit;
+  t.lane = lane;
   srv->arrivals.push_back(id);
   srv->st.submitted++;
   if (!srv->started) {
@@ -306,9 +392,10 @@ int hk_server_wait(hk_server *srv, uint64_t ticket, hk_server_result *res) {
     res->status = status;
     if (t.rec) {
       res->queue_ms = t.rec->t_admit >= 0 ? t.rec->t_admit - t.rec->t_submit : 0.0;
-      res->latency_ms = t.rec->t_done - t.rec->t_submit;
+      res->latency_ms = t.rec->t_done >= 0 ? t.rec->t_done - t.rec->t_submit : 0.0;
       res->submit_ms = t.rec->t_submit - srv->t0;
       res->paths = uint64_t(t.rec->npix) * t.rec->spp;
+      res->lane = t.lane;
     }
   }
   srv->tickets.erase(it);

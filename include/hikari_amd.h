@@ -218,15 +218,23 @@ int hk_render_batch(hk_scene *scene, const hk_request *reqs, uint32_t n, float *
 
 /* ---------------------------------------------------------------- serving
  * A request loop with continuous batching (no counterpart in the reference,
- * whose Render() handles one scene file per process).  A worker thread keeps
- * one wavefront per resident scene, each on its own stream; a request that
- * arrives while others run is admitted, first come first served, once its
- * film fits the HBM budget, and its paths join the next bounce of the
- * running ones.  hbm_budget bounds the server's path state and sample
- * buffers (fixed per scene, max_paths / max_samples of the context) plus
- * the films of admitted requests; 0 = unbounded.  Each result equals
- * hk_render of the same request on a context with the same options, bit for
- * bit.  Scenes must stay alive until hk_server_destroy. */
+ * whose Render() handles one scene file per process).  Each resident scene
+ * has two lanes, each a wavefront on its own stream: a bulk lane (the
+ * context's max_paths, a sample buffer of 4 x max_samples, first come first
+ * served) and an express lane (max_paths / 16 paths, max_samples / 8
+ * samples) for requests of at most max_samples / 16 path samples, fewest
+ * samples left first.  A worker thread advances every lane as soon as its
+ * previous bounce has finished on the device.  Bulk requests run one at a
+ * time in arrival order over all scenes (a request's Russian-roulette tail
+ * would otherwise be stretched by the next one's full bounces), while small
+ * requests pass them on the express lanes.  A request that arrives while
+ * others run is admitted, first come first served, once its film fits the
+ * HBM budget; an express request's paths join its lane's next bounce.
+ * hbm_budget bounds the server's path state and sample buffers
+ * (fixed per scene, both lanes) plus the films of admitted requests; 0 =
+ * unbounded.  Each result equals hk_render of the same request on a context
+ * with the same options, bit for bit.  Scenes must stay alive until
+ * hk_server_destroy. */
 typedef struct hk_server hk_server;
 typedef struct hk_server_result {
   int32_t status;          /* HK_OK or the request's error */
@@ -234,6 +242,7 @@ typedef struct hk_server_result {
   double latency_ms;       /* submit -> film in host memory */
   uint64_t paths;          /* path samples of the request */
   double submit_ms;        /* submit time on the server clock (ms since hk_server_create) */
+  int32_t lane;            /* 0 bulk, 1 express */
 } hk_server_result;
 typedef struct hk_server_stats {
   uint64_t submitted, completed, bounces, paths;
