@@ -22,6 +22,7 @@ ERRORS = {-1: "HK_ERR_INVALID", -2: "HK_ERR_HIP", -3: "HK_ERR_OOM", -4: "HK_ERR_
 
 EXPORTS = ["hk_ctx_create", "hk_ctx_destroy", "hk_last_error", "hk_version", "hk_ctx_sync", "hk_accel_build",
            "hk_accel_destroy", "hk_accel_get_info", "hk_accel_intersect", "hk_accel_intersect_device",
+           "hk_accel_intersect_device_noalloc",
            "hk_bsdf_batch", "hk_scene_create", "hk_scene_destroy", "hk_scene_device_bytes", "hk_render",
            "hk_render_pixel_samples", "hk_render_batch", "hk_set_option", "hk_server_create", "hk_server_add_scene",
            "hk_server_submit", "hk_server_wait", "hk_server_get_stats", "hk_server_destroy", "hk_autotune"]
@@ -94,6 +95,7 @@ def lib():
         L.hk_accel_get_info.argtypes = [P, C.POINTER(AccelInfo)]
         L.hk_accel_intersect.argtypes = [P, P, C.c_uint32, P, C.c_int]
         L.hk_accel_intersect_device.argtypes = [P, P, C.c_uint32, P, C.c_int]
+        L.hk_accel_intersect_device_noalloc.argtypes = [P, P, C.c_uint32, P, C.c_int]
         L.hk_bsdf_batch.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint32, P, P, P, P, P, P, P, P, P,
                                     P, P]
         L.hk_scene_create.argtypes = [P, C.POINTER(SceneDesc), C.POINTER(P)]
@@ -135,6 +137,10 @@ class Context:
 
     def set_option(self, name, value):
         self.check(lib().hk_set_option(self.h, name.encode(), int(value)))
+
+    def sync(self):
+        """Wait for the context stream (hk_ctx_sync)."""
+        self.check(lib().hk_ctx_sync(self.h))
 
     def close(self):
         if self.h:
@@ -201,6 +207,12 @@ class Accel:
         hits n x 4; runs on the context's stream without waiting for it."""
         self.ctx.check(lib().hk_accel_intersect_device(self.h, C.c_void_p(rays_ptr), int(n), C.c_void_p(hits_ptr),
                                                        int(any_hit)))
+
+    def intersect_device_noalloc(self, rays_ptr, hits_ptr, n, any_hit=False):
+        """intersect_device through the entry point that allocates nothing (its
+        fetch counter in the library's fixed per-device scratch)."""
+        self.ctx.check(lib().hk_accel_intersect_device_noalloc(self.h, C.c_void_p(rays_ptr), int(n),
+                                                               C.c_void_p(hits_ptr), int(any_hit)))
 
     def __del__(self):
         try:

@@ -75,6 +75,14 @@ hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, const DevRequest &
                           hipStream_t st);
 hipError_t launch_finalize(const void *accum, float *rgb, uint32_t n, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st);
+// The library's fixed per-device scratch (one 256-B block) used by the
+// entry point that allocates nothing: setup takes its device address once per
+// device (call with that device current; repeated calls return at once);
+// noalloc_scratch returns that address, or null before setup.
+constexpr uint32_t kNoAllocScratchWords = 64;
+constexpr int kNoAllocMaxDevices = 64;
+hipError_t noalloc_scratch_setup(int device);
+uint32_t *noalloc_scratch(int device);
 // zeroes every way of item slot `slot`'s finished count
 hipError_t launch_clear_done(uint32_t *item_done, uint32_t slot, hipStream_t st);
 // Shading records of nt triangles from the global vertex arrays (N / UV may

@@ -17,7 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <mutex>
 
 #include "hk_bsdf.h"
 #include "hk_cdf.h"
@@ -1205,6 +1207,33 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
   if (i < n) p[i] = v;
 }
 
+
+// ------------------------------------------------------------------ no-alloc scratch
+// Fixed scratch of hk_accel_intersect_device_noalloc: the persistent
+// intersect kernel's fetch counter in word 0, the rest of the 256-B block
+// padding, so that no other data shares the counter's memory line.  Every
+// device holds its own copy at its own address; host code uses only that
+// address, taken once per device with the device current
+// (noalloc_scratch_setup), never the array's name.
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+static std::atomic<uint32_t *> g_scratch_dev[kNoAllocMaxDevices];
+static std::mutex g_scratch_mu;
+
+hipError_t noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device].load()) return hipSuccess;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_dev[device].store(static_cast<uint32_t *>(p));
+  return hipSuccess;
+}
+
+uint32_t *noalloc_scratch(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+  return g_scratch_dev[device].load();
+}
 
 // ------------------------------------------------------------------ host launchers
 static uint32_t grid_for(uint32_t n, uint32_t cap) {

@@ -389,6 +389,8 @@ int hk_ctx_create(int device, hk_ctx **out) {
     throw HkError(HK_ERR_HIP, "no HIP device available (%s)", e == hipSuccess ? "0 devices" : hipGetErrorString(e));
   if (device < 0 || device >= n) throw HkError(HK_ERR_INVALID, "device %d out of range (%d devices)", device, n);
   hip_check(hipSetDevice(device), "hipSetDevice");
+  // the device address of this device's copy of the no-alloc scratch
+  hip_check(noalloc_scratch_setup(device), "no-alloc scratch address (hipGetSymbolAddress)");
   std::unique_ptr<hk_ctx> ctx(new hk_ctx());
   ctx->device = device;
   hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -516,6 +518,24 @@ int hk_accel_intersect_device(hk_accel *a, const hk_ray *d_rays, uint32_t n, hk_
   if (n > (1u << 31)) throw HkError(HK_ERR_INVALID, "at most 2^31 rays per call");
   hip_check(launch_intersect_persistent((const TraceNode *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, d_rays,
                                         d_hits, n, any_hit, a->fetch.as<uint32_t>(),
+                                        any_hit ? ctx->refill_shadow : ctx->refill_extend, ctx->fetch_batch,
+                                        ctx->stream),
+            "k_intersect launch");
+  return HK_OK;
+  HK_API_END(ctx)
+}
+
+int hk_accel_intersect_device_noalloc(hk_accel *a, const hk_ray *d_rays, uint32_t n, hk_hit *d_hits, int any_hit) {
+  hk_ctx *ctx = a ? a->ctx : nullptr;
+  HK_API_BEGIN
+  if (!a || (n && (!d_rays || !d_hits))) throw HkError(HK_ERR_INVALID, "null argument");
+  if (n > HK_NOALLOC_MAX_RAYS)
+    throw HkError(HK_ERR_INVALID, "hk_accel_intersect_device_noalloc: %u rays exceed the limit of %u per call", n,
+                  HK_NOALLOC_MAX_RAYS);
+  uint32_t *counter = noalloc_scratch(ctx->device);
+  if (!counter) throw HkError(HK_ERR_HIP, "no-alloc scratch of device %d is not set up", ctx->device);
+  hip_check(launch_intersect_persistent((const TraceNode *)a->bvh.nodes.p, (const TraceTri *)a->bvh.tris.p, 0, d_rays,
+                                        d_hits, n, any_hit, counter,
                                         any_hit ? ctx->refill_shadow : ctx->refill_extend, ctx->fetch_batch,
                                         ctx->stream),
             "k_intersect launch");

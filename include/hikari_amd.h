@@ -91,6 +91,22 @@ int hk_accel_intersect(hk_accel *accel, const hk_ray *rays, uint32_t n, hk_hit *
  * accelerator must not overlap (they share its fetch counter). */
 int hk_accel_intersect_device(hk_accel *accel, const hk_ray *d_rays, uint32_t n, hk_hit *d_hits, int any_hit);
 
+/* No counterpart in the reference.  hk_accel_intersect_device for callers
+ * that cannot allocate device memory: code under HIP stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Same rays, hits, options
+ * and results (closest hits bit for bit); it allocates nothing: the
+ * persistent kernels' fetch counter lives in a fixed 256-B scratch block of
+ * the library, one per device, held until the process ends (its device
+ * address is taken when the device's first context is created), and is
+ * cleared by a kernel at every call (no memset node in a captured graph).
+ * Every call on one device shares that block: calls that overlap on one
+ * device, from any accelerator or context, race and must be ordered by the
+ * caller.  At most HK_NOALLOC_MAX_RAYS rays per call (HK_ERR_INVALID beyond).
+ * Asynchronous, on the accelerator's context stream. */
+#define HK_NOALLOC_MAX_RAYS 2147483648u
+int hk_accel_intersect_device_noalloc(hk_accel *accel, const hk_ray *d_rays, uint32_t n, hk_hit *d_hits,
+                                      int any_hit);
+
 /* ------------------------------------------------------------ BSDFs */
 /* Material records: 32 floats each.  [0] type (0 diffuse, 1 conductor,
  * 2 dielectric, 3 plastic, 4 roughConductor, 5 roughDielectric, 6 roughPlastic,
