@@ -82,10 +82,22 @@ struct TraceCount {
 // ref == kDone.  Slab tests are widened by a few ulp (tnear <= tfar *
 // (1 + 2^-21)), which covers the rounding of (lo - o) * (1/d); the host pads
 // triangle boxes for the inexactness of the triangle test.
+//
+// Near / far slab test (node_step<true>): per axis the ray's sign of 1/d
+// picks which stored plane is the entry (near) and which the exit (far)
+// plane, so each plane costs one FMA, plane * (1/d) - o * (1/d), and no
+// min / max.  The product o * (1/d) is kept rounded up (cu*) for entry
+// distances and rounded down (cd*) for exit distances: a computed entry is
+// then never later, and an exit never earlier, than (plane - o) * (1/d) up to
+// the FMA's own final rounding, which the widening covers.  Where o * (1/d)
+// overflows, cu or cd is infinite or NaN and that axis stops constraining
+// the box (fmaxf / fminf drop NaN), which is conservative.
 constexpr float kWiden = 1.0000005f;
 struct RayTrav {
   vec3 o, d;
   float ix, iy, iz, tmin;
+  float cux, cuy, cuz, cdx, cdy, cdz;  // o * (1/d) rounded up / down, per axis
+  uint32_t offx, offy, offz;           // byte offset of the near plane array in a node
   float bt, bu, bv;
   int32_t bprim;
   int32_t ref;
@@ -100,6 +112,18 @@ __device__ __forceinline__ void trav_begin(RayTrav &R, uint64_t *lds_stack, uint
   R.iy = fabsf(d.y) > 1e-30f ? 1.0f / d.y : copysignf(1e30f, d.y);
   R.iz = fabsf(d.z) > 1e-30f ? 1.0f / d.z : copysignf(1e30f, d.z);
   R.tmin = tmin;
+  {
+    // p +- |p| 2^-23 lies at least one ulp of p away from p, beyond the
+    // rounding error (half an ulp) of p = o * (1/d)
+    float px = o.x * R.ix, py = o.y * R.iy, pz = o.z * R.iz;
+    const float k = 1.1920929e-07f;  // 2^-23
+    R.cux = fmaf(fabsf(px), k, px); R.cdx = fmaf(-fabsf(px), k, px);
+    R.cuy = fmaf(fabsf(py), k, py); R.cdy = fmaf(-fabsf(py), k, py);
+    R.cuz = fmaf(fabsf(pz), k, pz); R.cdz = fmaf(-fabsf(pz), k, pz);
+    R.offx = R.ix >= 0.0f ? 0u : 16u;
+    R.offy = R.iy >= 0.0f ? 32u : 48u;
+    R.offz = R.iz >= 0.0f ? 64u : 80u;
+  }
   R.bt = tmax;
   R.bu = R.bv = 0.0f;
   R.bprim = -1;
@@ -114,15 +138,39 @@ __device__ __forceinline__ void trav_begin(RayTrav &R, uint64_t *lds_stack, uint
 // the nearest one (or the next stack entry that can still hold a hit).  Any-
 // hit rays keep the near-to-far order too: taking the children in node order
 // saves the sort but finds occluders later (2-3% slower overall).
+// NF: the near / far slab test (fewer VALU, nine more live registers per
+// lane); otherwise (lo - o) * (1/d) and (hi - o) * (1/d) with min / max.
+template <bool NF>
 __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, RayTrav &R, TraceCount &cnt) {
 #ifdef HK_TRAVERSAL_STATS
   cnt.nodes++;
 #endif
-  const vec3 o = R.o;
   const float ix = R.ix, iy = R.iy, iz = R.iz;
   float tlim = R.bt * kWiden;
   float tn[4];
   int32_t r[4];
+  if constexpr (NF) {
+    const char *nb = reinterpret_cast<const char *>(nodes);
+    uint32_t base = uint32_t(R.ref) << 7;
+    float4 nx4 = *reinterpret_cast<const float4 *>(nb + (base + R.offx));
+    float4 fx4 = *reinterpret_cast<const float4 *>(nb + (base + (R.offx ^ 16u)));
+    float4 ny4 = *reinterpret_cast<const float4 *>(nb + (base + R.offy));
+    float4 fy4 = *reinterpret_cast<const float4 *>(nb + (base + (R.offy ^ 16u)));
+    float4 nz4 = *reinterpret_cast<const float4 *>(nb + (base + R.offz));
+    float4 fz4 = *reinterpret_cast<const float4 *>(nb + (base + (R.offz ^ 16u)));
+    int4 ch = *reinterpret_cast<const int4 *>(nb + (base + 96u));
+    float nxs[4] = {nx4.x, nx4.y, nx4.z, nx4.w}, fxs[4] = {fx4.x, fx4.y, fx4.z, fx4.w};
+    float nys[4] = {ny4.x, ny4.y, ny4.z, ny4.w}, fys[4] = {fy4.x, fy4.y, fy4.z, fy4.w};
+    float nzs[4] = {nz4.x, nz4.y, nz4.z, nz4.w}, fzs[4] = {fz4.x, fz4.y, fz4.z, fz4.w};
+    r[0] = ch.x; r[1] = ch.y; r[2] = ch.z; r[3] = ch.w;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      float a = fmaxf(fmaxf(fmaf(nxs[c], ix, -R.cux), fmaf(nys[c], iy, -R.cuy)), fmaxf(fmaf(nzs[c], iz, -R.cuz), R.tmin));
+      float b = fminf(fminf(fmaf(fxs[c], ix, -R.cdx), fmaf(fys[c], iy, -R.cdy)), fminf(fmaf(fzs[c], iz, -R.cdz), tlim));
+      tn[c] = (r[c] != kEmptyRef && a <= b * kWiden) ? a : kInfF;
+    }
+  } else {
+  const vec3 o = R.o;
   const float4 *np = reinterpret_cast<const float4 *>(nodes + R.ref);
   float4 lx4 = np[0], hx4 = np[1], ly4 = np[2], hy4 = np[3], lz4 = np[4], hz4 = np[5];
   int4 ch = *reinterpret_cast<const int4 *>(np + 6);
@@ -138,6 +186,7 @@ __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, R
     float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), R.tmin));
     float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tlim));
     tn[c] = (r[c] != kEmptyRef && a <= b * kWiden) ? a : kInfF;
+  }
   }
 #define HK_CSWAP(i, j)                                    \
   if (tn[j] < tn[i]) {                                    \
@@ -192,7 +241,7 @@ constexpr int kLeafEarly = 8;
 // Visiting nodes before a parked leaf has lowered bt only costs extra
 // visits: the result is the minimum t (largest prim among equal t) in any
 // order.
-template <bool ANY>
+template <bool ANY, bool NF = true>
 __device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, const TraceTri *__restrict__ tris,
                                            RayTrav &R, int32_t &pend, TraceCount &cnt) {
   int32_t pend2 = kDone;  // closest-hit rays park a second leaf rather than wait
@@ -212,7 +261,7 @@ __device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, 
     // (node steps, or at least one parked leaf tested), so the loop ends.
     int searching = __popcll(__ballot(R.ref >= 0 && pend == kDone));
     if (searching == 0 || (searching < kLeafEarly && __ballot(pend != kDone) != 0ull)) break;
-    if (R.ref >= 0) node_step(nodes, R, cnt);
+    if (R.ref >= 0) node_step<NF>(nodes, R, cnt);
   }
   if (pend != kDone) {
     leaf_tris<ANY>(tris, R, pend, cnt);
@@ -604,7 +653,12 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi) {
 // and 2 x 64 entries are fastest on the benchmark scenes.
 constexpr uint32_t kNoRay = 0xffffffffu;
 
-__global__ __launch_bounds__(kBlock) void k_extend(RenderArgs A, int qi) {
+// k_extend is held at 7 waves per SIMD (at most 72 VGPRs: it fits without
+// spills); left alone the near / far slab test's per-ray constants take it to
+// 80 VGPRs and 6 waves, which loses the gain.  k_shadow (95 VGPRs, 5 waves)
+// keeps the min / max slab test: the constants would cost it a wave.
+constexpr int kExtendMinWaves = 7;
+__global__ __launch_bounds__(kBlock, kExtendMinWaves) void k_extend(RenderArgs A, int qi) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
   uint32_t count = A.counters[ctr(qi)];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -993,7 +1047,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
       pend = kDone;
       traced++;
     }
-    trav_round<true>(A.sc.nodes, A.sc.tris, R, pend, tc);
+    trav_round<true, false>(A.sc.nodes, A.sc.tris, R, pend, tc);  // min / max slab test (5 waves)
     bool fin = false;
     if (ent != kNoRay && R.ref == kDone && pend == kDone) {
       if (R.bprim < 0) L = L + c;
