@@ -438,9 +438,13 @@ __device__ __forceinline__ void count_done(uint32_t *item_done, bool pred, uint3
 }
 
 
-__device__ __forceinline__ vec3 mat3_mul(const float *m, vec3 v) {  // row-major 3x3
-  return v3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
-            m[6] * v.x + m[7] * v.y + m[8] * v.z);
+// A 3x3 block (row stride rs) times v as the reference's Transform applies
+// it to a vector (src/core/Transform.cpp:36-39): Eigen's coefficient product
+// sums each row as a0 + (a1 + a2), every product and sum rounded on its own.
+__device__ __forceinline__ vec3 mat3_mul(const float *m, vec3 v, int rs = 3) {
+#pragma clang fp contract(off)
+  return v3(m[0] * v.x + (m[1] * v.y + m[2] * v.z), m[rs] * v.x + (m[rs + 1] * v.y + m[rs + 2] * v.z),
+            m[2 * rs] * v.x + (m[2 * rs + 1] * v.y + m[2 * rs + 2] * v.z));
 }
 
 // ---- emitters
@@ -631,8 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi) {
     float inv_z = 1.0f / dir.z;
     const float *c = rq.c2w;
     vec3 o = v3(c[3] / c[15], c[7] / c[15], c[11] / c[15]);
-    vec3 d = v3(c[0] * dir.x + c[1] * dir.y + c[2] * dir.z, c[4] * dir.x + c[5] * dir.y + c[6] * dir.z,
-                c[8] * dir.x + c[9] * dir.y + c[10] * dir.z);
+    vec3 d = mat3_mul(c, dir, 4);
     uint32_t pos = q_base + j;
     A.q_o[qi][pos] = make_float4(o.x, o.y, o.z, rq.near_clip * inv_z);
     A.q_d[qi][pos] = make_float4(d.x, d.y, d.z, rq.far_clip * inv_z);

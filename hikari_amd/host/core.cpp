@@ -26,26 +26,98 @@ HikariException::HikariException(const char *fmt, ...) : std::runtime_error("") 
 }
 
 // ------------------------------------------------------------------ Transform
+// FP32 arithmetic in Eigen 3.3.5's evaluation order, pinned bit for bit
+// against the reference's vendored Eigen (oracle/_ref/eigen_xform): a dot
+// product of three terms reduces as a0 + (a1 + a2) (Core/Redux.h
+// redux_novec_unroller), Matrix4f * Vector4f accumulates column by column
+// (((c0 x + c1 y) + c2 z) + c3 w, the packet product of
+// Core/ProductEvaluators.h).  The host build turns FP contraction off.
+namespace {
+inline float dot3(float a0, float b0, float a1, float b1, float a2, float b2) { return a0 * b0 + (a1 * b1 + a2 * b2); }
+Vector3f cross3(const Vector3f &a, const Vector3f &b) {  // Geometry/OrthoMethods.h
+  return Vector3f(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+}  // namespace
+
+Vector3f Normalized(const Vector3f &v) {
+  float z = v.x * v.x + (v.y * v.y + v.z * v.z);
+  if (!(z > 0.0f)) return v;
+  float s = std::sqrt(z);
+  return Vector3f(v.x / s, v.y / s, v.z / s);
+}
+
 Transform::Transform() {
   for (int i = 0; i < 4; i++)
-    for (int j = 0; j < 4; j++) m[i][j] = i == j ? 1.0 : 0.0;
+    for (int j = 0; j < 4; j++) m[i][j] = inv[i][j] = i == j ? 1.0f : 0.0f;
 }
-Transform Transform::FromRows(const double *a) {
-  Transform t;
-  for (int i = 0; i < 16; i++) t.m[i / 4][i % 4] = a[i];
-  return t;
+// Translation * Affine3f: the translation column gains v (Translation.h:125-129)
+void Transform::PreTranslate(const Vector3f &v) {
+  m[0][3] = m[0][3] + v.x;
+  m[1][3] = m[1][3] + v.y;
+  m[2][3] = m[2][3] + v.z;
+  UpdateInverse();
 }
-Transform Transform::operator*(const Transform &b) const {
-  Transform r;
-  for (int i = 0; i < 4; i++)
-    for (int j = 0; j < 4; j++) {
-      double s = 0;
-      for (int k = 0; k < 4; k++) s += m[i][k] * b.m[k][j];
-      r.m[i][j] = s;
-    }
-  return r;
+// DiagonalMatrix * Affine3f: row i of the top three rows times v_i
+// (Transform.h:478-486)
+void Transform::PreScale(const Vector3f &v) {
+  const float s[3] = {v.x, v.y, v.z};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 4; j++) m[i][j] = s[i] * m[i][j];
+  UpdateInverse();
 }
-Transform Transform::Inverse() const {
+// AngleAxis<float>(DegToRad(angle), axis) * Affine3f: the rotation matrix of
+// AngleAxis::toRotationMatrix (AngleAxis.h:218-243, the axis not normalised)
+// times the top three rows (Transform.h:1459-1472)
+void Transform::PreRotate(float angle_deg, const Vector3f &axis) {
+  float ang = angle_deg * (float(M_PI) / 180.0f);  // DegToRad, include/core/Common.hpp:742
+  const float a[3] = {axis.x, axis.y, axis.z};
+  float s = std::sin(ang), c = std::cos(ang);
+  float sa[3], ca[3];
+  for (int i = 0; i < 3; i++) {
+    sa[i] = s * a[i];
+    ca[i] = (1.0f - c) * a[i];
+  }
+  float r[3][3], t;
+  t = ca[0] * a[1]; r[0][1] = t - sa[2]; r[1][0] = t + sa[2];
+  t = ca[0] * a[2]; r[0][2] = t + sa[1]; r[2][0] = t - sa[1];
+  t = ca[1] * a[2]; r[1][2] = t - sa[0]; r[2][1] = t + sa[0];
+  for (int i = 0; i < 3; i++) r[i][i] = ca[i] * a[i] + c;
+  float o[3][4];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 4; j++) o[i][j] = dot3(r[i][0], m[0][j], r[i][1], m[1][j], r[i][2], m[2][j]);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 4; j++) m[i][j] = o[i][j];
+  UpdateInverse();
+}
+// Affine3f(A) * Affine3f: linear = A.linear * linear, translation =
+// A.linear * translation + A.translation, bottom row (0, 0, 0, 1)
+// (Transform.h:1481-1495)
+void Transform::PreAffine(const float *a) {
+  float o[3][4];
+  for (int i = 0; i < 3; i++) {
+    const float *ar = a + 4 * i;
+    for (int j = 0; j < 3; j++) o[i][j] = dot3(ar[0], m[0][j], ar[1], m[1][j], ar[2], m[2][j]);
+    o[i][3] = dot3(ar[0], m[0][3], ar[1], m[1][3], ar[2], m[2][3]) + ar[3];
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 4; j++) m[i][j] = o[i][j];
+  m[3][0] = m[3][1] = m[3][2] = 0.0f;
+  m[3][3] = 1.0f;
+  UpdateInverse();
+}
+// Parser.cpp:356-367: columns left, new up, direction, origin
+void Transform::PreLookAt(const Point3f &origin, const Point3f &target, const Vector3f &up) {
+  Vector3f dir = Normalized(Vector3f(target.x - origin.x, target.y - origin.y, target.z - origin.z));
+  Vector3f left = Normalized(cross3(Normalized(up), dir));
+  Vector3f nup = Normalized(cross3(dir, left));
+  const float a[16] = {left.x, nup.x, dir.x, origin.x, left.y, nup.y, dir.y, origin.y,
+                       left.z, nup.z, dir.z, origin.z, 0.0f,   0.0f,  0.0f,  1.0f};
+  PreAffine(a);
+}
+// The reference inverts with Eigen's FP32 4x4 routine (an SSE cofactor
+// scheme); this one inverts in FP64 and rounds, which agrees with it to a
+// few ulp.  Only normals use the inverse.
+void Transform::UpdateInverse() {
   double a[4][8];
   for (int i = 0; i < 4; i++)
     for (int j = 0; j < 4; j++) {
@@ -66,31 +138,25 @@ Transform Transform::Inverse() const {
         for (int j = 0; j < 8; j++) a[r][j] -= f * a[c][j];
       }
   }
-  Transform r;
   for (int i = 0; i < 4; i++)
-    for (int j = 0; j < 4; j++) r.m[i][j] = a[i][j + 4];
-  return r;
+    for (int j = 0; j < 4; j++) inv[i][j] = float(a[i][j + 4]);
 }
 Point3f Transform::ApplyPoint(const Point3f &p) const {
-  double x = m[0][0] * p.x + m[0][1] * p.y + m[0][2] * p.z + m[0][3];
-  double y = m[1][0] * p.x + m[1][1] * p.y + m[1][2] * p.z + m[1][3];
-  double z = m[2][0] * p.x + m[2][1] * p.y + m[2][2] * p.z + m[2][3];
-  double w = m[3][0] * p.x + m[3][1] * p.y + m[3][2] * p.z + m[3][3];
-  return Point3f(float(x / w), float(y / w), float(z / w));
+  float r[4];
+  for (int i = 0; i < 4; i++) r[i] = ((m[i][0] * p.x + m[i][1] * p.y) + m[i][2] * p.z) + m[i][3] * 1.0f;
+  return Point3f(r[0] / r[3], r[1] / r[3], r[2] / r[3]);
 }
 Vector3f Transform::ApplyVector(const Vector3f &v) const {
-  return Vector3f(float(m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z),
-                  float(m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z),
-                  float(m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z));
+  return Vector3f(dot3(m[0][0], v.x, m[0][1], v.y, m[0][2], v.z), dot3(m[1][0], v.x, m[1][1], v.y, m[1][2], v.z),
+                  dot3(m[2][0], v.x, m[2][1], v.y, m[2][2], v.z));
 }
 Vector3f Transform::ApplyNormal(const Vector3f &n) const {
-  Transform inv = Inverse();
-  return Vector3f(float(inv.m[0][0] * n.x + inv.m[1][0] * n.y + inv.m[2][0] * n.z),
-                  float(inv.m[0][1] * n.x + inv.m[1][1] * n.y + inv.m[2][1] * n.z),
-                  float(inv.m[0][2] * n.x + inv.m[1][2] * n.y + inv.m[2][2] * n.z));
+  return Vector3f(dot3(inv[0][0], n.x, inv[1][0], n.y, inv[2][0], n.z),
+                  dot3(inv[0][1], n.x, inv[1][1], n.y, inv[2][1], n.z),
+                  dot3(inv[0][2], n.x, inv[1][2], n.y, inv[2][2], n.z));
 }
 void Transform::ToFloat(float *out) const {
-  for (int i = 0; i < 16; i++) out[i] = float(m[i / 4][i % 4]);
+  for (int i = 0; i < 16; i++) out[i] = m[i / 4][i % 4];
 }
 
 // ------------------------------------------------------------------ PropertyList
@@ -353,66 +419,31 @@ struct Loader {
           case Tag::kTransform: check_attrs(node, {"name"}, true); plist.SetTransform(name, trans); break;
           case Tag::kTranslate: {
             check_attrs(node, {"value"}, true);
-            Vector3f v = to_vec3(value);
-            Transform t;
-            t.m[0][3] = v.x; t.m[1][3] = v.y; t.m[2][3] = v.z;
-            trans = t * trans;
+            trans.PreTranslate(to_vec3(value));
             break;
           }
           case Tag::kMatrix: {
             check_attrs(node, {"value"}, true);
             std::vector<std::string> tok = tokenize(value);
             if (tok.size() != 16) throw HikariException("Expected 16 values");
-            double a[16];
+            float a[16];
             for (int i = 0; i < 16; i++) a[i] = to_float(tok[size_t(i)]);
-            trans = Transform::FromRows(a) * trans;
+            trans.PreAffine(a);
             break;
           }
           case Tag::kScale: {
             check_attrs(node, {"value"}, true);
-            Vector3f v = to_vec3(value);
-            Transform t;
-            t.m[0][0] = v.x; t.m[1][1] = v.y; t.m[2][2] = v.z;
-            trans = t * trans;
+            trans.PreScale(to_vec3(value));
             break;
           }
           case Tag::kRotate: {
             check_attrs(node, {"angle", "axis"}, true);
-            double ang = double(to_float(attr(node, "angle"))) * M_PI / 180.0;
-            Vector3f ax = to_vec3(attr(node, "axis"));
-            double n = std::sqrt(double(ax.x) * ax.x + double(ax.y) * ax.y + double(ax.z) * ax.z);
-            double x = ax.x / n, y = ax.y / n, z = ax.z / n, c = std::cos(ang), s = std::sin(ang), C1 = 1 - c;
-            double r[16] = {c + x * x * C1, x * y * C1 - z * s, x * z * C1 + y * s, 0,
-                            y * x * C1 + z * s, c + y * y * C1, y * z * C1 - x * s, 0,
-                            z * x * C1 - y * s, z * y * C1 + x * s, c + z * z * C1, 0,
-                            0, 0, 0, 1};
-            trans = Transform::FromRows(r) * trans;
+            trans.PreRotate(to_float(attr(node, "angle")), to_vec3(attr(node, "axis")));
             break;
           }
           case Tag::kLookAt: {
             check_attrs(node, {"origin", "target", "up"}, true);
-            Vector3f o = to_vec3(attr(node, "origin")), t = to_vec3(attr(node, "target")), u = to_vec3(attr(node, "up"));
-            auto norm = [](double a[3]) {
-              double l = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-              a[0] /= l; a[1] /= l; a[2] /= l;
-            };
-            auto cross = [](const double a[3], const double b[3], double r[3]) {
-              r[0] = a[1] * b[2] - a[2] * b[1];
-              r[1] = a[2] * b[0] - a[0] * b[2];
-              r[2] = a[0] * b[1] - a[1] * b[0];
-            };
-            double dir[3] = {double(t.x) - o.x, double(t.y) - o.y, double(t.z) - o.z};
-            norm(dir);
-            double up[3] = {u.x, u.y, u.z};
-            norm(up);
-            double left[3], nup[3];
-            cross(up, dir, left);
-            norm(left);
-            cross(dir, left, nup);
-            norm(nup);
-            double r[16] = {left[0], nup[0], dir[0], o.x, left[1], nup[1], dir[1], o.y,
-                            left[2], nup[2], dir[2], o.z, 0, 0, 0, 1};
-            trans = Transform::FromRows(r) * trans;
+            trans.PreLookAt(to_vec3(attr(node, "origin")), to_vec3(attr(node, "target")), to_vec3(attr(node, "up")));
             break;
           }
           default: throw HikariException("Unhandled element \"%s\"", node.name.c_str());

@@ -43,18 +43,31 @@ struct Vector3f {
 };
 using Point3f = Vector3f;
 
-// 4x4 affine transform in double precision (row-major, column vectors).
+// 4x4 affine transform in FP32 (row-major, column vectors) with its inverse,
+// the reference's Transform over Eigen::Matrix4f (include/core/
+// Transform.hpp:16-57).  The XML ops premultiply the transform being built
+// with Eigen::Affine3f's arithmetic (src/core/Parser.cpp:308-369); products
+// and sums round in the order Eigen 3.3.5 evaluates them (a 3-term dot
+// product as a0 + (a1 + a2), the 4x4 point product left to right).
 struct Transform {
-  double m[4][4];
-  Transform();  // identity
-  static Transform FromRows(const double *a16);
-  Transform operator*(const Transform &b) const;
-  Transform Inverse() const;
-  Point3f ApplyPoint(const Point3f &p) const;
-  Vector3f ApplyVector(const Vector3f &v) const;
-  Vector3f ApplyNormal(const Vector3f &n) const;  // inverse transpose, not normalised
+  float m[4][4];
+  float inv[4][4];  // inverse: FP64 Gauss-Jordan of m, rounded to FP32
+  Transform();      // identity
+  // the XML ops (Parser.cpp:312-369), each premultiplied onto *this
+  void PreTranslate(const Vector3f &v);
+  void PreScale(const Vector3f &v);
+  void PreRotate(float angle_deg, const Vector3f &axis);  // axis used as given (AngleAxis)
+  void PreAffine(const float *rows16);                     // bottom row ignored, as Affine3f does
+  void PreLookAt(const Point3f &origin, const Point3f &target, const Vector3f &up);
+  Point3f ApplyPoint(const Point3f &p) const;     // Transform.cpp:46-50
+  Vector3f ApplyVector(const Vector3f &v) const;  // Transform.cpp:36-39
+  Vector3f ApplyNormal(const Vector3f &n) const;  // Transform.cpp:41-44 (inverse transpose, not normalised)
   void ToFloat(float *out16) const;
+  void UpdateInverse();
 };
+// Eigen's normalized() in FP32 (Dot.h:124-134: divide by the square root of
+// x*x + (y*y + z*z) when that is positive)
+Vector3f Normalized(const Vector3f &v);
 
 // Typed property bag (src/core/PropertyList.cpp): Get(name) throws when the
 // property is missing or has another type; Get(name, default) returns the

@@ -3,10 +3,12 @@
 // integrator and written as EXR or PFM, a test runs at load time and sets the
 // exit status.
 //
-//   hikari_render FILE.xml [-o OUT.exr|OUT.pfm] [--spp N] [--dump]
+//   hikari_render FILE.xml [-o OUT.exr|OUT.pfm] [--spp N] [--dump] [--dump-mesh PREFIX]
 //   hikari_render --convert IN.exr|IN.pfm OUT.exr|OUT.pfm
 //
 // --dump prints the parsed description as JSON and touches no device;
+// --dump-mesh also writes every mesh's world-space vertex positions and
+// normals as raw FP32 (PREFIX.<mesh>.v.f32, PREFIX.<mesh>.n.f32);
 // --convert rewrites an image (EXR written as uncompressed FLOAT RGB).
 #include <chrono>
 #include <cstdio>
@@ -24,6 +26,16 @@ static void json_floats(const float *v, size_t n) {
   printf("]");
 }
 
+static std::string g_mesh_prefix;
+
+static void write_f32(const std::string &path, const std::vector<float> &v) {
+  FILE *f = fopen(path.c_str(), "wb");
+  if (!f) throw HikariException("cannot write \"%s\"", path.c_str());
+  size_t n = fwrite(v.data(), sizeof(float), v.size(), f);
+  fclose(f);
+  if (n != v.size()) throw HikariException("short write to \"%s\"", path.c_str());
+}
+
 static void dump_scene(const Scene *s) {
   printf("{\"meshes\": [");
   const auto &ms = s->GetMeshes();
@@ -33,6 +45,10 @@ static void dump_scene(const Scene *s) {
     for (size_t k = 0; k < m->V.size(); k++) {
       lo[k % 3] = std::min(lo[k % 3], m->V[k]);
       hi[k % 3] = std::max(hi[k % 3], m->V[k]);
+    }
+    if (!g_mesh_prefix.empty()) {
+      write_f32(g_mesh_prefix + "." + std::to_string(i) + ".v.f32", m->V);
+      write_f32(g_mesh_prefix + "." + std::to_string(i) + ".n.f32", m->N);
     }
     std::vector<float> rec;
     m->bsdf->AppendRecords(rec);
@@ -88,14 +104,18 @@ int main(int argc, char **argv) {
     if (a == "-o" && i + 1 < argc) out = argv[++i];
     else if (a == "--spp" && i + 1 < argc) spp = std::atoi(argv[++i]);
     else if (a == "--dump") dump = true;
+    else if (a == "--dump-mesh" && i + 1 < argc) {
+      dump = true;
+      g_mesh_prefix = argv[++i];
+    }
     else if (file.empty()) file = a;
     else {
-      fprintf(stderr, "usage: hikari_render FILE.xml [-o OUT.exr|OUT.pfm] [--spp N] [--dump]\n");
+      fprintf(stderr, "usage: hikari_render FILE.xml [-o OUT.exr|OUT.pfm] [--spp N] [--dump] [--dump-mesh PREFIX]\n");
       return 2;
     }
   }
   if (file.empty()) {
-    fprintf(stderr, "usage: hikari_render FILE.xml [-o OUT.exr|OUT.pfm] [--spp N] [--dump]\n");
+    fprintf(stderr, "usage: hikari_render FILE.xml [-o OUT.exr|OUT.pfm] [--spp N] [--dump] [--dump-mesh PREFIX]\n");
     return 2;
   }
   SetDryRun(dump);

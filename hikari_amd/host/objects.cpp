@@ -336,9 +336,7 @@ class WavefrontObjMesh : public Mesh {
       } else if (pre == "vn") {
         Vector3f q;
         ls >> q.x >> q.y >> q.z;
-        Vector3f n = T.ApplyNormal(q);
-        double l = std::sqrt(double(n.x) * n.x + double(n.y) * n.y + double(n.z) * n.z);
-        nrm.push_back(Vector3f(float(n.x / l), float(n.y / l), float(n.z / l)));
+        nrm.push_back(Normalized(T.ApplyNormal(q)));
       } else if (pre == "f") {
         std::string a, b, c, d;
         ls >> a >> b >> c >> d;

@@ -306,14 +306,18 @@ struct M4 {
     for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) r.m[i][j] = a[i][j + 4];
     return r;
   }
-  template <typename R> V3<R> point(V3<R> p) const {  // src/core/Transform.cpp:46-50
+  // src/core/Transform.cpp:46-50: Matrix4f * Vector4f accumulates column by
+  // column, ((c0 x + c1 y) + c2 z) + c3 (pinned against oracle/_ref/eigen_xform)
+  template <typename R> V3<R> point(V3<R> p) const {
     R v[4];
     for (int i = 0; i < 4; i++) v[i] = R(m[i][0]) * p.x + R(m[i][1]) * p.y + R(m[i][2]) * p.z + R(m[i][3]);
     return {v[0] / v[3], v[1] / v[3], v[2] / v[3]};
   }
-  template <typename R> V3<R> vec(V3<R> p) const {  // Transform.cpp:36-39
+  // Transform.cpp:36-39: Eigen sums each row of the 3x3 block as
+  // a0 + (a1 + a2) (pinned against oracle/_ref/eigen_xform)
+  template <typename R> V3<R> vec(V3<R> p) const {
     R v[3];
-    for (int i = 0; i < 3; i++) v[i] = R(m[i][0]) * p.x + R(m[i][1]) * p.y + R(m[i][2]) * p.z;
+    for (int i = 0; i < 3; i++) v[i] = R(m[i][0]) * p.x + (R(m[i][1]) * p.y + R(m[i][2]) * p.z);
     return {v[0], v[1], v[2]};
   }
 };

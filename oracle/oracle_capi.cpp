@@ -91,6 +91,21 @@ void ho_vec_ops(int n, const float *a, const float *b, float *out) {
   }
 }
 
+// The scene transforms' FP32 point and vector products (M4::point / vec,
+// src/core/Transform.cpp:36-50) of a row-major 4x4 matrix on n points:
+// out rows of [point(3), vector(3)].
+void ho_xform(const float *m16, int n, const float *p, float *out) {
+  M4 m;
+  for (int i = 0; i < 16; i++) m.m[i / 4][i % 4] = m16[i];
+  for (int i = 0; i < n; i++) {
+    V3<float> x(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+    V3<float> a = m.point(x), b = m.vec(x);
+    float *o = out + 6 * i;
+    o[0] = a.x; o[1] = a.y; o[2] = a.z;
+    o[3] = b.x; o[4] = b.y; o[5] = b.z;
+  }
+}
+
 // Ray batch vs triangle soup.  tri: nt x 9 (p0, p1, p2); rays: n x 8
 // (o.xyz, tmin, d.xyz, tmax).  prim = -1 on miss.
 int ho_intersect(const float *tri, uint32_t nt, const float *rays, uint32_t n, int any, int use_bvh, float *out_t,

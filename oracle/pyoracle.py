@@ -31,6 +31,7 @@ def lib():
         L.ho_pcg32_demo.argtypes = [C.c_int, C.c_char_p, C.c_int]
         L.ho_path_floats.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P]
         L.ho_vec_ops.argtypes = [C.c_int, P, P, P]
+        L.ho_xform.argtypes = [P, C.c_int, P, P]
         L.ho_intersect.argtypes = [P, C.c_uint32, P, C.c_uint32, C.c_int, C.c_int, P, P, P, P]
         L.ho_bsdf_batch.argtypes = [C.c_int, P, C.c_int, C.c_int, C.c_char_p, C.c_int, P, P, P, P, P, P, P, P, P, P, P]
         L.ho_rft_reduce.argtypes = [C.c_char_p, C.c_float, C.c_float, C.c_int, P, P]
@@ -72,6 +73,15 @@ def path_floats(req_seed, pixel, sample, n):
     out = np.zeros(n, np.float32)
     lib().ho_path_floats(req_seed, pixel, sample, n, _p(out))
     return out
+
+
+def xform(m, pts):
+    """M4::point and M4::vec of the row-major 4x4 FP32 matrix m on pts (n x 3)."""
+    m = np.ascontiguousarray(m, np.float32).reshape(16)
+    pts = np.ascontiguousarray(pts, np.float32)
+    out = np.zeros((pts.shape[0], 6), np.float32)
+    lib().ho_xform(_p(m), pts.shape[0], _p(pts), _p(out))
+    return out[:, :3], out[:, 3:]
 
 
 def vec_ops(a, b):
