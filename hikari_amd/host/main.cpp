@@ -31,7 +31,7 @@ static std::string g_mesh_prefix;
 static void write_f32(const std::string &path, const std::vector<float> &v) {
   FILE *f = fopen(path.c_str(), "wb");
   if (!f) throw HikariException("cannot write \"%s\"", path.c_str());
-  size_t n = fwrite(v.data(), sizeof(float), v.size(), f);
+  size_t n = v.empty() ? 0 : fwrite(v.data(), sizeof(float), v.size(), f);
   fclose(f);
   if (n != v.size()) throw HikariException("short write to \"%s\"", path.c_str());
 }

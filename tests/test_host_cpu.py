@@ -16,7 +16,8 @@ from tests.xmlscene import parse_test
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-CLI = os.path.join(ROOT, "hikari_amd", "hikari_render")
+# HIKARI_RENDER selects another build of the driver (e.g. a sanitizer build)
+CLI = os.environ.get("HIKARI_RENDER", os.path.join(ROOT, "hikari_amd", "hikari_render"))
 
 pytestmark = pytest.mark.skipif(not os.path.exists(CLI), reason="hikari_render not built")
 

@@ -19,7 +19,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PROBE = os.path.join(ROOT, "oracle", "_ref", "eigen_xform")
-RENDER = os.path.join(ROOT, "hikari_amd", "hikari_render")
+# HIKARI_RENDER selects another build of the driver (e.g. a sanitizer build)
+RENDER = os.environ.get("HIKARI_RENDER", os.path.join(ROOT, "hikari_amd", "hikari_render"))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 pytestmark = [pytest.mark.skipif(not os.path.exists(PROBE), reason="reference Eigen probe not built"),
