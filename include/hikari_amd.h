@@ -197,7 +197,10 @@ typedef struct hk_render_stats {
 /* Whole-image render with the gpuPathMIS integrator (the wavefront form of
  * PathMISIntegrator::Li, src/integrator/PathMISIntegrator.cpp:16-157, driven
  * as src/Main.cpp:11-147).  out_rgb: host, height x width x 3, the filtered
- * image (ImageBlock::ToBitmap).  Synchronous. */
+ * image (ImageBlock::ToBitmap).  Synchronous.  The bounce loop is driven from
+ * the host: every bounce enqueues k_regen / k_extend / k_shade / k_shadow
+ * (replayed from a captured HIP graph after the first bounces) and the host
+ * reads the queue sizes between bounces; it is not one persistent launch. */
 int hk_render(hk_scene *scene, const hk_request *req, float *out_rgb, hk_render_stats *stats);
 
 /* ------------------------------------------------------------ autotuning */
@@ -228,7 +231,8 @@ int hk_render_pixel_samples(hk_scene *scene, uint32_t seed, uint32_t x, uint32_t
 /* Batched renders: n requests over one resident scene share one wavefront
  * (paths tagged by request).  out_rgb[i]: host buffer of request i
  * (height x width x 3).  Results equal the requests rendered alone, bit for
- * bit.  max_paths: paths in flight (0: the context's). */
+ * bit.  max_paths: paths in flight (0: the context's).  Host-driven bounce
+ * loop, as hk_render. */
 int hk_render_batch(hk_scene *scene, const hk_request *reqs, uint32_t n, float **out_rgb, uint32_t max_paths,
                     hk_render_stats *stats);
 

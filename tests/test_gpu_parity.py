@@ -495,6 +495,38 @@ def test_many_materials_equal_few(ctx):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_many_materials_hit_vs_oracle(ctx):
+    """36 small spheres in front of the box's back wall, each with its own
+    material (diffuse, plastic, rough plastic, Beckmann / GGX rough conductor,
+    conductor, rough diffuse, microfacet): 40 materials and 45 meshes, so the
+    shading kernel reads the records from global memory, and every sphere is
+    in view.  GPU image vs the FP32 oracle at equal streams."""
+    sc = scenes.cbox(width=48, height=48, sphere_subdiv=1)
+    kinds = [lambda k: D.diffuse((0.2 + 0.02 * k, 0.5, 0.8 - 0.02 * k)),
+             lambda k: D.plastic(int_ior=1.3 + 0.01 * k, kd=0.4),
+             lambda k: D.rough_plastic(alpha=0.05 + 0.01 * k, dist=D.GGX, kd=0.3),
+             lambda k: D.rough_conductor(alpha=0.1 + 0.01 * k, dist=D.BECKMANN, k=2.0),
+             lambda k: D.rough_conductor(alpha=0.1 + 0.01 * k, dist=D.GGX, k=3.0),
+             lambda k: D.conductor(k=1.5),
+             lambda k: D.rough_diffuse(albedo=0.6, alpha=0.1 + 0.02 * k),
+             lambda k: D.microfacet(alpha=0.1 + 0.01 * k, kd=0.3)]
+    n = 0
+    for i in range(6):
+        for j in range(6):
+            cen = (-0.625 + 0.25 * i, -0.625 + 0.25 * j, -0.8)
+            m = sc.add_material(kinds[n % len(kinds)](n))
+            v, f = scenes.icosphere(1, 0.1, cen)
+            sc.add_mesh(v, scenes._ensure_outward(v, f, cen), m)
+            n += 1
+    assert len(sc.materials) > 16 and len(sc.meshes) > 32
+    img, _ = ctx.scene(sc).render(16, seed=21)
+    ref, _, _ = O.render(sc, req_seed=21, spp=(0, 16))
+    rm = rel_mse(img, ref)
+    print("many materials: relMSE vs oracle", rm)
+    assert rm < 1e-3
+    assert abs(lum(img).mean() - lum(ref).mean()) / lum(ref).mean() < 0.01
+
+
 @pytest.mark.parametrize("filt,radius", [(D.FILTER_GAUSSIAN, 2.0), (D.FILTER_BOX, 0.5), (D.FILTER_MITCHELL, 2.0),
                                           (D.FILTER_TENT, 1.0), (D.FILTER_GAUSSIAN, 3.2), (D.FILTER_GAUSSIAN, 0.4)])
 def test_tiled_resolve_equals_plain(ctx, filt, radius):
