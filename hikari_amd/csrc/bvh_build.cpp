@@ -213,9 +213,15 @@ void build_bvh4(const float *tri9, uint32_t nt, Bvh4Result &out) {
   } else {
     N4.emplace_back();
     st.push_back({0, 0, 1});
-    while (!st.empty()) {
-      Todo t = st.back();
-      st.pop_back();
+    size_t head = 0;  // breadth-first while the top kTopNodesBfs slots fill, then depth-first
+    while (head < st.size()) {
+      Todo t;
+      if (N4.size() < size_t(kTopNodesBfs)) {
+        t = st[head++];
+      } else {
+        t = st.back();
+        st.pop_back();
+      }
       max_depth = std::max(max_depth, t.depth);
       int32_t ch[4] = {b.nodes[t.bnode].left, b.nodes[t.bnode].right, -1, -1};
       int nc = 2;

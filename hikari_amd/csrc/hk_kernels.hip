@@ -140,8 +140,11 @@ __device__ __forceinline__ void trav_begin(RayTrav &R, uint64_t *lds_stack, uint
 // saves the sort but finds occluders later (2-3% slower overall).
 // NF: the near / far slab test (fewer VALU, nine more live registers per
 // lane); otherwise (lo - o) * (1/d) and (hi - o) * (1/d) with min / max.
-template <bool NF>
-__device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, RayTrav &R, TraceCount &cnt) {
+// TOPN: nodes below TOPN are read from the LDS copy `top` of the first TOPN
+// nodes (0: no copy).
+template <bool NF, int TOPN = 0>
+__device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, const float4 *top, RayTrav &R,
+                                          TraceCount &cnt) {
 #ifdef HK_TRAVERSAL_STATS
   cnt.nodes++;
 #endif
@@ -150,15 +153,26 @@ __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, R
   float tn[4];
   int32_t r[4];
   if constexpr (NF) {
-    const char *nb = reinterpret_cast<const char *>(nodes);
-    uint32_t base = uint32_t(R.ref) << 7;
-    float4 nx4 = *reinterpret_cast<const float4 *>(nb + (base + R.offx));
-    float4 fx4 = *reinterpret_cast<const float4 *>(nb + (base + (R.offx ^ 16u)));
-    float4 ny4 = *reinterpret_cast<const float4 *>(nb + (base + R.offy));
-    float4 fy4 = *reinterpret_cast<const float4 *>(nb + (base + (R.offy ^ 16u)));
-    float4 nz4 = *reinterpret_cast<const float4 *>(nb + (base + R.offz));
-    float4 fz4 = *reinterpret_cast<const float4 *>(nb + (base + (R.offz ^ 16u)));
-    int4 ch = *reinterpret_cast<const int4 *>(nb + (base + 96u));
+    float4 nx4, fx4, ny4, fy4, nz4, fz4;
+    int4 ch;
+    if (TOPN > 0 && R.ref < TOPN) {
+      const float4 *q = top + 8 * R.ref;  // an LDS array: the loads are ds_read
+      nx4 = q[R.offx >> 4]; fx4 = q[(R.offx >> 4) ^ 1u];
+      ny4 = q[R.offy >> 4]; fy4 = q[(R.offy >> 4) ^ 1u];
+      nz4 = q[R.offz >> 4]; fz4 = q[(R.offz >> 4) ^ 1u];
+      float4 c4 = q[6];
+      ch = make_int4(__float_as_int(c4.x), __float_as_int(c4.y), __float_as_int(c4.z), __float_as_int(c4.w));
+    } else {
+      const char *nb = reinterpret_cast<const char *>(nodes);
+      uint32_t base = uint32_t(R.ref) << 7;
+      nx4 = *reinterpret_cast<const float4 *>(nb + (base + R.offx));
+      fx4 = *reinterpret_cast<const float4 *>(nb + (base + (R.offx ^ 16u)));
+      ny4 = *reinterpret_cast<const float4 *>(nb + (base + R.offy));
+      fy4 = *reinterpret_cast<const float4 *>(nb + (base + (R.offy ^ 16u)));
+      nz4 = *reinterpret_cast<const float4 *>(nb + (base + R.offz));
+      fz4 = *reinterpret_cast<const float4 *>(nb + (base + (R.offz ^ 16u)));
+      ch = *reinterpret_cast<const int4 *>(nb + (base + 96u));
+    }
     float nxs[4] = {nx4.x, nx4.y, nx4.z, nx4.w}, fxs[4] = {fx4.x, fx4.y, fx4.z, fx4.w};
     float nys[4] = {ny4.x, ny4.y, ny4.z, ny4.w}, fys[4] = {fy4.x, fy4.y, fy4.z, fy4.w};
     float nzs[4] = {nz4.x, nz4.y, nz4.z, nz4.w}, fzs[4] = {fz4.x, fz4.y, fz4.z, fz4.w};
@@ -171,9 +185,18 @@ __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, R
     }
   } else {
   const vec3 o = R.o;
-  const float4 *np = reinterpret_cast<const float4 *>(nodes + R.ref);
-  float4 lx4 = np[0], hx4 = np[1], ly4 = np[2], hy4 = np[3], lz4 = np[4], hz4 = np[5];
-  int4 ch = *reinterpret_cast<const int4 *>(np + 6);
+  float4 lx4, hx4, ly4, hy4, lz4, hz4;
+  int4 ch;
+  if (TOPN > 0 && R.ref < TOPN) {
+    const float4 *q = top + 8 * R.ref;  // an LDS array: the loads are ds_read
+    lx4 = q[0]; hx4 = q[1]; ly4 = q[2]; hy4 = q[3]; lz4 = q[4]; hz4 = q[5];
+    float4 c4 = q[6];
+    ch = make_int4(__float_as_int(c4.x), __float_as_int(c4.y), __float_as_int(c4.z), __float_as_int(c4.w));
+  } else {
+    const float4 *np = reinterpret_cast<const float4 *>(nodes + R.ref);
+    lx4 = np[0]; hx4 = np[1]; ly4 = np[2]; hy4 = np[3]; lz4 = np[4]; hz4 = np[5];
+    ch = *reinterpret_cast<const int4 *>(np + 6);
+  }
   float lxs[4] = {lx4.x, lx4.y, lx4.z, lx4.w}, hxs[4] = {hx4.x, hx4.y, hx4.z, hx4.w};
   float lys[4] = {ly4.x, ly4.y, ly4.z, ly4.w}, hys[4] = {hy4.x, hy4.y, hy4.z, hy4.w};
   float lzs[4] = {lz4.x, lz4.y, lz4.z, lz4.w}, hzs[4] = {hz4.x, hz4.y, hz4.z, hz4.w};
@@ -241,9 +264,10 @@ constexpr int kLeafEarly = 8;
 // Visiting nodes before a parked leaf has lowered bt only costs extra
 // visits: the result is the minimum t (largest prim among equal t) in any
 // order.
-template <bool ANY, bool NF = true>
-__device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, const TraceTri *__restrict__ tris,
-                                           RayTrav &R, int32_t &pend, TraceCount &cnt) {
+template <bool ANY, bool NF = true, int TOPN = 0>
+__device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, const float4 *top,
+                                           const TraceTri *__restrict__ tris, RayTrav &R, int32_t &pend,
+                                           TraceCount &cnt) {
   int32_t pend2 = kDone;  // closest-hit rays park a second leaf rather than wait
   for (;;) {
     if (R.ref < 0 && R.ref != kDone) {
@@ -261,7 +285,7 @@ __device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, 
     // (node steps, or at least one parked leaf tested), so the loop ends.
     int searching = __popcll(__ballot(R.ref >= 0 && pend == kDone));
     if (searching == 0 || (searching < kLeafEarly && __ballot(pend != kDone) != 0ull)) break;
-    if (R.ref >= 0) node_step<NF>(nodes, R, cnt);
+    if (R.ref >= 0) node_step<NF, TOPN>(nodes, top, R, cnt);
   }
   if (pend != kDone) {
     leaf_tris<ANY>(tris, R, pend, cnt);
@@ -663,6 +687,15 @@ constexpr uint32_t kNoRay = 0xffffffffu;
 constexpr int kExtendMinWaves = 7;
 __global__ __launch_bounds__(kBlock, kExtendMinWaves) void k_extend(RenderArgs A, int qi) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
+  // the top of the tree, read by every descent (2.5 KiB: with the stack the
+  // block's LDS stays within a seventh of the CU's 160 KiB)
+  __shared__ float4 s_top[kTopNodes * 8];
+  {
+    uint32_t n4 = uint32_t(min(A.sc.n_nodes, kTopNodes)) * 8u;
+    const float4 *src = reinterpret_cast<const float4 *>(A.sc.nodes);
+    for (uint32_t k = threadIdx.x; k < n4; k += kBlock) s_top[k] = src[k];
+    __syncthreads();
+  }
   uint32_t count = A.counters[ctr(qi)];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     A.counters[ctr(qi ^ 1)] = 0u;
@@ -696,7 +729,7 @@ __global__ __launch_bounds__(kBlock, kExtendMinWaves) void k_extend(RenderArgs A
       if (fetch_exhausted(wf)) break;
       continue;  // every lane idle: the next round fetches a full wave
     }
-    trav_round<false>(A.sc.nodes, A.sc.tris, R, pend, tc);
+    trav_round<false, true, kTopNodes>(A.sc.nodes, s_top, A.sc.tris, R, pend, tc);
     if (rid != kNoRay && R.ref == kDone && pend == kDone) {
       qh[rid] = R.bprim >= 0 ? make_float4(R.bt, R.bu, R.bv, __int_as_float(R.bprim))
                              : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
@@ -745,7 +778,7 @@ __global__ __launch_bounds__(kBlock) void k_intersect_persistent(const TraceNode
       if (fetch_exhausted(wf)) break;
       continue;
     }
-    trav_round<ANY>(nodes, tris, R, pend, tc);
+    trav_round<ANY>(nodes, nullptr, tris, R, pend, tc);
     if (rid != kNoRay && R.ref == kDone && pend == kDone) {
       hits[rid] = R.bprim >= 0 ? make_float4(R.bt, R.bu, R.bv, __int_as_float(R.bprim))
                                : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
@@ -1000,6 +1033,15 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
 // (fetch counter counters[5], cleared by k_extend).
 __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
+  // the top of the tree (10.6 KiB: at 5 waves per SIMD a block may use a
+  // fifth of the CU's 160 KiB of LDS)
+  __shared__ float4 s_top[kTopNodesShadow * 8];
+  {
+    uint32_t n4 = uint32_t(min(A.sc.n_nodes, kTopNodesShadow)) * 8u;
+    const float4 *src = reinterpret_cast<const float4 *>(A.sc.nodes);
+    for (uint32_t k = threadIdx.x; k < n4; k += kBlock) s_top[k] = src[k];
+    __syncthreads();
+  }
   uint32_t count = A.counters[ctr(2)];
   uint32_t traced = 0;
   TraceCount tc;
@@ -1050,7 +1092,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
       pend = kDone;
       traced++;
     }
-    trav_round<true, false>(A.sc.nodes, A.sc.tris, R, pend, tc);  // min / max slab test (5 waves)
+    // min / max slab test (5 waves)
+    trav_round<true, false, kTopNodesShadow>(A.sc.nodes, s_top, A.sc.tris, R, pend, tc);
     bool fin = false;
     if (ent != kNoRay && R.ref == kDone && pend == kDone) {
       if (R.bprim < 0) L = L + c;

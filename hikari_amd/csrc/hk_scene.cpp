@@ -267,6 +267,7 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
   for (int c = 0; c < 3; c++) ds.background[c] = d->background[c];
   ds.force_background = d->force_background;
   ds.root_ref = 0;
+  ds.n_nodes = int32_t(s->bvh.n_nodes);
   ds.n_mats = int32_t(mats.size());
   hip_check(hipStreamSynchronize(st), "scene upload");
   DBuf *all[] = {&s->bvh.nodes, &s->bvh.tris, &s->shade, &s->meshes, &s->mats,

@@ -61,6 +61,13 @@ static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node layout");
 
 // Node format the kernels walk: the 128-B FP32 node.
 using TraceNode = Bvh4Node;
+// The host builder lays out the first kTopNodesBfs nodes breadth-first (the
+// root and the next three levels of a full tree); the traversal kernels copy
+// the first kTopNodes (closest hit) or kTopNodesShadow (any hit) of them to
+// LDS, since every ray's descent passes through them.
+constexpr int kTopNodesBfs = 85;
+constexpr int kTopNodes = 20;
+constexpr int kTopNodesShadow = 84;
 
 // Leaf-ordered triangle for traversal (48 B): p0 + prim id, e1 = p1 - p0,
 // e2 = p2 - p0 (edges computed in FP32 on the host exactly as the reference
@@ -117,6 +124,7 @@ struct DevScene {
   float background[3];
   int32_t force_background;
   int32_t root_ref;
+  int32_t n_nodes;          // BVH nodes in `nodes`
   int32_t integrator;       // 0 pathMIS, 1 pathEMS, 2 pathMATS
   int32_t n_mats;           // records in mats
 };
