@@ -420,9 +420,11 @@ def main():
     if os.path.exists(tp):
         with open(tp) as fh:
             tj = json.load(fh)
-        if tj.get("hbm_bytes_per_ray"):
-            roofline["traffic"] = tj["hbm_bytes_per_ray"] * rays_per_launch
-            fp["traffic"] = {"hbm_bytes_per_ray": tj["hbm_bytes_per_ray"], "commit": tj.get("commit"),
+        wl = "c2" if a.workload == "c2" else "c1"
+        b_ray = tj.get("per_kernel", {}).get(f"{wl}_k_extend", {}).get("hbm_bytes_per_ray", tj.get("hbm_bytes_per_ray"))
+        if b_ray:
+            roofline["traffic"] = b_ray * rays_per_launch
+            fp["traffic"] = {"hbm_bytes_per_ray": b_ray, "commit": tj.get("commit"),
                              "source": "profiles/k_extend_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE)"}
     tv = os.path.join(ROOT, "profiles", "k_extend_traversal.json")
     if os.path.exists(tv) and ext_avg_ms > 0:
