@@ -35,6 +35,10 @@ constexpr int kLdsStack = HK_LDS_STACK;  // stack entries per lane kept in LDS
 constexpr int kPrivStack = 96 - kLdsStack;  // further entries in private memory
 constexpr float kInfF = __builtin_huge_valf();
 constexpr int32_t kDone = kEmptyRef;
+// A lane whose next reference is on its stack (a leaf ref is ~code with code
+// < 2^31 - 64, so it never takes this value).
+constexpr int32_t kPop = kEmptyRef + 1;
+__device__ __forceinline__ bool is_leaf(int32_t ref) { return ref < 0 && ref > kPop; }
 
 // ------------------------------------------------------------------ traversal
 // Traversal stack of one lane: the top kLdsStack entries in LDS, the rest in
@@ -54,22 +58,25 @@ struct TraceStack {
     else priv[sp - kLdsStack] = e;
     sp++;
   }
-  // The empty asm statement on the overflow branch keeps the two loads in
-  // separate branches (a ds_read and a scratch load); without it the compiler
-  // merges them into one generic-address load that is slow for LDS.
+  // One pop per call: the top entry's ref if its tnear is within tlim; a
+  // culled entry leaves kPop (more entries below) or kDone (stack empty), so
+  // a lane skips culled entries one per traversal iteration instead of the
+  // whole wave waiting in a loop for its slowest lane.  The empty asm
+  // statement on the overflow branch keeps the two loads in separate branches
+  // (a ds_read and a scratch load); without it the compiler merges them into
+  // one generic-address load that is slow for LDS.
   __device__ __forceinline__ int32_t pop(float tlim) {
-    while (sp > 0) {
-      sp--;
-      uint64_t e;
-      if (sp < kLdsStack) {
-        e = lds[sp * kBlock + tid];
-      } else {
-        e = priv[sp - kLdsStack];
-        __asm__ volatile("" : "+v"(e));
-      }
-      if (__uint_as_float((uint32_t)(e >> 32)) <= tlim) return (int32_t)(uint32_t)e;
+    if (sp == 0) return kDone;
+    sp--;
+    uint64_t e;
+    if (sp < kLdsStack) {
+      e = lds[sp * kBlock + tid];
+    } else {
+      e = priv[sp - kLdsStack];
+      __asm__ volatile("" : "+v"(e));
     }
-    return kDone;
+    if (__uint_as_float((uint32_t)(e >> 32)) <= tlim) return (int32_t)(uint32_t)e;
+    return sp > 0 ? kPop : kDone;
   }
 };
 
@@ -78,10 +85,11 @@ struct TraceCount {
 };
 
 // One lane's traversal state (closest or any hit over the BVH4).  Internal
-// nodes are processed by node_step, leaves by leaf_step; a lane is done when
-// ref == kDone.  Slab tests are widened by a few ulp (tnear <= tfar *
-// (1 + 2^-21)), which covers the rounding of (lo - o) * (1/d); the host pads
-// triangle boxes for the inexactness of the triangle test.
+// nodes are processed by node_step, leaves by leaf_tris; a lane is done when
+// ref == kDone (kPop: its next reference is on the stack).  Slab tests are
+// widened by a few ulp (tnear <= tfar + |tfar| 2^-21), which covers the
+// rounding of (lo - o) * (1/d); the host pads triangle boxes for the
+// inexactness of the triangle test.
 //
 // Near / far slab test (node_step<true>): per axis the ray's sign of 1/d
 // picks which stored plane is the entry (near) and which the exit (far)
@@ -92,7 +100,10 @@ struct TraceCount {
 // the FMA's own final rounding, which the widening covers.  Where o * (1/d)
 // overflows, cu or cd is infinite or NaN and that axis stops constraining
 // the box (fmaxf / fminf drop NaN), which is conservative.
-constexpr float kWiden = 1.0000005f;
+// x + |x| 2^-21: for x >= 0 bit for bit x * (1 + 2^-21) (one rounding of the
+// same exact value); for x < 0 it moves x up, where a product would move it
+// down and reject boxes of rays whose t range is negative.
+__device__ __forceinline__ float widen(float x) { return fmaf(fabsf(x), 4.76837158e-7f, x); }
 struct RayTrav {
   vec3 o, d;
   float ix, iy, iz, tmin;
@@ -149,7 +160,7 @@ __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, c
   cnt.nodes++;
 #endif
   const float ix = R.ix, iy = R.iy, iz = R.iz;
-  float tlim = R.bt * kWiden;
+  float tlim = widen(R.bt);
   float tn[4];
   int32_t r[4];
   if constexpr (NF) {
@@ -181,7 +192,7 @@ __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, c
     for (int c = 0; c < 4; c++) {
       float a = fmaxf(fmaxf(fmaf(nxs[c], ix, -R.cux), fmaf(nys[c], iy, -R.cuy)), fmaxf(fmaf(nzs[c], iz, -R.cuz), R.tmin));
       float b = fminf(fminf(fmaf(fxs[c], ix, -R.cdx), fmaf(fys[c], iy, -R.cdy)), fminf(fmaf(fzs[c], iz, -R.cdz), tlim));
-      tn[c] = (r[c] != kEmptyRef && a <= b * kWiden) ? a : kInfF;
+      tn[c] = (r[c] != kEmptyRef && a <= widen(b)) ? a : kInfF;
     }
   } else {
   const vec3 o = R.o;
@@ -208,7 +219,7 @@ __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, c
     float t0z = (lzs[c] - o.z) * iz, t1z = (hzs[c] - o.z) * iz;
     float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), R.tmin));
     float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tlim));
-    tn[c] = (r[c] != kEmptyRef && a <= b * kWiden) ? a : kInfF;
+    tn[c] = (r[c] != kEmptyRef && a <= widen(b)) ? a : kInfF;
   }
   }
 #define HK_CSWAP(i, j)                                    \
@@ -221,7 +232,7 @@ __device__ __forceinline__ void node_step(const TraceNode *__restrict__ nodes, c
   if (tn[3] != kInfF) R.st.push(r[3], tn[3]);
   if (tn[2] != kInfF) R.st.push(r[2], tn[2]);
   if (tn[1] != kInfF) R.st.push(r[1], tn[1]);
-  R.ref = tn[0] != kInfF ? r[0] : R.st.pop(R.bt * kWiden);
+  R.ref = tn[0] != kInfF ? r[0] : (R.st.sp > 0 ? kPop : kDone);
 }
 
 // The triangles of leaf `leaf`: exact FP32 Moller-Trumbore (no stack pop).
@@ -270,20 +281,25 @@ __device__ __forceinline__ void trav_round(const TraceNode *__restrict__ nodes, 
                                            TraceCount &cnt) {
   int32_t pend2 = kDone;  // closest-hit rays park a second leaf rather than wait
   for (;;) {
-    if (R.ref < 0 && R.ref != kDone) {
+    // park a leaf if a slot is free, then at most one stack read per lane
+    // (one pop site: each site costs the wave its instructions whenever any
+    // lane needs it)
+    if (is_leaf(R.ref)) {
       if (pend == kDone) {
         pend = R.ref;
-        R.ref = R.st.pop(R.bt * kWiden);
+        R.ref = kPop;
       } else if (!ANY && pend2 == kDone) {
         pend2 = R.ref;
-        R.ref = R.st.pop(R.bt * kWiden);
+        R.ref = kPop;
       }
     }
+    if (R.ref == kPop) R.ref = R.st.pop(widen(R.bt));
     // The leaf phase starts once fewer than kLeafEarly lanes still look for a
     // first leaf and some lane holds one: a few searching lanes no longer
     // keep the rest of the wave descending.  Every round makes progress
-    // (node steps, or at least one parked leaf tested), so the loop ends.
-    int searching = __popcll(__ballot(R.ref >= 0 && pend == kDone));
+    // (node steps, stack pops, or at least one parked leaf tested), so the
+    // loop ends.
+    int searching = __popcll(__ballot((R.ref >= 0 || R.ref == kPop) && pend == kDone));
     if (searching == 0 || (searching < kLeafEarly && __ballot(pend != kDone) != 0ull)) break;
     if (R.ref >= 0) node_step<NF, TOPN>(nodes, top, R, cnt);
   }
@@ -746,9 +762,10 @@ __global__ __launch_bounds__(kBlock, kExtendMinWaves) void k_extend(RenderArgs A
 // persistent waves that refill from reserved ranges of the ray array
 // (counter: one zeroed word) and speculative traversal rounds.  Closest hits
 // equal brute force bit for bit (the closest hit does not depend on the visit
-// order); any-hit queries agree on whether a hit exists.
+// order); any-hit queries agree on whether a hit exists.  Held at 6 waves
+// per SIMD (at most 80 VGPRs).
 template <bool ANY>
-__global__ __launch_bounds__(kBlock) void k_intersect_persistent(const TraceNode *__restrict__ nodes,
+__global__ __launch_bounds__(kBlock, 6) void k_intersect_persistent(const TraceNode *__restrict__ nodes,
                                                                  const TraceTri *__restrict__ tris, int32_t root,
                                                                  const float4 *__restrict__ rays,
                                                                  float4 *__restrict__ hits, uint32_t n,

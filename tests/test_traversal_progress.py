@@ -3,24 +3,27 @@
 A model of one wave (64 lanes) running the persistent traversal of
 hikari_amd/csrc/hk_kernels.hip: the refill of idle lanes (k_extend /
 k_intersect_persistent), the node phase of trav_round with its wave ballots
-(a lane parks one leaf, closest-hit lanes a second; the phase ends when no
-lane searches for a first leaf, or when fewer than kLeafEarly do and some
-lane holds a leaf), then the leaf phase.  Each lane's ray is a random finite
+(a lane parks one leaf, closest-hit lanes a second; a lane whose next
+reference is on its stack pops one entry per iteration, and a culled entry
+leaves it popping again in the next iteration; the phase ends when no lane
+searches for a first leaf, or when fewer than kLeafEarly do and some lane
+holds a leaf), then the leaf phase.  Each lane's ray is a random finite
 traversal: every node step or stack pop yields the next reference (an inner
-node, a leaf, or the end of the ray), and any-hit leaves may end the ray.
+node, a leaf, another pop, or the end of the ray), and any-hit leaves may end
+the ray.
 
 The property the kernel relies on: every round of the wave makes progress
-(a node step, a leaf test, or a finished ray), so the wave drains in a
-bounded number of rounds.  The same model without the "some lane holds a
-leaf" condition of the early leaf phase stalls, which is the hang that
-condition fixed.
+(a node step, a stack pop, a leaf test, or a finished ray), so the wave
+drains in a bounded number of rounds.  The same model without the "some lane
+holds a leaf" condition of the early leaf phase stalls, which is the hang
+that condition fixed.
 """
 import numpy as np
 import pytest
 
 LANES = 64
 K_LEAF_EARLY = 8  # kLeafEarly
-NODE, LEAF, DONE = 0, 1, 2
+NODE, LEAF, DONE, POP = 0, 1, 2, 3
 
 
 class Ray:
@@ -28,7 +31,7 @@ class Ray:
 
     def __init__(self, rng):
         n = int(rng.integers(1, 40))
-        kinds = rng.choice([NODE, LEAF], size=n, p=[0.7, 0.3])
+        kinds = rng.choice([NODE, LEAF, POP], size=n, p=[0.55, 0.25, 0.2])
         self.refs = list(kinds) + [DONE]
         self.pos = 0
         self.hit_p = float(rng.uniform(0.0, 0.5))
@@ -45,22 +48,24 @@ def trav_round(lanes, any_hit, early_needs_leaf, rng, stats):
     for ln in lanes:
         ln["pend2"] = DONE
     while True:
-        for ln in lanes:                               # park a leaf, then pop
+        for ln in lanes:                               # park a leaf, then one pop
             if ln["ray"] is None:
                 continue
             if ln["ref"] == LEAF:
                 if ln["pend"] == DONE:
-                    ln["pend"] = LEAF
-                    ln["ref"] = ln["ray"].next()
+                    ln["pend"], ln["ref"] = LEAF, POP
                 elif not any_hit and ln["pend2"] == DONE:
-                    ln["pend2"] = LEAF
-                    ln["ref"] = ln["ray"].next()
-        searching = sum(1 for ln in lanes if ln["ray"] is not None and ln["ref"] == NODE and ln["pend"] == DONE)
+                    ln["pend2"], ln["ref"] = LEAF, POP
+            if ln["ref"] == POP:                       # a culled entry yields POP again
+                ln["ref"] = ln["ray"].next()
+                stats["pops"] += 1
+        searching = sum(1 for ln in lanes
+                        if ln["ray"] is not None and ln["ref"] in (NODE, POP) and ln["pend"] == DONE)
         holds = any(ln["ray"] is not None and ln["pend"] != DONE for ln in lanes)
         early = searching < K_LEAF_EARLY and (holds or not early_needs_leaf)
         if searching == 0 or early:
             break
-        for ln in lanes:                               # node_step
+        for ln in lanes:                               # node_step: a child, POP or DONE
             if ln["ray"] is not None and ln["ref"] == NODE:
                 ln["ref"] = ln["ray"].next()
                 stats["nodes"] += 1
@@ -93,7 +98,7 @@ def run_wave(seed, any_hit, early_needs_leaf=True, n_rays=600, refill=24, max_ro
             if next_ray == n_rays:
                 return rounds, False
             continue
-        stats = {"nodes": 0, "leaves": 0}
+        stats = {"nodes": 0, "leaves": 0, "pops": 0}
         trav_round(lanes, any_hit, early_needs_leaf, rng, stats)
         done_now = 0
         for ln in lanes:                               # a finished ray leaves its lane
@@ -102,7 +107,7 @@ def run_wave(seed, any_hit, early_needs_leaf=True, n_rays=600, refill=24, max_ro
                 done_now += 1
         finished += done_now
         rounds += 1
-        if stats["nodes"] == 0 and stats["leaves"] == 0 and done_now == 0 and started == 0:
+        if stats["nodes"] == 0 and stats["leaves"] == 0 and stats["pops"] == 0 and done_now == 0 and started == 0:
             return rounds, True
         if rounds > max_rounds:
             return rounds, True
