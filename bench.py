@@ -78,7 +78,7 @@ def parse():
     ap.add_argument("--workload", choices=["c1", "c2", "batch", "serve", "accel"], default="c1")
     ap.add_argument("--requests", type=int, default=0, help="batch/serve: number of requests (default 64 / 40)")
     ap.add_argument("--qps", type=float, default=10.0, help="serve: Poisson arrival rate")
-    ap.add_argument("--budget-gb", type=float, default=96.0, help="serve: HBM budget of the server")
+    ap.add_argument("--budget-gb", type=float, default=160.0, help="serve: HBM budget of the server")
     ap.add_argument("--schedule", type=int, default=1, help="batch: 1 fewest samples left first (library "
                     "default), 0 first come first served")
     ap.add_argument("--detail", default="", help="batch / serve: write per-request timings (JSON) to this file")

@@ -447,11 +447,11 @@ int hk_set_option(hk_ctx *ctx, const char *name, int64_t value) {
     ctx->grid = uint32_t(value);
   } else if (n == "refill" || n == "refill_shadow") {
     if (value < 0 || value > 64) throw HkError(HK_ERR_INVALID, "%s: 1..64 (0: default)", name);
-    uint32_t v = value == 0 ? 24u : uint32_t(value);
+    uint32_t v = value == 0 ? (n == "refill" ? kDefaultRefill : kDefaultRefillShadow) : uint32_t(value);
     (n == "refill" ? ctx->refill_extend : ctx->refill_shadow) = v;
   } else if (n == "fetch_batch") {
     if (value < 0 || value > 64) throw HkError(HK_ERR_INVALID, "fetch_batch: 1..64 (0: default)");
-    ctx->fetch_batch = value == 0 ? 2u : uint32_t(value);
+    ctx->fetch_batch = value == 0 ? kDefaultFetchBatch : uint32_t(value);
   } else if (n == "schedule") {
     if (value < 0 || value > 1) throw HkError(HK_ERR_INVALID, "schedule: 0 (FIFO) or 1 (shortest first)");
     ctx->schedule = int(value);

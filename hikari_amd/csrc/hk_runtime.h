@@ -44,8 +44,11 @@ struct DevBvh {
 // floats on the host.  Synchronizes `st`.
 void build_bvh4_device(hipStream_t st, const float *tri9, uint32_t n, DevBvh &out);
 
-constexpr uint32_t kDefaultMaxPaths = 1u << 25;     // path slots of the wavefront (~250-350 B of HBM each)
+constexpr uint32_t kDefaultMaxPaths = 1u << 26;     // path slots of the wavefront (~250-350 B of HBM each)
 constexpr uint32_t kDefaultMaxSamples = 1u << 28;   // samples per segment (16 B each in HBM)
+// persistent traversal waves: idle lanes that trigger a refill, and queue
+// entries (in 64s) reserved per atomic on the fetch counter
+constexpr uint32_t kDefaultRefill = 40, kDefaultRefillShadow = 16, kDefaultFetchBatch = 4;
 
 // Owned pinned host allocation (staging for small per-bounce copies).
 struct PinnedBuf {
@@ -91,9 +94,9 @@ struct hk_ctx {
   bool resolve_tiles = true;  // film resolve through LDS tiles (false: the plain gather)
   uint32_t tile = 8;          // camera-ray tile edge (1: scanlines)
   bool graph = true;          // replay each bounce as a captured HIP graph
-  uint32_t refill_extend = 24;  // idle lanes that make a persistent k_extend wave fetch rays
-  uint32_t refill_shadow = 24;  // the same for k_shadow
-  uint32_t fetch_batch = 2;     // queue entries a persistent wave reserves per atomic, in 64s
+  uint32_t refill_extend = hk::kDefaultRefill;       // idle lanes that make a persistent k_extend wave fetch rays
+  uint32_t refill_shadow = hk::kDefaultRefillShadow; // the same for k_shadow
+  uint32_t fetch_batch = hk::kDefaultFetchBatch;     // queue entries a persistent wave reserves per atomic, in 64s
   int schedule = 1;             // chunk admission / regeneration order: 0 FIFO, 1 shortest first
   bool profile = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;

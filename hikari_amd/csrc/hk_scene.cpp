@@ -376,19 +376,19 @@ int hk_autotune(hk_scene *s, const hk_request *req, uint64_t calib_samples, hk_t
       }
     };
     uint64_t total = npix * cal.spp;
-    for (uint32_t p : {1u << 22, 1u << 23, 1u << 24, 1u << 25}) {
+    for (uint32_t p : {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26}) {
       if (p > total && p != (1u << 22)) break;  // a wavefront larger than the work is never filled
       Setting t = best;
       t.paths = p;
       consider(t);
     }
-    for (uint32_t fb : {1u, 2u, 4u}) {
+    for (uint32_t fb : {1u, 2u, 4u, 8u}) {
       if (fb == best.fb) continue;
       Setting t = best;
       t.fb = fb;
       consider(t);
     }
-    for (uint32_t re : {16u, 24u, 32u, 48u}) {
+    for (uint32_t re : {24u, 32u, 40u, 48u}) {
       if (re == best.re) continue;
       Setting t = best;
       t.re = re;

@@ -283,17 +283,17 @@ int hk_server_get_stats(hk_server *server, hk_server_stats *stats);
 int hk_server_destroy(hk_server *server);
 
 /* Tuning knobs (value 0 restores the default):
- *   "max_paths"   paths in flight (default 32M); at every bounce finished
+ *   "max_paths"   paths in flight (default 64M); at every bounce finished
  *                 paths are replaced by the next samples
  *   "max_samples" samples per segment (default 256M, 16 B of HBM each): a
  *                 request's samples are cut into chunks of
  *                 floor(max_samples / pixels) samples per pixel, resolved
  *                 into the film when their segment ends
  *   "grid"        cap on blocks per queue-kernel launch (0: one lane per item)
- *   "refill"      idle lanes (1..64, default 24) that make a wave of the
+ *   "refill"      idle lanes (1..64, default 40) that make a wave of the
  *                 persistent closest-hit kernel fetch new rays
- *   "refill_shadow" the same for the shadow-ray kernel
- *   "fetch_batch" queue entries (in 64s, 1..64, default 2) a wave of the
+ *   "refill_shadow" the same for the shadow-ray kernel (default 16)
+ *   "fetch_batch" queue entries (in 64s, 1..64, default 4) a wave of the
  *                 persistent kernels reserves per atomic on the shared fetch
  *                 counter
  *   "schedule"    order in which chunks are admitted and their samples
