@@ -15,9 +15,10 @@ Default run (one JSON line):
     against the same requests rendered one at a time with hk_render, images
     compared bit for bit, with the plain path's rate per film size;
   * serve: Poisson arrivals over three resident scenes (box, displaced
-    sphere, full heightfield) through hk_server against a first-come-first-
-    served queue of plain hk_render calls on the same trace: p50/p99 latency
-    and path samples/s of both;
+    sphere, full heightfield) through hk_server, with its express lanes (the
+    default) and without them (option "express" = 0), against a first-come-
+    first-served queue of plain hk_render calls on the same trace: p50/p99
+    latency and path samples/s of each;
   * cpu_baseline: the oracle (the C++ restatement) on the host cores on a
     bounded sample of the displaced-sphere workload, with its measured ratio
     to the reference renderer on the box scene.
@@ -158,26 +159,15 @@ def batch_leg(gs, n_requests, schedule, detail=""):
             "schedule": ["fifo", "shortest_first"][schedule]}
 
 
-def serve_leg(ctx, gss, trace, budget_gb, detail=""):
-    """Poisson arrivals through hk_server vs a FCFS queue of plain renders."""
+def _run_server(ctx, gss, trace, budget_gb, express):
+    """The trace through hk_server at its arrival times; per-request rows."""
     import threading
     import hikari_amd as hk
-    total = sum(r.width * r.height * r.spp for _, _, r in trace)
-    # plain: each request alone; first come first served on one queue
-    service = []
-    for _, sid, r in trace:
-        t0 = time.perf_counter()
-        gss[sid].render(0, request=r)
-        service.append(time.perf_counter() - t0)
-    t_free, lat_plain = 0.0, []
-    for (arr, _, _), sv in zip(trace, service):
-        t_free = max(arr, t_free) + sv
-        lat_plain.append(t_free - arr)
-    span_plain = t_free - trace[0][0]
-    log(f"serve: plain FCFS busy {sum(service):.2f} s, span {span_plain:.2f} s")
+    ctx.set_option("express", express)
     srv = hk.Server(ctx, hbm_budget=int(budget_gb * 2 ** 30))
     for g in gss:
         srv.add_scene(g)
+    ctx.set_option("express", 1)
     tickets = [None] * len(trace)
     t_start = time.perf_counter()
 
@@ -199,27 +189,55 @@ def serve_leg(ctx, gss, trace, budget_gb, detail=""):
         t_last = max(t_last, time.perf_counter() - t_start)
         arr, sid, r = trace[i]
         rows.append({"scene": sid, "pixels": r.width * r.height, "spp": r.spp, "arrival_s": arr, "lane": res["lane"],
-                     "alone_ms": service[i] * 1e3, "server_ms": res["latency_ms"], "queue_ms": res["queue_ms"],
-                     "submit_ms": res["submit_ms"], "plain_fcfs_ms": lat_plain[i] * 1e3})
+                     "server_ms": res["latency_ms"], "queue_ms": res["queue_ms"], "submit_ms": res["submit_ms"]})
     th.join()
-    span = t_last - trace[0][0]
     stats = srv.stats()
     srv.close()
+    return lat, t_last - trace[0][0], stats, rows
+
+
+def serve_leg(ctx, gss, trace, budget_gb, detail=""):
+    """Poisson arrivals through hk_server, with and without its express
+    lanes, vs a FCFS queue of plain renders."""
+    total = sum(r.width * r.height * r.spp for _, _, r in trace)
+    # plain: each request alone; first come first served on one queue
+    service = []
+    for _, sid, r in trace:
+        t0 = time.perf_counter()
+        gss[sid].render(0, request=r)
+        service.append(time.perf_counter() - t0)
+    t_free, lat_plain = 0.0, []
+    for (arr, _, _), sv in zip(trace, service):
+        t_free = max(arr, t_free) + sv
+        lat_plain.append(t_free - arr)
+    span_plain = t_free - trace[0][0]
+    log(f"serve: plain FCFS busy {sum(service):.2f} s, span {span_plain:.2f} s")
+    out = {}
+    for express in (1, 0):
+        lat, span, stats, rows = _run_server(ctx, gss, trace, budget_gb, express)
+        for row, sv, lp in zip(rows, service, lat_plain):
+            row["alone_ms"], row["plain_fcfs_ms"] = sv * 1e3, lp * 1e3
+        log(f"serve: server (express lanes {'on' if express else 'off'}) span {span:.2f} s, "
+            f"p50 {_pct(lat, 50) * 1e3:.0f} ms, p99 {_pct(lat, 99) * 1e3:.0f} ms "
+            f"(plain FCFS p50 {_pct(lat_plain, 50) * 1e3:.0f} ms, p99 {_pct(lat_plain, 99) * 1e3:.0f} ms)")
+        out[express] = {"value": total / span, "span_s": span,
+                        "latency_ms": {"p50": _pct(lat, 50) * 1e3, "p99": _pct(lat, 99) * 1e3},
+                        "served_over_plain": span_plain / span,
+                        "express_requests": sum(1 for x in rows if x["lane"] == 1),
+                        "server": {"bounces": stats["bounces"], "shared_bounces": stats["shared_bounces"],
+                                   "max_in_flight": stats["max_in_flight"], "fixed_gb": stats["fixed_bytes"] / 2 ** 30,
+                                   "hbm_budget_gb": budget_gb},
+                        "rows": rows}
     if detail:
         with open(detail, "w") as fh:
-            json.dump(rows, fh)
-    log(f"serve: server span {span:.2f} s, p50 {_pct(lat, 50) * 1e3:.0f} ms, p99 {_pct(lat, 99) * 1e3:.0f} ms "
-        f"(plain FCFS p50 {_pct(lat_plain, 50) * 1e3:.0f} ms, p99 {_pct(lat_plain, 99) * 1e3:.0f} ms)")
-    return {"value": total / span, "unit": "path samples/s", "requests": len(trace), "path_samples": total,
-            "qps": len(trace) / trace[-1][0], "span_s": span,
-            "latency_ms": {"p50": _pct(lat, 50) * 1e3, "p99": _pct(lat, 99) * 1e3},
-            "plain_fcfs": {"value": total / span_plain, "span_s": span_plain, "busy_s": sum(service),
-                           "latency_ms": {"p50": _pct(lat_plain, 50) * 1e3, "p99": _pct(lat_plain, 99) * 1e3}},
-            "served_over_plain": span_plain / span,
-            "express_requests": sum(1 for x in rows if x["lane"] == 1),
-            "server": {"bounces": stats["bounces"], "shared_bounces": stats["shared_bounces"],
-                       "max_in_flight": stats["max_in_flight"], "fixed_gb": stats["fixed_bytes"] / 2 ** 30,
-                       "hbm_budget_gb": budget_gb}}
+            json.dump({"express": out[1]["rows"], "fcfs_lanes": out[0]["rows"]}, fh)
+    res = {k: v for k, v in out[1].items() if k != "rows"}
+    res.update({"unit": "path samples/s", "requests": len(trace), "path_samples": total,
+                "qps": len(trace) / trace[-1][0],
+                "plain_fcfs": {"value": total / span_plain, "span_s": span_plain, "busy_s": sum(service),
+                               "latency_ms": {"p50": _pct(lat_plain, 50) * 1e3, "p99": _pct(lat_plain, 99) * 1e3}},
+                "express_off": {k: v for k, v in out[0].items() if k not in ("rows", "express_requests")}})
+    return res
 
 
 def serve_scenes(sphere):

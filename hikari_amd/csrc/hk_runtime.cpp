@@ -455,6 +455,9 @@ int hk_set_option(hk_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "schedule") {
     if (value < 0 || value > 1) throw HkError(HK_ERR_INVALID, "schedule: 0 (FIFO) or 1 (shortest first)");
     ctx->schedule = int(value);
+  } else if (n == "express") {
+    if (value < 0 || value > 1) throw HkError(HK_ERR_INVALID, "express: 0 (every request on the bulk lane) or 1");
+    ctx->express = value != 0;
   } else if (n == "graph") {
     ctx->graph = value != 0;
   } else if (n == "tile") {

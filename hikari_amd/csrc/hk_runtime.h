@@ -98,6 +98,7 @@ struct hk_ctx {
   uint32_t refill_shadow = hk::kDefaultRefillShadow; // the same for k_shadow
   uint32_t fetch_batch = hk::kDefaultFetchBatch;     // queue entries a persistent wave reserves per atomic, in 64s
   int schedule = 1;             // chunk admission / regeneration order: 0 FIFO, 1 shortest first
+  bool express = true;          // server: small requests take an express lane beside the bulk lane
   bool profile = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   uint32_t *host_counters = nullptr;  // pinned

@@ -6,7 +6,8 @@
 //     4 x max_samples holding up to four chunks of max_samples;
 //   * the express lane: a wavefront of max_paths / 16 and a buffer of
 //     max_samples / 8, for requests of at most half that many samples,
-//     fewest samples left first.
+//     fewest samples left first (context option "express" = 0 turns it off:
+//     every request then waits its turn on the bulk lane).
 // Bulk requests run one at a time, in arrival order over all scenes, as a
 // first-come-first-served queue of plain renders would run them.  A request
 // finishes only when its longest Russian-roulette path does; sharing a
@@ -325,10 +326,12 @@ int hk_server_add_scene(hk_server *srv, hk_scene *scene, int32_t *scene_id) {
   sc->scene = scene;
   uint64_t bulk_samples = std::min<uint64_t>(uint64_t(ctx->max_samples) * 4, 0x7fffffffu);
   sc->lane[kBulk] = make_lane(ctx, scene, ctx->max_paths, uint32_t(bulk_samples), ctx->max_samples, false);
-  uint32_t ex_samples = std::max<uint32_t>(1, ctx->max_samples / 8);
-  sc->lane[kExpress] = make_lane(ctx, scene, std::max<uint32_t>(1, ctx->max_paths / 16), ex_samples, ex_samples,
-                                 true);
-  sc->lane[kExpress]->max_request = ex_samples / 2;
+  // express lane (option "express" = 0: a placeholder that takes nothing, so
+  // every request runs on the bulk lane in arrival order)
+  uint32_t ex_samples = ctx->express ? std::max<uint32_t>(1, ctx->max_samples / 8) : 1u;
+  uint32_t ex_paths = ctx->express ? std::max<uint32_t>(1, ctx->max_paths / 16) : 1u;
+  sc->lane[kExpress] = make_lane(ctx, scene, ex_paths, ex_samples, ex_samples, true);
+  sc->lane[kExpress]->max_request = ctx->express ? ex_samples / 2 : 0;
   uint64_t ws_bytes = sc->lane[kBulk]->ws.device_bytes() + sc->lane[kExpress]->ws.device_bytes();
   if (srv->budget && srv->fixed_bytes + ws_bytes > srv->budget)
     throw HkError(HK_ERR_BUDGET, "scene workspaces (%llu B) exceed the HBM budget (%llu B)",
@@ -354,7 +357,7 @@ int hk_server_submit(hk_server *srv, int32_t scene_id, const hk_request *req, fl
   uint64_t w = uint64_t(req->use_scene_camera ? d.width : req->width);
   uint64_t h = uint64_t(req->use_scene_camera ? d.height : req->height);
   const Lane &ex = *sc.lane[kExpress];
-  int lane = (w * h * req->spp <= ex.max_request && w * h <= ex.chunk_cap) ? kExpress : kBulk;
+  int lane = (ex.max_request > 0 && w * h * req->spp <= ex.max_request && w * h <= ex.chunk_cap) ? kExpress : kBulk;
   std::unique_ptr<RequestRec> r = prepare_request(sc.scene, *req, out_rgb, sc.lane[lane]->chunk_cap);
   uint64_t id = srv->next_ticket++;
   r->id = id;

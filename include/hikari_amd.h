@@ -301,6 +301,11 @@ int hk_server_destroy(hk_server *server);
  *                 left first (a chunk that does not fit is passed over for
  *                 smaller ones); 0 first come first served.  Films do not
  *                 depend on it
+ *   "express"     read by a server when a scene is added: 1 (default) requests of at
+ *                 most max_samples / 16 samples take an express lane beside
+ *                 the bulk lane (low median latency); 0: every request waits
+ *                 its turn on its scene's bulk lane, first come first served
+ *                 (no interference with large requests: lowest tail latency)
  *   "graph"       1 (default): after a few eager bounces a render replays
  *                 each bounce as a captured HIP graph; 0: eager launches
  *                 (profiled renders always launch eagerly)

@@ -401,6 +401,29 @@ def test_server_results_equal_alone(schedule):
     c.close()
 
 
+def test_server_without_express_lanes():
+    """Option "express" = 0: every request waits its turn on its scene's bulk
+    lane (lane 0), and films still equal hk_render of each request alone."""
+    box = scenes.cbox(width=32, height=24, sphere_subdiv=2)
+    field = scenes.heightfield(width=28, height=20, n=24)
+    c = hk.Context(0)
+    c.set_option("max_paths", 3000)
+    c.set_option("express", 0)
+    g_box, g_field = c.scene(box), c.scene(field)
+    reqs = _serve_requests(box, field)
+    alone = [(g_box if sid == 0 else g_field).render(0, request=r)[0] for sid, r in reqs]
+    srv = hk.Server(c)
+    srv.add_scene(g_box)
+    srv.add_scene(g_field)
+    tickets = [srv.submit(sid, r) for sid, r in reqs]
+    for t, a, (sid, r) in zip(tickets, alone, reqs):
+        img, res = srv.wait(t)
+        assert res["status"] == 0 and res["lane"] == 0
+        assert np.array_equal(img.view(np.uint32), a.view(np.uint32)), (sid, r.spp, r.seed)
+    srv.close()
+    c.close()
+
+
 def test_server_hbm_budget_admission():
     """A budget that holds the workspaces plus one film serves requests one at
     a time; a film larger than the whole budget is refused with HK_ERR_BUDGET."""
