@@ -23,7 +23,7 @@ Default run (one JSON line):
     bounded sample of the displaced-sphere workload, with its measured ratio
     to the reference renderer on the box scene.
 
---workload c2|batch|serve|accel runs one leg alone (its own JSON line).
+--workload c2|batch|serve|accel|copy runs one leg alone (its own JSON line).
 """
 import argparse
 import json
@@ -76,7 +76,7 @@ def parse():
     ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
                     help="BVH construction: binned SAH on the host or linear BVH on the device")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["c1", "c2", "batch", "serve", "accel"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "batch", "serve", "accel", "copy"], default="c1")
     ap.add_argument("--requests", type=int, default=0, help="batch/serve: number of requests (default 64 / 40)")
     ap.add_argument("--qps", type=float, default=10.0, help="serve: Poisson arrival rate")
     ap.add_argument("--budget-gb", type=float, default=160.0, help="serve: HBM budget of the server")
@@ -327,6 +327,28 @@ def bench_accel(a):
     print(json.dumps(line), flush=True)
 
 
+def bench_copy(a):
+    """Device-to-device copy of 256 MiB (the runtime's copy kernel): the
+    reference launch for scaling the FETCH_SIZE / WRITE_SIZE counters of the
+    render kernels to bytes, and the copy bandwidth the HBM roofline is read
+    against."""
+    import torch
+    n = 256 << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    print(json.dumps({"metric": "device copy GB/s (read + write)", "value": 2 * n / dt / 1e9, "unit": "GB/s",
+                      "n_gpus": 1, "steps": a.steps, "warmup": 1, "ms_per_step": dt * 1e3, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+                      "config": {"workload": "256 MiB device-to-device copy"}}), flush=True)
+
+
 def relmse_leg(scene_fn, alpha):
     """The workload's scene on a reduced film (64 x 48, 16 spp): GPU vs the
     oracle at equal per-path streams (same random paths), so that the
@@ -357,6 +379,8 @@ def main():
         return bench_serve(a)
     if a.workload == "accel":
         return bench_accel(a)
+    if a.workload == "copy":
+        return bench_copy(a)
     import hikari_amd as hk
     from hikari_amd import scenes, workloads
 
