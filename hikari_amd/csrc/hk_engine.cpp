@@ -48,25 +48,39 @@ void Workspace::ensure(uint32_t paths, uint32_t emitters, uint32_t samples) {
     sample_L.alloc(size_t(samples) * 16);
     cap_samples = samples;
   }
-  if (paths <= cap_paths && emitters <= cap_emitters) return;
-  paths = std::max(paths, cap_paths);
-  emitters = std::max(emitters, cap_emitters);
-  size_t P = paths;
-  for (int i = 0; i < 2; i++) {
-    q_o[i].alloc(P * 16); q_d[i].alloc(P * 16); q_beta[i].alloc(P * 16); q_L[i].alloc(P * 16);
-    q_aux[i].alloc(P * 16); q_meta[i].alloc(P * 4);
+  if (paths > cap_paths) {
+    size_t P = paths;
+    for (int i = 0; i < 2; i++) {
+      q_o[i].alloc(P * 16); q_d[i].alloc(P * 16); q_beta[i].alloc(P * 16); q_L[i].alloc(P * 16);
+      q_aux[i].alloc(P * 16); q_meta[i].alloc(P * 4);
+    }
+    hit.alloc(P * 16);
+    s_ent.alloc(P * 16);
+    cap_paths = paths;
   }
-  hit.alloc(P * 16);
-  s_ent.alloc(P * 16);
-  sh_o.alloc(P * emitters * 16); sh_d.alloc(P * emitters * 16); sh_c.alloc(P * emitters * 16);
-  cap_paths = paths;
-  cap_emitters = emitters;
+  // shadow slot (e, i) at e x paths + i; mask word w >= 1 of path i at
+  // (w - 1) x paths + i
+  size_t slots = size_t(paths) * emitters;
+  if (slots > cap_slots) {
+    sh_o.alloc(slots * 16); sh_d.alloc(slots * 16); sh_c.alloc(slots * 16);
+    cap_slots = slots;
+  }
+  size_t words = size_t(paths) * ((emitters + 31) / 32 - 1);
+  if (words > cap_words) {
+    sh_mask.alloc(words * 4);
+    cap_words = words;
+  }
+}
+
+uint32_t paths_for_emitters(uint32_t paths, uint32_t emitters) {
+  uint64_t cap = kShadowSlotBytes / (uint64_t(48) * std::max(1u, emitters));
+  return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(paths, cap)));
 }
 
 uint64_t Workspace::device_bytes() const {
   const DBuf *all[] = {&sample_L, &q_o[0], &q_o[1], &q_d[0], &q_d[1], &q_beta[0], &q_beta[1], &q_L[0], &q_L[1],
                        &q_aux[0], &q_aux[1], &q_meta[0], &q_meta[1], &hit, &s_ent, &sh_o, &sh_d, &sh_c,
-                       &counters, &stats, &reqs, &items, &plan, &item_done, &hdr};
+                       &sh_mask, &counters, &stats, &reqs, &items, &plan, &item_done, &hdr};
   uint64_t b = 0;
   for (const DBuf *x : all) b += x->bytes;
   return b;
@@ -74,12 +88,13 @@ uint64_t Workspace::device_bytes() const {
 
 void Workspace::release() {
   DBuf *all[] = {&sample_L, &q_o[0], &q_o[1], &q_d[0], &q_d[1], &q_beta[0], &q_beta[1], &q_L[0], &q_L[1],
-                 &q_aux[0], &q_aux[1], &q_meta[0], &q_meta[1], &hit, &s_ent, &sh_o, &sh_d, &sh_c, &counters,
+                 &q_aux[0], &q_aux[1], &q_meta[0], &q_meta[1], &hit, &s_ent, &sh_o, &sh_d, &sh_c, &sh_mask, &counters,
                  &stats, &reqs, &items, &plan, &item_done, &hdr};
   for (DBuf *b : all) b->release();
   PinnedBuf *pin[] = {&h_reqs, &h_items, &h_plan, &h_done, &h_counters, &h_hdr};
   for (PinnedBuf *b : pin) b->release();
-  cap_paths = cap_emitters = cap_samples = 0;
+  cap_paths = cap_samples = 0;
+  cap_slots = cap_words = 0;
   tables = false;
 }
 
@@ -93,7 +108,8 @@ static uint32_t kernel_grid(uint32_t count, uint32_t cap) {
 // ------------------------------------------------------------------ engine
 Engine::Engine(hk_scene *s, Workspace &ws, hipStream_t st, uint32_t pool_paths, uint32_t buffer_samples,
                uint32_t chunk_cap, bool profile, hipEvent_t ev0, hipEvent_t ev1)
-    : s_(s), ws_(ws), st_(st), pool_(std::max(1u, pool_paths)), cap_samples_(std::max(1u, buffer_samples)),
+    : s_(s), ws_(ws), st_(st), pool_(paths_for_emitters(std::max(1u, pool_paths), uint32_t(std::max(1, s->dev.n_emitters)))),
+      cap_samples_(std::max(1u, buffer_samples)),
       chunk_cap_(std::max(1u, chunk_cap)), profile_(profile), ev0_(ev0), ev1_(ev1) {
   ws_.ensure(pool_, uint32_t(s->dev.n_emitters), cap_samples_);
   shortest_first_ = s->ctx->schedule == 1;
@@ -149,6 +165,7 @@ RenderArgs Engine::args() const {
   A.sh_o = ws_.sh_o.as<float4>();
   A.sh_d = ws_.sh_d.as<float4>();
   A.sh_c = ws_.sh_c.as<float4>();
+  A.sh_mask = ws_.sh_mask.as<uint32_t>();
   A.counters = ws_.counters.as<uint32_t>();
   A.stats = ws_.stats.as<unsigned long long>();
   return A;

@@ -43,10 +43,12 @@ struct RenderArgs {
   uint32_t *q_meta[2]; // depth | flags << 16 | item slot << 20
   float4 *hit;         // closest hit of the current queue: t, u, v, prim bits
   // shadow queue: {source queue index, target (next-queue index, or sample
-  // buffer index), emitter mask, item slot | kShadowLast for a finished path};
-  // per (emitter, source index) rays
+  // buffer index), mask of emitters 0..31, item slot | kShadowLast for a
+  // finished path}; per (emitter, source index) rays at e * n_paths + i; the
+  // mask words of emitters 32 and up at (w - 1) * n_paths + i
   uint4 *s_ent;
   float4 *sh_o, *sh_d, *sh_c;
+  uint32_t *sh_mask;
   uint32_t *counters;            // at ctr(k): [0],[1] extension queues, [2] shadow queue, [4],[5] fetch
                                  // counters
   unsigned long long *stats;     // [0] extension rays, [1] shadow rays, [2..5] traversal counters

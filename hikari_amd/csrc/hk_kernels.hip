@@ -818,7 +818,9 @@ constexpr uint32_t kSortKeys = 32;  // material keys of the block sort (last two
 // records (at most kShadeMeshes) are copied to LDS once per block; the BSDF
 // code and the hit's emission read them there instead of from global memory.
 constexpr int kShadeMats = 16, kShadeMeshes = 32;
-template <bool LDS_TABLES>
+// MANY: more than 32 emitters (shadow masks beyond the entry's one word);
+// scenes with at most 32 use a build without that bookkeeping.
+template <bool LDS_TABLES, bool MANY>
 __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs A, int qi) {
   const DevScene &sc = A.sc;
   __shared__ DevMaterial s_mats[LDS_TABLES ? kShadeMats : 1];
@@ -870,7 +872,10 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
     uint32_t i = base + s_perm[threadIdx.x];
     bool active = i < count;
     bool cont = false;
+    // smask: emitters 0..31 with a shadow ray; sany (MANY): any emitter (the
+    // words of emitters 32 and up go to A.sh_mask as they complete)
     uint32_t smask = 0, pix = 0, local = 0, islot = 0, meta_out = 0;
+    bool sany = false;
     vec3 P = v3(0.0f), nd = v3(0.0f), L = v3(0.0f), beta = v3(0.0f);
     float pdf_b = 0.0f, w_mats = 0.0f;
     Rng rng;
@@ -914,8 +919,11 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
           // PathMATSIntegrator.cpp:42-50 and WhittedIntegrator.cpp:54-62
           // (always); PathEMSIntegrator.cpp:44-60 (camera rays, after a
           // non-diffuse surface, or the same emitter again)
+          // pathEMS keeps the last surface's emitter + 1 in the W_MATS slot
+          // (unused by that estimator); a fresh path holds 1.0f there, which
+          // no emitter index matches
           bool count = mode >= 2 || depth == 0 || (flags & kFlagLastSpecular) ||
-                       (flags >> 2) == (uint32_t)(mesh.emitter + 1);
+                       (mode == 1 && __float_as_uint(w_mats) == (uint32_t)(mesh.emitter + 1));
           if (dot(fr.n, rd) < 0.0f) le = ld3(mesh.radiance);
           if (count) L = L + beta * le;
         } else if (mesh.emitter >= 0) {  // :73-81 (+ the MATS weight of :122-140)
@@ -940,7 +948,13 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
         bool ems = mode == 0 || ((mode == 1 || mode == 3) && diffuse_mat);
         // One Eval / Pdf site serves both the emitter samples and whitted's
         // own-emitter term (each inlined BSDF copy costs registers).
+        uint32_t word = 0;  // mask bits of the current 32 emitters
         for (int e = 0; ems && e < sc.n_emitters; e++) {
+          if (MANY && e >= 32 && (e & 31) == 0) {  // a word of emitters 32.. is complete
+            if (e > 32) A.sh_mask[size_t((e >> 5) - 2) * A.n_paths + i] = word;
+            else smask = word;
+            word = 0;
+          }
           bool self = mode != 0 && e == mesh.emitter;
           if (self && mode != 3) continue;
           BRec br;
@@ -976,9 +990,12 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
             A.sh_o[slot] = make_float4(P.x, P.y, P.z, 0.0f);
             A.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, 0.0f);
             A.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.0f);
-            smask |= 1u << e;
+            word |= 1u << (e & 31);
+            if (MANY) sany = true;
           }
         }
+        if (!MANY) smask = word;
+        else if (ems) A.sh_mask[size_t(((sc.n_emitters - 1) >> 5) - 1) * A.n_paths + i] = word;
         // BSDF sampling (:111-120), MIS bookkeeping (:122-140), RR (:143-151).
         // whitted (WhittedIntegrator.cpp:100-113): a diffuse vertex ends the
         // path; any other continues with probability 0.95, drawn before the
@@ -1012,15 +1029,18 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
               depth++;
               cont = depth < (uint32_t)sc.max_depth;
             }
-            if (mode == 1)
-              flags = (diffuse_mat ? 0u : kFlagLastSpecular) | ((uint32_t)(mesh.emitter + 1) << 2);
+            if (mode == 1) {
+              flags = diffuse_mat ? 0u : kFlagLastSpecular;
+              w_mats = __uint_as_float((uint32_t)(mesh.emitter + 1));
+            }
           }
         }
       }
       meta_out = (depth & kDepthMask) | (flags << kFlagShift) | (islot << kSlotShift);
     }
+    if (!MANY) sany = smask != 0u;
     uint32_t slot, ss;
-    block_append2(s_app, &A.counters[ctr(qn)], cont, &A.counters[ctr(2)], smask != 0u, slot, ss);
+    block_append2(s_app, &A.counters[ctr(qn)], cont, &A.counters[ctr(2)], sany, slot, ss);
     if (cont) {  // the path moves to the compacted next queue
       A.q_o[qn][slot] = make_float4(P.x, P.y, P.z, 1e-4f);
       A.q_d[qn][slot] = make_float4(nd.x, nd.y, nd.z, kInfF);
@@ -1035,8 +1055,8 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
     bool fin = active && !cont;
     uint32_t sbuf = A.items[islot].buf_base + local;
     if (fin) A.sample_L[sbuf] = make_float4(L.x, L.y, L.z, 0.0f);
-    count_done(A.item_done, fin && smask == 0u, islot);
-    if (smask != 0u)
+    count_done(A.item_done, fin && !sany, islot);
+    if (sany)
       A.s_ent[ss] = cont ? make_uint4(i, slot, smask, 0u) : make_uint4(i, sbuf, smask, islot | kShadowLast);
   }
 }
@@ -1048,6 +1068,7 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
 // contributions to the path's radiance, so no atomics are needed and the
 // additions happen in the reference's order.  Lanes refill as in k_extend
 // (fetch counter counters[5], cleared by k_extend).
+template <bool MANY>
 __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   __shared__ uint64_t stack[kLdsStack * kBlock];
   // the top of the tree (10.6 KiB: at 5 waves per SIMD a block may use a
@@ -1067,6 +1088,11 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
   R.ref = kDone;
   int32_t pend = kDone;  // parked leaf
   uint32_t ent = kNoRay, mask = 0, src = 0;
+  // mask: the current word's emitters still to trace (word mword: emitters
+  // 32 * mword ..); scenes with more than 32 emitters keep the further words
+  // in A.sh_mask
+  uint32_t mword = 0;
+  const uint32_t last_word = MANY ? uint32_t(max(A.sc.n_emitters, 1) - 1) >> 5 : 0u;
   float4 *lp = nullptr;
   vec3 L = v3(0.0f), c = v3(0.0f);
   float lw = 0.0f;
@@ -1085,6 +1111,9 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
         uint4 e4 = A.s_ent[idx];
         src = e4.x;
         mask = e4.z;
+        mword = 0;
+        if (MANY)
+          while (mask == 0u && mword < last_word) mask = A.sh_mask[size_t(mword++) * A.n_paths + src];
         last = (e4.w & kShadowLast) != 0u;
         islot = e4.w & ~kShadowLast;
         lp = last ? &A.sample_L[e4.y] : &A.q_L[qn][e4.y];
@@ -1100,8 +1129,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow(RenderArgs A, int qn) {
     }
     if (start) {  // the entry's next emitter ray
       start = false;
-      int e = __ffs((int)mask) - 1;
+      int e = int(mword << 5) + __ffs((int)mask) - 1;
       mask &= mask - 1u;
+      if (MANY)
+        while (mask == 0u && mword < last_word) mask = A.sh_mask[size_t(mword++) * A.n_paths + src];
       size_t slot = size_t(e) * A.n_paths + src;
       float4 o = A.sh_o[slot], d = A.sh_d[slot], c4 = A.sh_c[slot];
       c = v3(c4.x, c4.y, c4.z);
@@ -1409,15 +1440,24 @@ hipError_t launch_extend(const RenderArgs &A, int qi, uint32_t grid, hipStream_t
   return hipGetLastError();
 }
 hipError_t launch_shade(const RenderArgs &A, int qi, uint32_t grid, hipStream_t st) {
-  if (A.sc.n_mats <= kShadeMats && A.sc.n_meshes <= kShadeMeshes)
-    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kBlock), 0, st, A, qi);
-  else
-    hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kBlock), 0, st, A, qi);
+  bool lds = A.sc.n_mats <= kShadeMats && A.sc.n_meshes <= kShadeMeshes, many = A.sc.n_emitters > 32;
+  if (many) {
+    if (lds) hipLaunchKernelGGL((k_shade<true, true>), dim3(grid), dim3(kBlock), 0, st, A, qi);
+    else hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(kBlock), 0, st, A, qi);
+  } else {
+    if (lds) hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(kBlock), 0, st, A, qi);
+    else hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(kBlock), 0, st, A, qi);
+  }
   return hipGetLastError();
 }
 hipError_t launch_shadow(const RenderArgs &A, int qn, uint32_t grid, hipStream_t st) {
-  static uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_shadow));
-  hipLaunchKernelGGL(k_shadow, dim3(std::min(grid, cap)), dim3(kBlock), 0, st, A, qn);
+  if (A.sc.n_emitters > 32) {
+    static uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_shadow<true>));
+    hipLaunchKernelGGL(k_shadow<true>, dim3(std::min(grid, cap)), dim3(kBlock), 0, st, A, qn);
+  } else {
+    static uint32_t cap = resident_blocks(reinterpret_cast<const void *>(k_shadow<false>));
+    hipLaunchKernelGGL(k_shadow<false>, dim3(std::min(grid, cap)), dim3(kBlock), 0, st, A, qn);
+  }
   return hipGetLastError();
 }
 hipError_t launch_resolve(const RenderArgs &A, uint32_t slot, const DevRequest &rq, uint32_t npix, bool tiles,

@@ -49,6 +49,10 @@ constexpr uint32_t kDefaultMaxSamples = 1u << 28;   // samples per segment (16 B
 // persistent traversal waves: idle lanes that trigger a refill, and queue
 // entries (in 64s) reserved per atomic on the fetch counter
 constexpr uint32_t kDefaultRefill = 40, kDefaultRefillShadow = 16, kDefaultFetchBatch = 4;
+// HBM for the per-emitter shadow slots (48 B per path and emitter): scenes
+// with many emitters run narrower wavefronts (paths <= this / (48 x emitters))
+constexpr uint64_t kShadowSlotBytes = uint64_t(24) << 30;
+uint32_t paths_for_emitters(uint32_t paths, uint32_t emitters);
 
 // Owned pinned host allocation (staging for small per-bounce copies).
 struct PinnedBuf {
@@ -64,12 +68,17 @@ struct PinnedBuf {
 };
 
 // Wavefront buffers of one engine, grown on demand and reused across renders.
+// The per-emitter shadow slots (and the emitter mask words beyond the first
+// 32 emitters) are sized for paths x emitters of the current engine, apart
+// from the queues, so a scene with many emitters and a narrow wavefront does
+// not multiply with another scene's wide one.
 struct Workspace {
-  uint32_t cap_paths = 0, cap_emitters = 0, cap_samples = 0;
+  uint32_t cap_paths = 0, cap_samples = 0;
+  size_t cap_slots = 0, cap_words = 0;
   bool tables = false;
   DBuf sample_L;
   DBuf q_o[2], q_d[2], q_beta[2], q_L[2], q_aux[2], q_meta[2], hit;
-  DBuf s_ent, sh_o, sh_d, sh_c;
+  DBuf s_ent, sh_o, sh_d, sh_c, sh_mask;
   DBuf counters, stats;
   DBuf reqs, items, plan, item_done, hdr;          // request / item tables, regen plan, per-item counts,
                                                    // bounce header

@@ -84,7 +84,7 @@ static_assert(sizeof(TraceTri) == 48, "TraceTri layout");
 // where the mesh has none), its mesh, material and attribute flags.  A hit
 // reads this one line instead of the face's index triple followed by nine
 // scattered vertex attributes.
-constexpr int kMaxEmitters = 32;  // emitters of one scene (bits of a shadow entry's mask)
+constexpr int kMaxEmitters = 1 << 16;  // emitters of one scene (the wavefront narrows as they grow)
 
 constexpr uint32_t kShadeHasN = 1u, kShadeHasUV = 2u;
 struct alignas(128) ShadeTri {

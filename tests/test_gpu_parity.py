@@ -550,6 +550,43 @@ def test_many_materials_hit_vs_oracle(ctx):
     assert abs(lum(img).mean() - lum(ref).mean()) / lum(ref).mean() < 0.01
 
 
+def _many_emitter_scene(integrator):
+    """A diffuse floor under 40 small emissive spheres and a sky map: 41
+    emitters (the sky is the last), so shadow masks span two words."""
+    sc = D.Scene(width=40, height=32, fov=50.0, max_depth=1000, integrator=integrator)
+    floor = sc.add_material(D.diffuse((0.6, 0.55, 0.5)))
+    black = sc.add_material(D.diffuse(0.0))
+    sc.add_mesh(np.array([[-3, 0, -3], [3, 0, -3], [3, 0, 3], [-3, 0, 3]], np.float32),
+                np.array([[0, 2, 1], [0, 3, 2]], np.uint32), floor)
+    n = 0
+    for i in range(8):
+        for j in range(5):
+            cen = (-1.4 + 0.4 * i, 0.35 + 0.15 * ((i + j) % 3), -1.0 + 0.5 * j)
+            v, f = scenes.icosphere(1, 0.08, cen)
+            rad = (2.0 + 0.3 * (n % 7), 1.0 + 0.2 * (n % 5), 0.5 + 0.4 * (n % 3))
+            sc.add_mesh(v, scenes._ensure_outward(v, f, cen), black, radiance=rad)
+            n += 1
+    sc.env = scenes.env_map(seed=3, w=64, h=32) * 0.2
+    sc.cam_to_world = D.lookat((0.0, 2.2, 2.6), (0.0, 0.0, -0.3), (0, 1, 0))
+    return sc
+
+
+@pytest.mark.parametrize("integrator", ["pathMIS", "pathEMS", "pathMATS", "whitted"])
+def test_many_emitters_vs_oracle(ctx, integrator):
+    """More than 32 emitters (the shadow entry's mask word covers 32; the
+    rest lives in per-path mask words): GPU image vs the FP32 oracle at
+    equal streams, for every estimator (pathEMS also keeps the last hit's
+    emitter index in the path state)."""
+    sc = _many_emitter_scene(integrator)
+    img, st = ctx.scene(sc).render(16, seed=31)
+    ref, _, _ = O.render(sc, req_seed=31, spp=(0, 16))
+    rm = rel_mse(img, ref)
+    print(integrator, "41 emitters: relMSE vs oracle", rm, "shadow rays", st["shadow_rays"])
+    assert (st["shadow_rays"] > 0) == (integrator != "pathMATS") and lum(ref).mean() > 0.01
+    assert rm < 1e-3
+    assert abs(lum(img).mean() - lum(ref).mean()) / lum(ref).mean() < 0.01
+
+
 @pytest.mark.parametrize("filt,radius", [(D.FILTER_GAUSSIAN, 2.0), (D.FILTER_BOX, 0.5), (D.FILTER_MITCHELL, 2.0),
                                           (D.FILTER_TENT, 1.0), (D.FILTER_GAUSSIAN, 3.2), (D.FILTER_GAUSSIAN, 0.4)])
 def test_tiled_resolve_equals_plain(ctx, filt, radius):
