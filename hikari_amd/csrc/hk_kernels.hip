@@ -609,11 +609,12 @@ __device__ __forceinline__ void post_intersect(const DevScene &sc, int32_t prim,
   }
 }
 
-// q_meta: depth (12 bits) | flags (8 bits) << 12 | item slot (12 bits) << 20.
+// q_meta: depth (18 bits) | flags (2 bits) << 18 | item slot (12 bits) << 20.
 // Flags: bit 0 the last BSDF sample was discrete (pathMIS); bit 1 the last
-// surface was not diffuse, bits 2..7 its emitter + 1 (pathEMS).
-constexpr uint32_t kDepthMask = 0xfffu;
-constexpr uint32_t kFlagShift = 12;
+// surface was not diffuse (pathEMS, which keeps that surface's emitter + 1 in
+// the W_MATS slot of q_L).
+constexpr uint32_t kDepthMask = 0x3ffffu;
+constexpr uint32_t kFlagShift = 18;
 constexpr uint32_t kFlagPrevDiscrete = 1u;
 constexpr uint32_t kFlagLastSpecular = 2u;
 constexpr uint32_t kShadowLast = 0x80000000u;  // shadow entry flag: the path ended at this bounce
@@ -889,7 +890,7 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
       islot = meta >> kSlotShift;
       rng.state = (uint64_t)aux.x | ((uint64_t)aux.y << 32);
       rng.inc = rng_inc(pix);
-      uint32_t depth = meta & kDepthMask, flags = (meta >> kFlagShift) & 0xffu;
+      uint32_t depth = meta & kDepthMask, flags = (meta >> kFlagShift) & 3u;
       const int mode = sc.integrator;  // 0 pathMIS, 1 pathEMS, 2 pathMATS, 3 whitted
       beta = v3(b4.x, b4.y, b4.z);
       L = v3(l4.x, l4.y, l4.z);

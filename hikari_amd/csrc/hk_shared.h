@@ -85,6 +85,7 @@ static_assert(sizeof(TraceTri) == 48, "TraceTri layout");
 // reads this one line instead of the face's index triple followed by nine
 // scattered vertex attributes.
 constexpr int kMaxEmitters = 1 << 16;  // emitters of one scene (the wavefront narrows as they grow)
+constexpr uint32_t kMaxPathDepth = (1u << 18) - 1;  // vertices per path (18 bits of the path state)
 
 constexpr uint32_t kShadeHasN = 1u, kShadeHasUV = 2u;
 struct alignas(128) ShadeTri {

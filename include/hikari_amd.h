@@ -163,7 +163,7 @@ typedef struct hk_scene_desc {
  * only, PathEMSIntegrator.cpp:16-119), pathMATS (BSDF sampling only,
  * PathMATSIntegrator.cpp:16-87) and whitted (WhittedIntegrator.cpp:26-116).
  * max_depth: negative means unlimited (the reference reads it as uint32);
- * clamped to 4095.
+ * clamped to 262143.
  *
  * Replaces Scene::Activate (src/core/Scene.cpp:128-165) plus the Activate()
  * of every mesh / BSDF / emitter / camera: uploads geometry, BVH, material
