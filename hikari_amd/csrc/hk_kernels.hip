@@ -693,8 +693,9 @@ __global__ __launch_bounds__(kBlock) void k_regen(RenderArgs A, int qi) {
 // A.refill_extend of its lanes have finished their rays, gives them new queue
 // entries from its reserved range (wave_fetch: A.fetch_batch x 64 entries per
 // atomic on counters[4], cleared by k_regen / k_shade before), so a few long
-// rays no longer hold a whole wave.  Both are run-time options; 24 idle lanes
-// and 2 x 64 entries are fastest on the benchmark scenes.
+// rays no longer hold a whole wave.  Both are run-time options; 40 idle lanes
+// and 4 x 64 entries are fastest on the benchmark scenes (k_shadow: 16 idle
+// lanes).
 constexpr uint32_t kNoRay = 0xffffffffu;
 
 // k_extend is held at 7 waves per SIMD (at most 72 VGPRs: it fits without
