@@ -550,22 +550,22 @@ def test_many_materials_hit_vs_oracle(ctx):
     assert abs(lum(img).mean() - lum(ref).mean()) / lum(ref).mean() < 0.01
 
 
-def _many_emitter_scene(integrator):
-    """A diffuse floor under 40 small emissive spheres and a sky map: 41
-    emitters (the sky is the last), so shadow masks span two words."""
-    sc = D.Scene(width=40, height=32, fov=50.0, max_depth=1000, integrator=integrator)
+def _many_emitter_scene(integrator, n_lights=40, width=40, height=32):
+    """A diffuse floor under n_lights small emissive spheres and a sky map:
+    n_lights + 1 emitters (the sky is the last); with 40 lights the shadow
+    masks span two words."""
+    sc = D.Scene(width=width, height=height, fov=50.0, max_depth=1000, integrator=integrator)
     floor = sc.add_material(D.diffuse((0.6, 0.55, 0.5)))
     black = sc.add_material(D.diffuse(0.0))
     sc.add_mesh(np.array([[-3, 0, -3], [3, 0, -3], [3, 0, 3], [-3, 0, 3]], np.float32),
                 np.array([[0, 2, 1], [0, 3, 2]], np.uint32), floor)
-    n = 0
-    for i in range(8):
-        for j in range(5):
-            cen = (-1.4 + 0.4 * i, 0.35 + 0.15 * ((i + j) % 3), -1.0 + 0.5 * j)
-            v, f = scenes.icosphere(1, 0.08, cen)
-            rad = (2.0 + 0.3 * (n % 7), 1.0 + 0.2 * (n % 5), 0.5 + 0.4 * (n % 3))
-            sc.add_mesh(v, scenes._ensure_outward(v, f, cen), black, radiance=rad)
-            n += 1
+    rows = (n_lights + 7) // 8
+    for n in range(n_lights):
+        i, j = n % 8, n // 8
+        cen = (-1.4 + 0.4 * i, 0.35 + 0.15 * ((i + j) % 3), -1.0 + 2.0 * j / max(1, rows - 1) if rows > 1 else -0.5)
+        v, f = scenes.icosphere(1, 0.08, cen)
+        rad = (2.0 + 0.3 * (n % 7), 1.0 + 0.2 * (n % 5), 0.5 + 0.4 * (n % 3))
+        sc.add_mesh(v, scenes._ensure_outward(v, f, cen), black, radiance=rad)
     sc.env = scenes.env_map(seed=3, w=64, h=32) * 0.2
     sc.cam_to_world = D.lookat((0.0, 2.2, 2.6), (0.0, 0.0, -0.3), (0, 1, 0))
     return sc
@@ -585,6 +585,19 @@ def test_many_emitters_vs_oracle(ctx, integrator):
     assert (st["shadow_rays"] > 0) == (integrator != "pathMATS") and lum(ref).mean() > 0.01
     assert rm < 1e-3
     assert abs(lum(img).mean() - lum(ref).mean()) / lum(ref).mean() < 0.01
+
+
+@pytest.mark.parametrize("n_emitters", [32, 33, 64, 65])
+def test_emitter_mask_word_boundaries(ctx, n_emitters):
+    """Emitter counts at the 32-emitter mask-word boundaries (one word, a
+    second word of one emitter, two full words, a third word of one): GPU
+    image vs the FP32 oracle at equal streams."""
+    sc = _many_emitter_scene("pathMIS", n_lights=n_emitters - 1, width=20, height=16)
+    img, st = ctx.scene(sc).render(8, seed=33)
+    ref, _, _ = O.render(sc, req_seed=33, spp=(0, 8))
+    rm = rel_mse(img, ref)
+    print(n_emitters, "emitters: relMSE vs oracle", rm)
+    assert st["shadow_rays"] > 0 and rm < 1e-3
 
 
 @pytest.mark.parametrize("filt,radius", [(D.FILTER_GAUSSIAN, 2.0), (D.FILTER_BOX, 0.5), (D.FILTER_MITCHELL, 2.0),
