@@ -61,13 +61,15 @@ double inc_beta(double a, double b, double x) {
 
 bool StudentsTTest(double mean, double variance, double reference, long n, double significance, int n_tests,
                    std::string &msg) {
+  // The statistic floors the sample variance at 1e-5 (ext/hypothesis/hypothesis.h:324),
+  // so a deterministic estimator (zero variance) whose mean matches the reference up to
+  // float rounding is accepted rather than given an infinite t.
   double nu = double(n - 1);
-  double se = std::sqrt(variance / double(n));
-  double t = se > 0 ? (mean - reference) / se : (mean == reference ? 0.0 : INFINITY);
+  double t = std::fabs(mean - reference) * std::sqrt(double(n) / std::max(variance, 1e-5));
   // two-sided p-value of Student's t with nu degrees of freedom
   double p = std::isinf(t) ? 0.0 : inc_beta(0.5 * nu, 0.5, nu / (nu + t * t));
   double alpha = 1.0 - std::pow(1.0 - significance, 1.0 / double(std::max(1, n_tests)));
-  bool ok = p > alpha;
+  bool ok = !(p < alpha);  // rejected only when p < alpha (hypothesis.h:342)
   char buf[256];
   snprintf(buf, sizeof buf, "%s: sample mean %.6g vs reference %.6g (t = %.3f, p-value = %.4g, level %.4g)",
            ok ? "Accepted the null hypothesis" : "Reject the null hypothesis", mean, reference, t, p, alpha);

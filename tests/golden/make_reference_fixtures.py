@@ -9,7 +9,14 @@ tests/golden/envmap from the reference's scene files:
   * envmap/: scene/pa6 envmap0 and envmap3 (whitted, a diffuse sphere and
     plane under an environment map): the XML, OBJ and environment EXR files
     copied as data, and envmap<n>-blocks16.npy, the 16 x 16-pixel block means
-    (32 x 32 blocks) of the reference's renders envmap<n>.exr (256 spp).
+    (32 x 32 blocks) of the reference's renders envmap<n>.exr (256 spp);
+  * pa4tests/: scene/pa4/tests test-mesh.xml and test-mesh-furnace.xml
+    (whitted t-tests with their reference values) and their meshes/;
+  * cbox4/ and motto/: scene/pa4 cbox-distributed and motto-diffuse /
+    motto-dielectric (whitted over area lights, diffuse and dielectric):
+    XML and meshes/, and <name>-blocks<b>.npy, the b x b-pixel block means of
+    the reference's renders <name>.exr (512 spp; b = 20 for the 800 x 600
+    cbox, 16 for the 768 x 384 motto).
 
 Images are decoded with this project's EXR reader (hikari_render --convert).
 Run once in a tree where the reference is present:
@@ -69,6 +76,19 @@ def main(ref_root):
         np.save(os.path.join(out, f"envmap{n}-blocks16.npy"), blk)
         print("envmap", n, blk.shape, blk.mean(axis=(0, 1)))
     copy(src, out, ["sphere.obj", "plane.obj"])
+    src = os.path.join(ref_root, "scene", "pa4")
+    copy(os.path.join(src, "tests"), os.path.join(HERE, "pa4tests"), ["test-mesh.xml", "test-mesh-furnace.xml"])
+    for sub, out_name, names, b in (("cbox", "cbox4", ["cbox-distributed"], 20),
+                                    ("motto", "motto", ["motto-diffuse", "motto-dielectric"], 16)):
+        out = os.path.join(HERE, out_name)
+        copy(os.path.join(src, sub), out, [n + ".xml" for n in names])
+        for n in names:
+            blk = block_means(os.path.join(src, sub, n + ".exr"), b)
+            np.save(os.path.join(out, f"{n}-blocks{b}.npy"), blk)
+            print(n, blk.shape, blk.mean(axis=(0, 1)))
+    for sub, out_name in (("tests", "pa4tests"), ("cbox", "cbox4"), ("motto", "motto")):
+        meshes = os.path.join(src, sub, "meshes")
+        copy(meshes, os.path.join(HERE, out_name, "meshes"), sorted(n for n in os.listdir(meshes) if n.endswith(".obj")))
 
 
 if __name__ == "__main__":

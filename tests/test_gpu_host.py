@@ -29,11 +29,13 @@ def run(*args):
 
 
 @pytest.mark.parametrize("name,total", [("test-direct.xml", 15), ("test-furnace.xml", 6),
-                                        ("ttest-microfacet.xml", 5)])
+                                        ("ttest-microfacet.xml", 5), ("pa4tests/test-mesh.xml", 5),
+                                        ("pa4tests/test-mesh-furnace.xml", 2)])
 def test_reference_ttest_files(name, total):
-    """scene/pa5/tests/*.xml through the C++ harness (src/core/StudentTTest.cpp
-    semantics): every scene (pathMIS, pathEMS, pathMATS) and every incidence
-    angle of the microfacet BSDF test passes."""
+    """scene/pa5/tests/*.xml and scene/pa4/tests/*.xml through the C++ harness
+    (src/core/StudentTTest.cpp semantics): every scene (pathMIS, pathEMS,
+    pathMATS, whitted) and every incidence angle of the microfacet BSDF test
+    passes."""
     r = run(os.path.join(GOLDEN, name))
     assert r.returncode == 0 and f"Passed {total}/{total} tests." in r.stdout
 
@@ -139,4 +141,23 @@ def test_envmap_matches_reference_render(tmp_path, n):
     ratio, med, p90, mx = _block_check(img, np.load(os.path.join(GOLDEN, "envmap", f"envmap{n}-blocks16.npy")), 16)
     print(n, "mean ratio", ratio, "block rel median", med, "p90", p90, "max", mx)
     # measured on the MI355X: mean within 4e-5, median < 0.02%, p90 < 0.4%
+    assert abs(ratio - 1) < 0.003 and med < 0.005 and p90 < 0.012
+
+
+@pytest.mark.parametrize("sub,name,b", [("cbox4", "cbox-distributed", 20), ("motto", "motto-diffuse", 16),
+                                        ("motto", "motto-dielectric", 16)])
+def test_whitted_scenes_match_reference_render(tmp_path, sub, name, b):
+    """scene/pa4 cbox-distributed (diffuse box, area light) and motto-diffuse /
+    motto-dielectric (diffuse and dielectric lettering over a floor, two area
+    lights), whitted at the reference's 512 spp, against b x b block means of
+    the reference's own renders (fixtures from
+    tests/golden/make_reference_fixtures.py)."""
+    out = tmp_path / f"{name}.pfm"
+    r = run(os.path.join(GOLDEN, sub, f"{name}.xml"), "-o", str(out))
+    assert r.returncode == 0
+    img = read_pfm(out)
+    ref = np.load(os.path.join(GOLDEN, sub, f"{name}-blocks{b}.npy"))
+    assert img.shape == (ref.shape[0] * b, ref.shape[1] * b, 3)
+    ratio, med, p90, mx = _block_check(img, ref, b)
+    print(name, "mean ratio", ratio, "block rel median", med, "p90", p90, "max", mx)
     assert abs(ratio - 1) < 0.003 and med < 0.005 and p90 < 0.012
