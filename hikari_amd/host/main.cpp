@@ -5,13 +5,17 @@
 //
 //   hikari_render FILE.xml [-o OUT.exr|OUT.pfm] [--spp N] [--dump] [--dump-mesh PREFIX]
 //   hikari_render --convert IN.exr|IN.pfm OUT.exr|OUT.pfm
+//   hikari_render --t-test MEAN VARIANCE REFERENCE N LEVEL N_TESTS
 //
 // --dump prints the parsed description as JSON and touches no device;
 // --dump-mesh also writes every mesh's world-space vertex positions and
 // normals as raw FP32 (PREFIX.<mesh>.v.f32, PREFIX.<mesh>.n.f32);
-// --convert rewrites an image (EXR written as uncompressed FLOAT RGB).
+// --convert rewrites an image (EXR written as uncompressed FLOAT RGB);
+// --t-test runs the Student t-test of the test harness on given statistics
+// and prints its verdict (exit status 0 accepted, 1 rejected).
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -95,6 +99,13 @@ int main(int argc, char **argv) {
       fprintf(stderr, "error: %s\n", e.what());
       return 1;
     }
+  }
+  if (argc == 8 && std::strcmp(argv[1], "--t-test") == 0) {
+    std::string msg;
+    bool ok = StudentsTTest(std::atof(argv[2]), std::atof(argv[3]), std::atof(argv[4]), std::atol(argv[5]),
+                            std::atof(argv[6]), std::atoi(argv[7]), msg);
+    printf("%s\n", msg.c_str());
+    return ok ? 0 : 1;
   }
   std::string file, out;
   int spp = -1;

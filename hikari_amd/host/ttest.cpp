@@ -71,7 +71,7 @@ bool StudentsTTest(double mean, double variance, double reference, long n, doubl
   double alpha = 1.0 - std::pow(1.0 - significance, 1.0 / double(std::max(1, n_tests)));
   bool ok = !(p < alpha);  // rejected only when p < alpha (hypothesis.h:342)
   char buf[256];
-  snprintf(buf, sizeof buf, "%s: sample mean %.6g vs reference %.6g (t = %.3f, p-value = %.4g, level %.4g)",
+  snprintf(buf, sizeof buf, "%s: sample mean %.6g vs reference %.6g (t = %.6g, p-value = %.6g, level %.6g)",
            ok ? "Accepted the null hypothesis" : "Reject the null hypothesis", mean, reference, t, p, alpha);
   msg = buf;
   return ok;

@@ -180,3 +180,33 @@ def test_exr_decodes_reference_renders(tmp_path):
         b = lambda a: a.reshape(18, 24, 32, 24).mean(axis=(1, 3))  # noqa: E731
         rel = np.abs(b(lum[k]) - b(lum["mis"])) / (b(lum["mis"]) + 0.05)
         assert np.median(rel) < 0.03, (k, float(np.median(rel)))
+
+
+T_CASES = [  # mean, variance, reference, n, level, tests
+    (1.0000000119, 0.0, 0.99999998, 10000, 0.01, 2),  # a deterministic estimator: variance floored
+    (1.05, 0.5, 1.0, 1000, 0.01, 1),
+    (1.1, 0.5, 1.0, 1000, 0.01, 1),
+    (0.5, 2e-6, 0.4999, 100000, 0.01, 5),               # below the floor: t uses 1e-5
+    (3.0, 4.0, 3.2, 50, 0.05, 15),
+    (0.2, 1e-3, 0.2, 100000, 0.01, 6),
+    (0.7, 0.09, 0.69, 100000, 0.01, 6),
+]
+
+
+@pytest.mark.parametrize("case", T_CASES)
+def test_student_t_matches_scipy(case):
+    """The harness's t-test (hikari_render --t-test) against scipy's Student t:
+    t = |mean - reference| sqrt(n / max(variance, 1e-5)) on n - 1 degrees of
+    freedom, two-sided p-value, Sidak level 1 - (1 - level)^(1 / tests),
+    rejected iff p < level (ext/hypothesis/hypothesis.h:319-351)."""
+    from scipy import stats
+    mean, var, ref, n, level, tests = case
+    r = run("--t-test", *(repr(x) for x in case), check=False)
+    t = abs(mean - ref) * np.sqrt(n / max(var, 1e-5))
+    p = 2 * stats.t.sf(t, n - 1)
+    alpha = 1 - (1 - level) ** (1 / tests)
+    got = dict(kv.split(" = ") for kv in r.stdout.split("(")[1].rstrip(")\n").replace("level ", "level = ").split(", "))
+    assert float(got["t"]) == pytest.approx(t, rel=1e-5)
+    assert float(got["p-value"]) == pytest.approx(p, rel=1e-4, abs=1e-12)
+    assert float(got["level"]) == pytest.approx(alpha, rel=1e-5)
+    assert r.returncode == (1 if p < alpha else 0)
