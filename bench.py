@@ -20,6 +20,9 @@ Default run (one JSON line):
     first-served queue of plain hk_render calls on the same trace: p50/p99
     latency and path samples/s of each, at --qps (10: the GPU ~70% busy) and
     for the same requests at three times that rate (serve_saturated);
+  * c0: the 256 x 256 box at 64 spp (the configuration the reference
+    renderer was timed on, 3.35e5 path samples/s on 8 host threads) as plain
+    renders, beside that rate;
   * cpu_baseline: the oracle (the C++ restatement) on the host cores on a
     bounded sample of the displaced-sphere workload, with its measured ratio
     to the reference renderer on the box scene.
@@ -110,6 +113,26 @@ def cpu_baseline(scene, target_s):
                       f"({paths} paths, {secs:.1f} s render time, BVH build excluded)",
             "anchor_c0": ANCHOR_C0,
             "reference_estimate": value / ANCHOR_C0["oracle_over_reference"]}
+
+
+def c0_leg(gs_box, spp=64, renders=5):
+    """The box scene (256 x 256, 64 spp: the configuration the reference
+    renderer was timed on, ANCHOR_C0) as plain renders on the GPU, beside
+    the reference's measured CPU rate."""
+    gs_box.render(spp, seed=900)
+    paths = rays = 0
+    t0 = time.perf_counter()
+    for i in range(renders):
+        _, st = gs_box.render(spp, seed=i)
+        paths += st["paths"]
+        rays += st["extension_rays"] + st["shadow_rays"]
+    wall = time.perf_counter() - t0
+    value = paths / wall
+    return {"workload": f"256x256 box, {spp} spp, plain gpuPathMIS (one render per step)", "value": value,
+            "unit": "path samples/s", "ms_per_render": wall / renders * 1e3, "rays_per_path": rays / max(1, paths),
+            "reference_cpu": {"value": ANCHOR_C0["reference_path_samples_per_s"], "threads": 8,
+                              "rays_per_path": ANCHOR_C0["reference_rays_per_path"]},
+            "gpu_over_reference_cpu": value / ANCHOR_C0["reference_path_samples_per_s"]}
 
 
 def _pct(xs, q):
@@ -530,6 +553,8 @@ def main():
         out["serve_saturated"] = serve_leg(ctx, gss, trace, a.budget_gb, service=out["serve"]["service_s"])
         for k in ("serve", "serve_saturated"):
             out[k].pop("service_s")
+        out["c0"] = c0_leg(gss[0])
+        log(f"c0: {out['c0']['value']:.3e} path samples/s, {out['c0']['ms_per_render']:.1f} ms per render")
     if not a.no_cpu_baseline:
         t = time.time()
         out["cpu_baseline"] = cpu_baseline(scene, a.cpu_seconds)
