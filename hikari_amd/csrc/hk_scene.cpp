@@ -260,7 +260,7 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
   // The depth property is a uint32 in the reference (PathMISIntegrator.cpp:13,
   // WhittedIntegrator.cpp:13): negative means unlimited.  The depth lives in
   // 18 bits of the path state, capping paths at 262143 vertices; RR ends a
-  // path long before (0.95^4095 ~ 1e-91).
+  // path long before (0.95^262143 ~ 1e-5840).
   ds.max_depth = d->max_depth < 0 ? int32_t(kMaxPathDepth) : std::min(d->max_depth, int32_t(kMaxPathDepth));
   if (d->integrator < 0 || d->integrator > 3) throw HkError(HK_ERR_INVALID, "unknown integrator %d", d->integrator);
   ds.integrator = d->integrator;
