@@ -18,14 +18,15 @@ LIB_PATH = os.environ.get("HIKARI_AMD_LIB", os.path.join(HERE, "libhikari_amd.so
 
 HK_OK = 0
 ERRORS = {-1: "HK_ERR_INVALID", -2: "HK_ERR_HIP", -3: "HK_ERR_OOM", -4: "HK_ERR_UNSUPPORTED", -5: "HK_ERR_IO",
-          -6: "HK_ERR_BUDGET"}
+          -6: "HK_ERR_BUDGET", -7: "HK_ERR_CANCELLED"}
+HK_ERR_CANCELLED = -7
 
 EXPORTS = ["hk_ctx_create", "hk_ctx_destroy", "hk_last_error", "hk_version", "hk_ctx_sync", "hk_accel_build",
            "hk_accel_destroy", "hk_accel_get_info", "hk_accel_intersect", "hk_accel_intersect_device",
            "hk_accel_intersect_device_noalloc",
            "hk_bsdf_batch", "hk_scene_create", "hk_scene_destroy", "hk_scene_device_bytes", "hk_render",
            "hk_render_pixel_samples", "hk_render_batch", "hk_set_option", "hk_server_create", "hk_server_add_scene",
-           "hk_server_submit", "hk_server_wait", "hk_server_get_stats", "hk_server_destroy", "hk_autotune"]
+           "hk_server_submit", "hk_server_wait", "hk_server_cancel", "hk_server_get_stats", "hk_server_destroy", "hk_autotune"]
 
 
 class HkError(RuntimeError):
@@ -55,7 +56,7 @@ class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("bounces_launched", C.c_uint64), ("gpu_ms", C.c_double), ("trace_ms", C.c_double),
                 ("trace_launches", C.c_uint64), ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64),
-                ("graph_bounces", C.c_uint64)]
+                ("graph_bounces", C.c_uint64), ("invalid_samples", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -63,13 +64,13 @@ class RenderStats(C.Structure):
 
 class ServerResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("queue_ms", C.c_double), ("latency_ms", C.c_double), ("paths", C.c_uint64),
-                ("submit_ms", C.c_double), ("lane", C.c_int32)]
+                ("submit_ms", C.c_double), ("lane", C.c_int32), ("invalid_samples", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("bounces", C.c_uint64), ("paths", C.c_uint64),
                 ("fixed_bytes", C.c_uint64), ("film_bytes", C.c_uint64), ("shared_bounces", C.c_uint64),
-                ("max_in_flight", C.c_uint64)]
+                ("max_in_flight", C.c_uint64), ("cancelled", C.c_uint64), ("requests_in_flight", C.c_uint64)]
 
 
 _lib = None
@@ -110,6 +111,7 @@ def lib():
         L.hk_server_add_scene.argtypes = [P, P, C.POINTER(C.c_int32)]
         L.hk_server_submit.argtypes = [P, C.c_int32, C.POINTER(Request), P, C.POINTER(C.c_uint64)]
         L.hk_server_wait.argtypes = [P, C.c_uint64, C.POINTER(ServerResult)]
+        L.hk_server_cancel.argtypes = [P, C.c_uint64]
         L.hk_server_get_stats.argtypes = [P, C.POINTER(ServerStats)]
         L.hk_server_destroy.argtypes = [P]
         L.hk_autotune.argtypes = [P, C.POINTER(Request), C.c_uint64, C.POINTER(TuneResult)]
@@ -345,6 +347,11 @@ class Server:
         img = self._pending.pop(ticket, None)
         self.ctx.check(rc)
         return img, {k: getattr(res, k) for k, _ in ServerResult._fields_}
+
+    def cancel(self, ticket):
+        """hk_server_cancel: the ticket's wait then raises HK_ERR_CANCELLED
+        (unless the request had already finished)."""
+        self.ctx.check(lib().hk_server_cancel(self.h, ticket))
 
     def stats(self):
         st = ServerStats()

@@ -36,7 +36,9 @@ void Workspace::ensure(uint32_t paths, uint32_t emitters, uint32_t samples) {
     hdr.alloc(sizeof(BounceHdr));
     counters.alloc(sizeof(uint32_t) * kCounterWords);
     stats.alloc(64);
+    req_invalid.alloc(sizeof(unsigned long long) * kMaxRequests);
     h_reqs.alloc(sizeof(DevRequest) * kMaxRequests);
+    h_req_invalid.alloc(sizeof(unsigned long long) * kMaxRequests);
     h_items.alloc(sizeof(WorkItem) * kMaxItems);
     h_plan.alloc(sizeof(RegenEntry) * kMaxItems);
     h_done.alloc(sizeof(uint32_t) * kMaxItems * kDoneWays);
@@ -80,7 +82,7 @@ uint32_t paths_for_emitters(uint32_t paths, uint32_t emitters) {
 uint64_t Workspace::device_bytes() const {
   const DBuf *all[] = {&sample_L, &q_o[0], &q_o[1], &q_d[0], &q_d[1], &q_beta[0], &q_beta[1], &q_L[0], &q_L[1],
                        &q_aux[0], &q_aux[1], &q_meta[0], &q_meta[1], &hit, &s_ent, &sh_o, &sh_d, &sh_c,
-                       &sh_mask, &counters, &stats, &reqs, &items, &plan, &item_done, &hdr};
+                       &sh_mask, &counters, &stats, &reqs, &items, &plan, &item_done, &hdr, &req_invalid};
   uint64_t b = 0;
   for (const DBuf *x : all) b += x->bytes;
   return b;
@@ -89,9 +91,9 @@ uint64_t Workspace::device_bytes() const {
 void Workspace::release() {
   DBuf *all[] = {&sample_L, &q_o[0], &q_o[1], &q_d[0], &q_d[1], &q_beta[0], &q_beta[1], &q_L[0], &q_L[1],
                  &q_aux[0], &q_aux[1], &q_meta[0], &q_meta[1], &hit, &s_ent, &sh_o, &sh_d, &sh_c, &sh_mask, &counters,
-                 &stats, &reqs, &items, &plan, &item_done, &hdr};
+                 &stats, &reqs, &items, &plan, &item_done, &hdr, &req_invalid};
   for (DBuf *b : all) b->release();
-  PinnedBuf *pin[] = {&h_reqs, &h_items, &h_plan, &h_done, &h_counters, &h_hdr};
+  PinnedBuf *pin[] = {&h_reqs, &h_items, &h_plan, &h_done, &h_counters, &h_hdr, &h_req_invalid};
   for (PinnedBuf *b : pin) b->release();
   cap_paths = cap_samples = 0;
   cap_slots = cap_words = 0;
@@ -168,6 +170,7 @@ RenderArgs Engine::args() const {
   A.sh_mask = ws_.sh_mask.as<uint32_t>();
   A.counters = ws_.counters.as<uint32_t>();
   A.stats = ws_.stats.as<unsigned long long>();
+  A.req_invalid = ws_.req_invalid.as<unsigned long long>();
   return A;
 }
 
@@ -215,6 +218,9 @@ void Engine::submit(RequestRec *r) {
   free_reqs_.pop_back();
   reqs_by_slot_[r->slot] = r;
   if (!r->keep_samples) hip_check(hipMemsetAsync(r->accum.p, 0, size_t(r->npix) * 16, st_), "accum clear");
+  hip_check(launch_fill_u32(reinterpret_cast<uint32_t *>(ws_.req_invalid.as<unsigned long long>() + r->slot), 0u, 2u,
+                            st_),
+            "invalid-sample counter clear");
   DevRequest *hr = ws_.h_reqs.as<DevRequest>() + r->slot;
   *hr = r->dev;
   hip_check(hipMemcpyAsync(ws_.reqs.as<DevRequest>() + r->slot, hr, sizeof(DevRequest), hipMemcpyHostToDevice, st_),
@@ -296,6 +302,7 @@ bool Engine::admit_chunks(RequestRec *r) {
     c.index = r->next_admit;
     c.size = size;
     c.buf_base = base;
+    c.buf_len = size;
     WorkItem *hw = ws_.h_items.as<WorkItem>() + slot;
     std::memset(hw, 0, sizeof *hw);
     hw->req = r->slot;
@@ -479,7 +486,9 @@ void Engine::resolve_ready(std::vector<RequestRec *> &finished) {
       int32_t slot = r->chunk_item[r->next_resolve];
       ChunkRec &c = slots_[uint32_t(slot)];
       if (!c.complete) break;
-      if (r->keep_samples) {
+      if (r->cancelled) {
+        // the chunk's paths have finished writing its buffer range: nothing to resolve
+      } else if (r->keep_samples) {
         float *dst = r->samples.data() + size_t(r->next_resolve) * r->chunk * r->npix * 4;
         hip_check(hipMemcpyAsync(dst, ws_.sample_L.as<float4>() + c.buf_base, size_t(c.size) * 16,
                                  hipMemcpyDeviceToHost, st_),
@@ -488,7 +497,7 @@ void Engine::resolve_ready(std::vector<RequestRec *> &finished) {
       } else {
         hip_check(launch_resolve(A, uint32_t(slot), r->dev, r->npix, s_->ctx->resolve_tiles, st_), "k_resolve");
       }
-      free_range(c.buf_base, c.size);
+      free_range(c.buf_base, c.buf_len);
       free_items_.push_back(uint32_t(slot));
       c.req = nullptr;
       r->chunk_item[r->next_resolve] = -1;
@@ -505,13 +514,18 @@ void Engine::resolve_ready(std::vector<RequestRec *> &finished) {
 }
 
 void Engine::finalize(RequestRec *r) {
-  if (r->out_rgb && !r->keep_samples) {
-    DBuf rgb;
+  unsigned long long *inv = ws_.h_req_invalid.as<unsigned long long>() + r->slot;
+  hip_check(hipMemcpyAsync(inv, ws_.req_invalid.as<unsigned long long>() + r->slot, sizeof *inv, hipMemcpyDeviceToHost,
+                           st_),
+            "invalid-sample count D2H");
+  DBuf rgb;
+  if (r->out_rgb && !r->keep_samples && !r->cancelled) {
     rgb.alloc(size_t(r->npix) * 12);
     hip_check(launch_finalize(r->accum.p, rgb.as<float>(), r->npix, st_), "k_finalize");
     hip_check(hipMemcpyAsync(r->out_rgb, rgb.p, size_t(r->npix) * 12, hipMemcpyDeviceToHost, st_), "film D2H");
-    hip_check(hipStreamSynchronize(st_), "film sync");
   }
+  hip_check(hipStreamSynchronize(st_), "film sync");
+  r->invalid_samples = *inv;
   r->accum.release();
   reqs_by_slot_[r->slot] = nullptr;
   free_reqs_.push_back(r->slot);
@@ -521,6 +535,36 @@ void Engine::finalize(RequestRec *r) {
 
 void Engine::run() {
   while (busy()) bounce();
+}
+
+std::vector<RequestRec *> Engine::cancel(RequestRec *r) {
+  std::vector<RequestRec *> finished;
+  if (r->done || r->cancelled) return finished;
+  r->cancelled = true;
+  r->n_chunks = r->next_admit;
+  pending_.erase(std::remove(pending_.begin(), pending_.end(), r), pending_.end());
+  // Host-side started counts include the samples of a bounce launched but
+  // not yet collected, so a shrunk chunk completes when every path it
+  // started has counted itself done: at once if it started none or the last
+  // collected counts already show them all (nothing may be in flight then),
+  // otherwise in a later collect().
+  const uint32_t *done = ws_.h_done.as<uint32_t>();
+  for (uint32_t k = r->next_resolve; k < r->next_admit; k++) {
+    uint32_t slot = uint32_t(r->chunk_item[k]);
+    ChunkRec &c = slots_[slot];
+    c.size = c.started;
+    uint32_t n = 0;
+    if (!launched_)
+      for (uint32_t w = 0; w < kDoneWays; w++) n += done[w * kMaxItems + slot];
+    if (c.size == 0 || n == c.size) c.complete = true;
+  }
+  if (r->next_resolve == r->n_chunks) {  // nothing admitted and unresolved
+    finalize(r);
+    finished.push_back(r);
+    return finished;
+  }
+  resolve_ready(finished);
+  return finished;
 }
 
 uint64_t Engine::unstarted() const {

@@ -38,10 +38,12 @@ struct RequestRec {
   std::vector<float> samples;      // keep_samples: n x 4 radiance (rgb, unused)
   uint32_t next_admit = 0, next_resolve = 0;
   uint64_t started = 0;            // samples handed to k_regen so far
-  uint64_t left() const { return uint64_t(npix) * spp - started; }
+  uint64_t left() const { return cancelled ? 0 : uint64_t(npix) * spp - started; }
   std::vector<int32_t> chunk_item;  // item slot per admitted, unresolved chunk (-1 otherwise)
   double t_submit = 0, t_admit = -1, t_done = -1;  // ms on the engine clock
   bool done = false;
+  bool cancelled = false;          // Engine::cancel: no film is produced
+  uint64_t invalid_samples = 0;    // samples the resolve dropped (set at finalize)
   uint64_t film_bytes() const { return uint64_t(npix) * 16; }
 };
 
@@ -49,9 +51,10 @@ struct RequestRec {
 struct ChunkRec {
   RequestRec *req = nullptr;
   uint32_t index = 0;     // chunk index inside the request
-  uint32_t size = 0;      // samples = npix * k
+  uint32_t size = 0;      // samples = npix * k (a cancelled chunk: the samples it started)
   uint32_t started = 0;   // samples handed to k_regen so far
   uint32_t buf_base = 0;  // first sample-buffer entry
+  uint32_t buf_len = 0;   // sample-buffer entries held (npix * k, whatever size becomes)
   bool complete = false;
 };
 
@@ -75,6 +78,13 @@ class Engine {
   // Request slots still free (submit needs one per request in flight).
   uint32_t free_request_slots() const { return uint32_t(free_reqs_.size()); }
   uint32_t requests_in_flight() const { return kMaxRequests - uint32_t(free_reqs_.size()); }
+  // Stops a submitted request: its unadmitted chunks are dropped and its
+  // admitted ones shrink to the samples already started, so they finish with
+  // the paths in flight and free their slots, buffer ranges and film without
+  // a resolve.  Returns the requests finalized by the call (r itself when
+  // nothing of it is in flight); the rest come out of later collect() calls,
+  // marked cancelled.  No effect on a finished request.
+  std::vector<RequestRec *> cancel(RequestRec *r);
   // Non-blocking progress: a launched bounce whose work has finished on the
   // device (collect() then returns without waiting).
   bool launched() const { return launched_; }

@@ -52,6 +52,7 @@ struct RenderArgs {
   uint32_t *counters;            // at ctr(k): [0],[1] extension queues, [2] shadow queue, [4],[5] fetch
                                  // counters
   unsigned long long *stats;     // [0] extension rays, [1] shadow rays, [2..5] traversal counters
+  unsigned long long *req_invalid;  // per request slot: samples the film resolve dropped
 };
 
 // Counter k of the bounce kernels lives at word ctr(k), 256 B from the

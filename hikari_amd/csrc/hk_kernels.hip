@@ -1177,10 +1177,16 @@ __global__ __launch_bounds__(kBlock) void k_resolve_plain(RenderArgs A, uint32_t
   float R = rq.filter_radius;
   int rad = (int)floorf(R + 0.5f);
   uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t bad = 0;  // the pixel's own samples that the film drops
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < it.npix; p += stride) {
     int x = (int)(p % (uint32_t)W), y = (int)(p / (uint32_t)W);
     float4 *accum = reinterpret_cast<float4 *>(rq.accum);
     float4 acc = accum[p];
+    uint32_t own = rq.tile ? tile_rank((uint32_t)x, (uint32_t)y, (uint32_t)W, (uint32_t)H, rq.tile) : p;
+    for (uint32_t s = 0; s < it.k; s++) {
+      float4 l4 = A.sample_L[it.buf_base + s * it.npix + own];
+      if (!color_valid(v3(l4.x, l4.y, l4.z))) bad++;
+    }
     for (uint32_t s = 0; s < it.k; s++) {
       for (int qy = max(0, y - rad); qy <= min(H - 1, y + rad); qy++) {
         for (int qx = max(0, x - rad); qx <= min(W - 1, x + rad); qx++) {
@@ -1206,6 +1212,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_plain(RenderArgs A, uint32_t
     }
     accum[p] = acc;
   }
+  if (bad) atomicAdd(&A.req_invalid[it.req], (unsigned long long)bad);
 }
 
 // The same accumulation for whole images, a 16x16 pixel tile per block: for
@@ -1243,6 +1250,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t
   int tiles_x = (W + kResTile - 1) / kResTile, tiles_y = (H + kResTile - 1) / kResTile;
   float4 *accum = reinterpret_cast<float4 *>(rq.accum);
   const float lookup = rq.lookup;
+  uint32_t bad = 0;  // dropped samples of the pixels inside this block's tiles (each counted once)
   for (int tile = blockIdx.x; tile < tiles_x * tiles_y; tile += gridDim.x) {
     int tx0 = (tile % tiles_x) * kResTile, ty0 = (tile / tiles_x) * kResTile;
     int x = tx0 + (int)threadIdx.x % kResTile, y = ty0 + (int)threadIdx.x / kResTile;
@@ -1279,6 +1287,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t
           if (s + 1 < it.k) next[j] = A.sample_L[it.buf_base + (s + 1) * it.npix + rank[j]];
           // invalid samples are dropped (Block.cpp:97-101)
           if (color_valid(v3(l4.x, l4.y, l4.z))) l = make_float4(l4.x, l4.y, l4.z, 1.0f);
+          else if (qx >= tx0 && qx < tx0 + kResTile && qy >= ty0 && qy < ty0 + kResTile) bad++;
           int x0 = (int)ceilf(cx - R), x1 = (int)floorf(cx + R);
           int y0 = (int)ceilf(cy - R), y1 = (int)floorf(cy + R);
           for (int k = 0; k <= 2 * rad; k++) {
@@ -1306,6 +1315,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_tiles(RenderArgs A, uint32_t
     }
     if (own) accum[y * W + x] = acc;
   }
+  if (bad) atomicAdd(&A.req_invalid[it.req], (unsigned long long)bad);
 }
 
 __global__ __launch_bounds__(kBlock) void k_finalize(const float4 *accum, float *rgb, uint32_t n) {

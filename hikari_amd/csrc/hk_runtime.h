@@ -82,7 +82,8 @@ struct Workspace {
   DBuf counters, stats;
   DBuf reqs, items, plan, item_done, hdr;          // request / item tables, regen plan, per-item counts,
                                                    // bounce header
-  PinnedBuf h_reqs, h_items, h_plan, h_done, h_counters, h_hdr;  // host mirrors / staging
+  DBuf req_invalid;                                // invalid samples dropped per request slot (u64)
+  PinnedBuf h_reqs, h_items, h_plan, h_done, h_counters, h_hdr, h_req_invalid;  // host mirrors / staging
   void ensure(uint32_t paths, uint32_t emitters, uint32_t samples);
   uint64_t device_bytes() const;
   void release();

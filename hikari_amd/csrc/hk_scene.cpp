@@ -311,6 +311,8 @@ int hk_render_batch(hk_scene *s, const hk_request *reqs, uint32_t n, float **out
   for (auto &r : rs) eng.submit(r.get());
   eng.run();
   eng.take_stats(stats);
+  if (stats)
+    for (auto &r : rs) stats->invalid_samples += r->invalid_samples;
   return HK_OK;
   HK_API_END(ctx)
 }

@@ -80,6 +80,7 @@ struct Ticket {
   std::string err;
   bool done = false;
   bool admitted = false;  // its film counts against the budget
+  bool submitted = false; // handed to its lane's engine (worker only)
 };
 
 }  // namespace
@@ -91,6 +92,7 @@ struct hk_server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<uint64_t> arrivals;          // tickets waiting for admission
+  std::vector<uint64_t> cancels;          // admitted tickets to cancel (the worker's next round)
   std::map<uint64_t, Ticket> tickets;
   uint64_t next_ticket = 1;
   std::thread worker;
@@ -106,6 +108,7 @@ static double clock_ms() {
 }
 
 static void finish_ticket(hk_server *srv, Ticket &t, int status, const std::string &err) {
+  if (status == HK_ERR_CANCELLED) srv->st.cancelled++;
   t.status = status;
   t.err = err;
   t.done = true;
@@ -146,6 +149,7 @@ static void worker_main(hk_server *srv) {
   std::deque<Ticket *> bulk_wait;
   for (;;) {
     std::vector<Ticket *> admitted;
+    std::vector<uint64_t> cancels;
     {
       std::unique_lock<std::mutex> lk(srv->mu);
       // HBM-budget admission, first come first served.  A request whose
@@ -177,7 +181,8 @@ static void worker_main(hk_server *srv) {
         free--;
         admitted.push_back(&t);
       }
-      if (admitted.empty() && bulk_wait.empty() && !any_lane_active(srv)) {
+      cancels.swap(srv->cancels);
+      if (admitted.empty() && bulk_wait.empty() && cancels.empty() && !any_lane_active(srv)) {
         // (nothing active means no request is in flight, so every queued
         // request would have been admitted or refused above)
         if (srv->stop) return;
@@ -194,6 +199,39 @@ static void worker_main(hk_server *srv) {
     for (Ticket *tp : admitted) {
       if (tp->lane == kBulk) bulk_wait.push_back(tp);
       else to_submit.push_back(tp);
+    }
+    // Cancellations of admitted requests: one waiting for the bulk lane is
+    // dropped here; one in an engine stops there, and finishes (cancelled)
+    // now or when its started paths have finished.
+    std::vector<RequestRec *> cancelled_now;
+    for (uint64_t id : cancels) {
+      Ticket *tp = nullptr;
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->tickets.find(id);
+        if (it != srv->tickets.end() && !it->second.done) tp = &it->second;
+      }
+      if (!tp) continue;
+      auto wit = std::find(bulk_wait.begin(), bulk_wait.end(), tp);
+      auto sit = std::find(to_submit.begin(), to_submit.end(), tp);
+      if (wit != bulk_wait.end() || sit != to_submit.end()) {
+        if (wit != bulk_wait.end()) bulk_wait.erase(wit);
+        if (sit != to_submit.end()) to_submit.erase(sit);
+        std::lock_guard<std::mutex> lk(srv->mu);
+        finish_ticket(srv, *tp, HK_ERR_CANCELLED, "request cancelled");
+        srv->cv_done.notify_all();
+      } else if (tp->submitted) {
+        try {
+          std::vector<RequestRec *> f = srv->scenes[size_t(tp->scene)]->lane[tp->lane]->eng->cancel(tp->rec.get());
+          cancelled_now.insert(cancelled_now.end(), f.begin(), f.end());
+        } catch (const HkError &e) {
+          stop_worker(srv, e.code, e.msg);
+          return;
+        } catch (const std::exception &e) {
+          stop_worker(srv, HK_ERR_HIP, e.what());
+          return;
+        }
+      }
     }
     if (!bulk_wait.empty() && bulk_idle(srv)) {
       to_submit.push_back(bulk_wait.front());
@@ -214,6 +252,7 @@ static void worker_main(hk_server *srv) {
         msg = e.what();
       }
       if (code == HK_OK) {
+        t.submitted = true;
         accepted.push_back(&t);
         continue;
       }
@@ -222,8 +261,8 @@ static void worker_main(hk_server *srv) {
       srv->cv_done.notify_all();
     }
     try {
-      bool progressed = !to_submit.empty();
-      std::vector<RequestRec *> finished;
+      bool progressed = !to_submit.empty() || !cancels.empty();
+      std::vector<RequestRec *> finished = cancelled_now;
       // bounces the device has finished
       for (auto &sc : srv->scenes)
         for (auto &l : sc->lane)
@@ -233,7 +272,7 @@ static void worker_main(hk_server *srv) {
             progressed = true;
           }
       // next bounces of every lane with work
-      uint64_t in_flight = 0;
+      uint64_t in_flight = 0;  // requests holding an engine's request slot
       for (auto &sc : srv->scenes)
         for (auto &l : sc->lane) in_flight += l->eng->requests_in_flight();
       uint64_t launches = 0;
@@ -252,8 +291,13 @@ static void worker_main(hk_server *srv) {
         srv->st.bounces += launches;
         if (in_flight >= 2) srv->st.shared_bounces += launches;
         srv->st.max_in_flight = std::max<uint64_t>(srv->st.max_in_flight, in_flight);
+        srv->st.requests_in_flight = in_flight;
         for (RequestRec *r : finished) {
           Ticket &t = srv->tickets[r->id];
+          if (r->cancelled) {
+            finish_ticket(srv, t, HK_ERR_CANCELLED, "request cancelled");
+            continue;
+          }
           finish_ticket(srv, t, HK_OK, "");
           srv->st.completed++;
           srv->st.paths += uint64_t(r->npix) * r->spp;
@@ -399,10 +443,36 @@ int hk_server_wait(hk_server *srv, uint64_t ticket, hk_server_result *res) {
       res->submit_ms = t.rec->t_submit - srv->t0;
       res->paths = uint64_t(t.rec->npix) * t.rec->spp;
       res->lane = t.lane;
+      res->invalid_samples = t.rec->invalid_samples;
     }
   }
   srv->tickets.erase(it);
   if (status != HK_OK) throw HkError(status, "%s", err.c_str());
+  return HK_OK;
+  HK_API_END(ctx)
+}
+
+int hk_server_cancel(hk_server *srv, uint64_t ticket) {
+  hk_ctx *ctx = srv ? srv->ctx : nullptr;
+  HK_API_BEGIN
+  if (!srv) throw HkError(HK_ERR_INVALID, "null argument");
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->tickets.find(ticket);
+    if (it == srv->tickets.end()) throw HkError(HK_ERR_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
+    Ticket &t = it->second;
+    if (t.done) return HK_OK;  // finished (or failed) before the call: keeps its result
+    auto a = std::find(srv->arrivals.begin(), srv->arrivals.end(), ticket);
+    if (a != srv->arrivals.end()) {  // not admitted yet: the worker never saw it
+      srv->arrivals.erase(a);
+      finish_ticket(srv, t, HK_ERR_CANCELLED, "request cancelled");
+      srv->cv_done.notify_all();
+      return HK_OK;
+    }
+    if (std::find(srv->cancels.begin(), srv->cancels.end(), ticket) == srv->cancels.end())
+      srv->cancels.push_back(ticket);
+  }
+  srv->cv_work.notify_one();
   return HK_OK;
   HK_API_END(ctx)
 }

@@ -29,6 +29,7 @@ extern "C" {
 #define HK_ERR_UNSUPPORTED (-4)
 #define HK_ERR_IO (-5)
 #define HK_ERR_BUDGET (-6)
+#define HK_ERR_CANCELLED (-7)
 
 typedef struct hk_ctx hk_ctx;
 typedef struct hk_accel hk_accel;
@@ -192,6 +193,8 @@ typedef struct hk_render_stats {
   uint64_t trace_launches;
   uint64_t node_visits, triangle_tests;  /* BVH work; non-zero only in builds with HK_TRAVERSAL_STATS */
   uint64_t graph_bounces;                /* bounces replayed from a captured HIP graph */
+  uint64_t invalid_samples;              /* samples the film dropped (NaN / inf / negative radiance,
+                                            src/core/Block.cpp:97-101) */
 } hk_render_stats;
 
 /* Whole-image render with the gpuPathMIS integrator (the wavefront form of
@@ -263,12 +266,15 @@ typedef struct hk_server_result {
   uint64_t paths;          /* path samples of the request */
   double submit_ms;        /* submit time on the server clock (ms since hk_server_create) */
   int32_t lane;            /* 0 bulk, 1 express */
+  uint64_t invalid_samples; /* samples the film dropped (NaN / inf / negative radiance) */
 } hk_server_result;
 typedef struct hk_server_stats {
   uint64_t submitted, completed, bounces, paths;
   uint64_t fixed_bytes, film_bytes;   /* HBM held: workspaces, admitted films */
   uint64_t shared_bounces;            /* bounces launched while two or more requests were in flight */
   uint64_t max_in_flight;             /* most requests in flight at once */
+  uint64_t cancelled;                 /* requests ended by hk_server_cancel */
+  uint64_t requests_in_flight;        /* requests holding a lane's request slot now */
 } hk_server_stats;
 
 int hk_server_create(hk_ctx *ctx, uint64_t hbm_budget, hk_server **out);
@@ -278,6 +284,14 @@ int hk_server_add_scene(hk_server *server, hk_scene *scene, int32_t *scene_id);
 int hk_server_submit(hk_server *server, int32_t scene_id, const hk_request *req, float *out_rgb, uint64_t *ticket);
 /* blocks until the request is done; its status is also the return value */
 int hk_server_wait(hk_server *server, uint64_t ticket, hk_server_result *result);
+/* No counterpart in the reference (one render per process there).  Cancels a
+ * submitted request that has not been waited for: one still queued is dropped
+ * at once; one in flight starts no further samples, and its chunk slots,
+ * sample-buffer ranges, film memory and request slot are reclaimed as soon as
+ * the paths it already started have finished.  Its out_rgb is left as it was
+ * and hk_server_wait returns HK_ERR_CANCELLED.  A request that finished before
+ * the call keeps its result.  HK_ERR_INVALID for an unknown ticket. */
+int hk_server_cancel(hk_server *server, uint64_t ticket);
 int hk_server_get_stats(hk_server *server, hk_server_stats *stats);
 /* finishes every submitted request, then frees the server */
 int hk_server_destroy(hk_server *server);

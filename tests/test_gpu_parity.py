@@ -477,6 +477,108 @@ def test_server_more_requests_than_slots():
     c.close()
 
 
+def test_server_cancel_reclaims_slots():
+    """hk_server_cancel on requests that are queued, waiting for the bulk lane
+    and in flight on either lane: each ends with HK_ERR_CANCELLED (or keeps
+    its result when it had finished first), the server gives back every
+    request slot and film byte, and requests submitted afterwards return the
+    films they have alone."""
+    import time
+    box = scenes.cbox(width=16, height=16, sphere_subdiv=1)
+    c = hk.Context(0)
+    c.set_option("max_paths", 20000)
+    c.set_option("max_samples", 1 << 20)  # express lane: requests of at most 65,536 samples
+    g = c.scene(box)
+    srv = hk.Server(c)
+    srv.add_scene(g)
+    far = D.lookat((0.0, 0.0, 0.99), (0, 0, -1), (0, 1, 0))
+    # bulk requests (16 chunks of 16 spp each): the first is cancelled while
+    # it runs, the second while it waits behind the first
+    long1 = hk.make_request(cam_to_world=far, fov=45.0, width=256, height=256, spp=256, seed=7)
+    long2 = hk.make_request(cam_to_world=far, fov=45.0, width=256, height=256, spp=256, seed=8)
+    t1, t2 = srv.submit(0, long1), srv.submit(0, long2)
+    # express requests, every other one cancelled right after its submit
+    small = [hk.make_request(box, spp=1 + i % 3, seed=9000 + i) for i in range(1500)]
+    ts = []
+    for i, r in enumerate(small):
+        ts.append(srv.submit(0, r))
+        if i % 2:
+            srv.cancel(ts[-1])
+    time.sleep(0.05)
+    srv.cancel(t2)
+    srv.cancel(t1)
+    outcome = {"cancelled": 0, "done": 0}
+    for i, t in enumerate([t1, t2] + ts):
+        try:
+            img, res = srv.wait(t)
+        except hk.HkError as e:
+            assert e.code == hk.HK_ERR_CANCELLED
+            outcome["cancelled"] += 1
+            continue
+        assert res["status"] == 0
+        outcome["done"] += 1
+        if i >= 2 and (i - 2) % 2 == 0 and (i - 2) % 97 == 0:
+            r = small[i - 2]
+            assert np.array_equal(img.view(np.uint32), g.render(0, request=r)[0].view(np.uint32)), i
+    st = srv.stats()
+    print(outcome, st)
+    assert outcome["cancelled"] >= 2 and st["cancelled"] == outcome["cancelled"]
+    assert st["requests_in_flight"] == 0 and st["film_bytes"] == 0
+    # the slots and films came back: new requests render as they do alone
+    after = [hk.make_request(box, spp=2, seed=12000 + i) for i in range(3)] + [long1]
+    for r in after:
+        img, res = srv.wait(srv.submit(0, r))
+        assert np.array_equal(img.view(np.uint32), g.render(0, request=r)[0].view(np.uint32))
+    assert srv.stats()["requests_in_flight"] == 0
+    srv.close()
+    c.close()
+
+
+def _negative_light_scene():
+    """A diffuse floor lit by a quad whose radiance has a negative channel:
+    every sample that sees the light, directly or through an emitter sample,
+    carries a negative component and is dropped by the film
+    (src/core/Block.cpp:97-101, the test of src/core/Color.cpp:44-55)."""
+    sc = D.Scene(width=12, height=10, fov=50.0, max_depth=1000)
+    floor = sc.add_material(D.diffuse(0.5))
+    black = sc.add_material(D.diffuse(0.0))
+    sc.add_mesh(np.array([[-3, 0, -3], [3, 0, -3], [3, 0, 3], [-3, 0, 3]], np.float32),
+                np.array([[0, 2, 1], [0, 3, 2]], np.uint32), floor)
+    sc.add_mesh(np.array([[-0.4, 1.0, -0.4], [0.4, 1.0, -0.4], [0.4, 1.0, 0.4], [-0.4, 1.0, 0.4]], np.float32),
+                np.array([[0, 1, 2], [0, 2, 3]], np.uint32), black, radiance=(-1.0, 2.0, 2.0))
+    sc.cam_to_world = D.lookat((0.0, 1.6, 2.4), (0.0, 0.3, 0.0), (0, 1, 0))
+    return sc
+
+
+def test_invalid_samples_counted(ctx):
+    """The film's count of dropped samples (hk_render_stats.invalid_samples,
+    hk_server_result.invalid_samples) equals the number of samples whose
+    radiance is negative or not finite, counted one by one through the
+    per-sample path (hk_render_pixel_samples: the same streams), and agrees
+    with the FP32 oracle's samples."""
+    sc = _negative_light_scene()
+    spp = 8
+    gs = ctx.scene(sc)
+    img, st = gs.render(spp, seed=5)
+    bad = bad_o = 0
+    for y in range(sc.height):
+        for x in range(sc.width):
+            v = gs.pixel_samples(x, y, 0, spp, seed=5)[:, :3]
+            bad += int((~(np.isfinite(v).all(1) & (v >= 0).all(1))).sum())
+            o = O.pixel_samples(sc, x, y, 0, spp, req_seed=5)[:, :3]
+            bad_o += int((~(np.isfinite(o).all(1) & (o >= 0).all(1))).sum())
+    total = sc.width * sc.height * spp
+    print("invalid samples", st["invalid_samples"], "per-sample count", bad, "oracle", bad_o, "of", total)
+    assert 0 < bad < total
+    assert st["invalid_samples"] == bad
+    assert abs(bad_o - bad) <= max(2, total // 200)
+    srv = hk.Server(ctx)
+    srv.add_scene(gs)
+    _, res = srv.wait(srv.submit(0, hk.make_request(sc, spp=spp, seed=5)))
+    assert res["invalid_samples"] == bad
+    srv.close()
+
+
 @pytest.mark.parametrize("env_h", [16, 32])
 def test_environment_sampling_closed_form_gpu(ctx, env_h):
     """The GPU environment sampler + estimator on the constant-sky plane
