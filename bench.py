@@ -454,7 +454,8 @@ def main():
         _, st = gs.render(a.spp, seed=1000 + i)
         log(f"warmup {i}: {st['gpu_ms']:.1f} ms")
     lat, tot = [], {"paths": 0, "extension_rays": 0, "shadow_rays": 0, "trace_ms": 0.0, "trace_launches": 0,
-                    "gpu_ms": 0.0, "node_visits": 0, "triangle_tests": 0, "bounces_launched": 0, "graph_bounces": 0}
+                    "gpu_ms": 0.0, "node_visits": 0, "triangle_tests": 0, "bounces_launched": 0, "graph_bounces": 0,
+                    "invalid_samples": 0}
     t_all = time.perf_counter()
     for i in range(a.steps):
         t0 = time.perf_counter()
@@ -526,6 +527,7 @@ def main():
         "autotune": None if tune is None else {k: tune[k] for k in ("max_paths", "fetch_batch", "refill",
                                                                     "refill_shadow", "grid", "tile", "total_ms")},
         "bounces": tot["bounces_launched"], "graph_bounces": tot["graph_bounces"],
+        "invalid_samples": tot["invalid_samples"],
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "rays_per_s": rays / wall,
         "rays_per_path": rays / max(1, tot["paths"]),
