@@ -1001,8 +1001,12 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
         // BSDF sampling (:111-120), MIS bookkeeping (:122-140), RR (:143-151).
         // whitted (WhittedIntegrator.cpp:100-113): a diffuse vertex ends the
         // path; any other continues with probability 0.95, drawn before the
-        // BSDF sample, while depth < the depth property.
-        bool sample_bsdf = mode != 3 || (!diffuse_mat && rng.next() < 0.95f && depth < (uint32_t)sc.max_depth);
+        // BSDF sample, while depth < the depth property.  Below depth
+        // sc.rr_start (0 unless set) the roulette's number is drawn but the
+        // path continues at weight 1.
+        const bool rr = depth >= (uint32_t)sc.rr_start;
+        bool sample_bsdf =
+            mode != 3 || (!diffuse_mat && (rng.next() < 0.95f || !rr) && depth < (uint32_t)sc.max_depth);
         if (sample_bsdf) {
           BRec br;
           br.wi = wi_loc;
@@ -1015,7 +1019,7 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
           nd = to_world(fr, br.wo);
           beta = beta * f;
           if (mode == 3) {
-            beta = beta * (1.0f / 0.95f);
+            if (rr) beta = beta * (1.0f / 0.95f);
             flags = 0u;
             cont = !(beta.x == 0.0f && beta.y == 0.0f && beta.z == 0.0f);
             if (cont) depth++;
@@ -1026,8 +1030,8 @@ __global__ __launch_bounds__(kBlock, HK_SHADE_MIN_WAVES) void k_shade(RenderArgs
               w_mats = 1.0f;
               flags = kFlagPrevDiscrete;
             }
-            if (rng.next() < 0.95f) {
-              beta = beta * (1.0f / 0.95f);
+            if (rng.next() < 0.95f || !rr) {
+              if (rr) beta = beta * (1.0f / 0.95f);
               depth++;
               cont = depth < (uint32_t)sc.max_depth;
             }

@@ -264,6 +264,7 @@ int hk_scene_create(hk_ctx *ctx, const hk_scene_desc *d, hk_scene **out) {
   ds.max_depth = d->max_depth < 0 ? int32_t(kMaxPathDepth) : std::min(d->max_depth, int32_t(kMaxPathDepth));
   if (d->integrator < 0 || d->integrator > 3) throw HkError(HK_ERR_INVALID, "unknown integrator %d", d->integrator);
   ds.integrator = d->integrator;
+  ds.rr_start = std::max(0, d->rr_start_depth);
   for (int c = 0; c < 3; c++) ds.background[c] = d->background[c];
   ds.force_background = d->force_background;
   ds.root_ref = 0;

@@ -128,6 +128,7 @@ struct DevScene {
   int32_t n_nodes;          // BVH nodes in `nodes`
   int32_t integrator;       // 0 pathMIS, 1 pathEMS, 2 pathMATS
   int32_t n_mats;           // records in mats
+  int32_t rr_start;         // Russian roulette from this path depth on (0: every vertex)
 };
 
 // One render request as the kernels see it.

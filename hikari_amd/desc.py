@@ -122,7 +122,7 @@ class SceneDesc(C.Structure):
                 ("filter", C.c_int32), ("filter_radius", C.c_float), ("filter_stddev", C.c_float),
                 ("filter_b", C.c_float), ("filter_c", C.c_float),
                 ("max_depth", C.c_int32), ("background", C.c_float * 3), ("force_background", C.c_int32),
-                ("rft_dir", C.c_char_p), ("integrator", C.c_int32)]
+                ("rft_dir", C.c_char_p), ("integrator", C.c_int32), ("rr_start_depth", C.c_int32)]
 
 INTEGRATORS = {"pathMIS": 0, "pathEMS": 1, "pathMATS": 2, "whitted": 3}
 
@@ -176,6 +176,7 @@ class Scene:
     background: tuple = (0.0, 0.0, 0.0)
     force_background: bool = False
     integrator: str = "pathMIS"
+    rr_start_depth: int = 0
 
     def add_material(self, rec) -> int:
         self.materials.append(np.asarray(rec, np.float32))
@@ -236,6 +237,7 @@ class Scene:
         s.background[:] = [float(x) for x in self.background]
         s.force_background = int(self.force_background)
         s.integrator = INTEGRATORS[self.integrator or "pathMIS"]
+        s.rr_start_depth = int(self.rr_start_depth)
         rft = DATA_DIR.encode()
         keep.append(rft)
         s.rft_dir = rft

@@ -44,9 +44,17 @@ namespace {
 class GPUPathMISIntegrator : public Integrator {
  public:
   // depth: required by the path integrators (PathMISIntegrator.cpp:13),
-  // optional for whitted (-1, no limit: WhittedIntegrator.cpp:13)
+  // optional for whitted (-1, no limit: WhittedIntegrator.cpp:13).
+  // Two properties the reference does not have, both optional: queueSize,
+  // the paths in flight (the context's "max_paths"; 0 keeps the context's
+  // setting; no film depends on it), and rrStartDepth, the path depth from
+  // which Russian roulette plays (0, the reference's schedule: every vertex).
   explicit GPUPathMISIntegrator(const PropertyList &p, int estimator = 0) : estimator_(estimator) {
     depth_ = estimator == 3 ? p.GetInteger("depth", -1) : p.GetInteger("depth");
+    queue_size_ = p.GetInteger("queueSize", 0);
+    rr_start_ = p.GetInteger("rrStartDepth", 0);
+    if (queue_size_ < 0) throw HikariException("queueSize must be >= 0 (got %d)", queue_size_);
+    if (rr_start_ < 0) throw HikariException("rrStartDepth must be >= 0 (got %d)", rr_start_);
   }
   int EstimatorId() const override { return estimator_; }
   ~GPUPathMISIntegrator() override {
@@ -60,6 +68,7 @@ class GPUPathMISIntegrator : public Integrator {
     std::vector<float> records;
     std::vector<hk_mesh_desc> meshes;
     hk_scene_desc d = scene->Describe(depth_, estimator_, records, meshes);
+    d.rr_start_depth = rr_start_;
     check_ctx(hk_scene_create(GpuContext(), &d, &scene_), "gpuPathMIS: scene upload");
     width_ = d.width;
     height_ = d.height;
@@ -71,6 +80,7 @@ class GPUPathMISIntegrator : public Integrator {
     req.use_scene_camera = 1;
     req.spp = uint32_t(scene->GetSampler()->GetSampleCount());
     rgb.assign(size_t(width_) * height_ * 3, 0.0f);
+    if (queue_size_ > 0) check_ctx(hk_set_option(GpuContext(), "max_paths", queue_size_), "gpuPathMIS: queueSize");
     check_ctx(hk_render(scene_, &req, rgb.data(), nullptr), "gpuPathMIS: render");
   }
   // per-sample radiance of one pixel (the t-test harness)
@@ -81,12 +91,16 @@ class GPUPathMISIntegrator : public Integrator {
   }
   std::string ToString() const override {
     static const char *names[] = {"pathMIS", "pathEMS", "pathMATS", "whitted"};
-    return std::string("GPUPathIntegrator[") + names[estimator_] + ", depth = " + std::to_string(depth_) + "]";
+    std::string s = std::string("GPUPathIntegrator[") + names[estimator_] + ", depth = " + std::to_string(depth_);
+    if (rr_start_) s += ", rrStartDepth = " + std::to_string(rr_start_);
+    if (queue_size_) s += ", queueSize = " + std::to_string(queue_size_);
+    return s + "]";
   }
 
  private:
   int estimator_;
   int depth_;
+  int queue_size_ = 0, rr_start_ = 0;
   hk_scene *scene_ = nullptr;
   int width_ = 0, height_ = 0;
 };

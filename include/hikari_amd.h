@@ -157,6 +157,8 @@ typedef struct hk_scene_desc {
   int32_t max_depth; float background[3]; int32_t force_background;
   const char *rft_dir;                          /* nullable */
   int32_t integrator;                           /* 0 pathMIS, 1 pathEMS, 2 pathMATS, 3 whitted */
+  int32_t rr_start_depth;                       /* Russian roulette from this path depth on (0: from the
+                                                   first vertex, as the reference) */
 } hk_scene_desc;
 
 /* Integrators (estimators of the bounce kernel): pathMIS
@@ -164,7 +166,11 @@ typedef struct hk_scene_desc {
  * only, PathEMSIntegrator.cpp:16-119), pathMATS (BSDF sampling only,
  * PathMATSIntegrator.cpp:16-87) and whitted (WhittedIntegrator.cpp:26-116).
  * max_depth: negative means unlimited (the reference reads it as uint32);
- * clamped to 262143.
+ * clamped to 262143.  rr_start_depth: no counterpart in the reference, whose
+ * estimators play Russian roulette (continue with probability 0.95, weight
+ * 1 / 0.95) at every vertex; vertices of depth below it continue with weight
+ * 1 instead (the roulette's random number is drawn all the same).  Unbiased
+ * either way; 0 (the default) is the reference's schedule, bit for bit.
  *
  * Replaces Scene::Activate (src/core/Scene.cpp:128-165) plus the Activate()
  * of every mesh / BSDF / emitter / camera: uploads geometry, BVH, material

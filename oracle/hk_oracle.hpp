@@ -42,7 +42,9 @@
 //   estimator           furnace 1/(1-a) (scene/pa5/tests/test-furnace.xml) and
 //                       direct-lighting references (test-direct.xml); the
 //                       environment sampler against the closed-form
-//                       expectation of a plane under a constant sky.
+//                       expectation of a plane under a constant sky; the
+//                       roulette start depth (no reference counterpart)
+//                       against the furnace closed form.
 #pragma once
 
 #include <algorithm>
@@ -75,6 +77,7 @@ enum Slip {
   kSlipRoughCoatingT,      // rough coating nested term with T(cos_i) twice
   kSlipRftDiffuse,         // EvalDiffuse from the table half of 1/eta
   kSlipFilmCentre,         // film footprint without the half-pixel centre offset
+  kSlipRrStartWeight,      // roulette weight 1 / 0.95 also below the roulette start depth
 };
 inline int &slip() {
   static int s = kSlipNone;

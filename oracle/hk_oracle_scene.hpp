@@ -376,6 +376,7 @@ struct SceneIn {
   int force_background = 0;
   const char *rft_dir = nullptr;
   int integrator = 0;  // 0 pathMIS, 1 pathEMS, 2 pathMATS, 3 whitted
+  int rr_start_depth = 0;
 };
 
 template <typename R>
@@ -413,6 +414,10 @@ struct Scene {
   FilmFilter film;
   int max_depth = 1000;
   int integrator = 0;  // 0 pathMIS, 1 pathEMS, 2 pathMATS, 3 whitted
+  // Russian roulette from this depth on; below it a vertex continues with
+  // weight 1 (the draw is made all the same).  Not a reference parameter:
+  // 0 is the reference's schedule (roulette at every vertex).
+  int rr_start = 0;
   V3<R> background;
   bool force_background = false;
   double bbox_lo[3], bbox_hi[3];
@@ -519,9 +524,10 @@ struct Scene {
     film.setup(in.filter, in.filter_radius, in.filter_stddev, in.filter_b, in.filter_c);
     // the depth property is read as uint32 (PathMISIntegrator.cpp:13,
     // WhittedIntegrator.cpp:13): a negative value means no limit; paths are
-    // capped at 4095 vertices as on the device
-    max_depth = in.max_depth < 0 ? 4095 : std::min(in.max_depth, 4095);
+    // capped at 262143 vertices as on the device
+    max_depth = in.max_depth < 0 ? 262143 : std::min(in.max_depth, 262143);
     integrator = in.integrator;
+    rr_start = std::max(0, in.rr_start_depth);
     background = V3<R>(in.background[0], in.background[1], in.background[2]);
     force_background = in.force_background != 0;
   }
@@ -698,6 +704,21 @@ struct PcgU1 : U1Source {
 // src/integrator/PathMISIntegrator.cpp:16-157.  Random draw order (the
 // reference's is compiler dependent, src/integrator/PathMISIntegrator.cpp:92):
 // per emitter (x, y, s1); BSDF (x, y [, extra draws inside Sample]); RR.
+// Russian roulette of the path estimators (PathMISIntegrator.cpp:143-151,
+// PathEMSIntegrator.cpp, PathMATSIntegrator.cpp): continue with probability
+// 0.95 at weight 1 / 0.95.  Below sc.rr_start (not a reference parameter;
+// 0 by default) the path continues at weight 1; the draw is made either way.
+template <typename R>
+inline bool roulette(const Scene<R> &sc, Pcg32 &rng, int depth, V3<R> &beta) {
+  bool go = rng.next_float() < 0.95f;
+  if (depth < sc.rr_start) {
+    if (slip() == kSlipRrStartWeight) beta = beta * R(1.0f / 0.95f);
+    return true;
+  }
+  if (go) beta = beta * R(1.0f / 0.95f);
+  return go;
+}
+
 template <typename R>
 V3<R> path_mis_li(const Scene<R> &sc, Pcg32 &rng, V3<R> ro, V3<R> rd, R rtmin, R rtmax, PathStats *st) {
   using Isect = typename Scene<R>::Isect;
@@ -773,8 +794,7 @@ V3<R> path_mis_li(const Scene<R> &sc, Pcg32 &rng, V3<R> ro, V3<R> rd, R rtmin, R
       if (pbm + plm != R(0)) w_mats = pbm / (pbm + plm);
     }
     if (br.measure == kDiscrete) { w_ems = 0; w_mats = 1; }
-    if (rng.next_float() < 0.95f) beta = beta * R(1.0f / 0.95f);
-    else break;
+    if (!roulette(sc, rng, depth, beta)) break;
     depth++;
   }
   return li;
@@ -811,8 +831,7 @@ V3<R> path_mats_li(const Scene<R> &sc, Pcg32 &rng, V3<R> ro, V3<R> rd, R rtmin, 
     R sx = rng.next_float(), sy = rng.next_float();
     beta = beta * bsdf.sample(br, sx, sy, &u1);
     if (is_zero_fuzzy(beta)) break;
-    if (rng.next_float() < 0.95f) beta = beta * R(1.0f / 0.95f);
-    else break;
+    if (!roulette(sc, rng, depth, beta)) break;
     to = is.p; td = is.sh.to_world(br.wo); ttmin = R(1e-4f); ttmax = std::numeric_limits<R>::infinity();
     depth++;
   }
@@ -876,8 +895,7 @@ V3<R> path_ems_li(const Scene<R> &sc, Pcg32 &rng, V3<R> ro, V3<R> rd, R rtmin, R
     R sx = rng.next_float(), sy = rng.next_float();
     beta = beta * bsdf.sample(br, sx, sy, &u1);
     if (is_zero_fuzzy(beta)) break;
-    if (rng.next_float() < 0.95f) beta = beta * R(1.0f / 0.95f);
-    else break;
+    if (!roulette(sc, rng, depth, beta)) break;
     last_emitter = mesh.emitter;
     to = is.p; td = is.sh.to_world(br.wo); ttmin = R(1e-4f); ttmax = std::numeric_limits<R>::infinity();
     depth++;
@@ -946,12 +964,13 @@ V3<R> whitted_li(const Scene<R> &sc, Pcg32 &rng, V3<R> ro, V3<R> rd, R rtmin, R 
       }
       break;
     }
-    if (!(rng.next_float() < 0.95f && depth < sc.max_depth)) break;  // :100-113
+    bool go = rng.next_float() < 0.95f, rr = depth >= sc.rr_start;
+    if (!((go || !rr) && depth < sc.max_depth)) break;  // :100-113
     BRec<R> br;
     br.wi = is.sh.to_local(R(-1) * td);
     R sx = rng.next_float(), sy = rng.next_float();
     V3<R> c = bsdf.sample(br, sx, sy, &u1);
-    beta = beta * c * R(1.0f / 0.95f);
+    beta = beta * c * (rr ? R(1.0f / 0.95f) : R(1));
     if (beta.x == R(0) && beta.y == R(0) && beta.z == R(0)) break;
     to = is.p; td = is.sh.to_world(br.wo); ttmin = R(1e-4f); ttmax = std::numeric_limits<R>::infinity();
     depth++;

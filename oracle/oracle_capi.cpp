@@ -27,6 +27,7 @@ struct ho_scene {
   int32_t max_depth; float background[3]; int32_t force_background;
   const char *rft_dir;
   int32_t integrator;
+  int32_t rr_start_depth;
 };
 
 static char g_err[512];
@@ -291,6 +292,7 @@ static void to_in(const ho_scene *s, SceneIn &in) {
   in.force_background = s->force_background;
   in.rft_dir = s->rft_dir;
   in.integrator = s->integrator;
+  in.rr_start_depth = s->rr_start_depth;
 }
 
 // Render samples [s0, s1) of every pixel.  out_rgb: W*H*3 resolved image

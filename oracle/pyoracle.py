@@ -144,7 +144,7 @@ def rft_eval_diffuse(path, stage, eta, alpha):
 # the tests that show a pinning check fails when the oracle is wrong.
 SLIPS = ["none", "ggx_g1", "beckmann_g1", "conductor_fresnel", "rough_conductor_g", "rough_dielectric_eta",
          "plastic_fresnel", "plastic_eta2", "rough_plastic_eta", "coating_sigma", "coating_exit", "rough_coating_t",
-         "rft_diffuse", "film_centre"]
+         "rft_diffuse", "film_centre", "rr_start_weight"]
 
 
 def set_slip(name):

@@ -315,6 +315,36 @@ def test_cbox_image_vs_oracle(ctx):
     assert abs(lum(img).mean() - lum(ref).mean()) / lum(ref).mean() < 0.01
 
 
+@pytest.mark.parametrize("integrator", ["pathMIS", "pathEMS", "pathMATS"])
+def test_rr_start_depth_vs_oracle(ctx, integrator):
+    """Russian roulette from path depth 3 on (rr_start_depth; 0, the
+    reference's schedule, is the default): GPU vs FP32 oracle at equal
+    streams, and the same mean as the default schedule."""
+    sc = scenes.cbox(width=48, height=40, sphere_subdiv=2)
+    sc.integrator = integrator
+    base, _ = ctx.scene(sc).render(16, seed=41)
+    sc.rr_start_depth = 3
+    img, st = ctx.scene(sc).render(16, seed=41)
+    ref, _, _ = O.render(sc, req_seed=41, spp=(0, 16))
+    rm = rel_mse(img, ref)
+    print(integrator, "relMSE vs oracle (rr from depth 3):", rm, "mean", lum(img).mean(), "default", lum(base).mean())
+    assert rm < 1e-3
+    assert not np.array_equal(img, base)
+    assert abs(lum(img).mean() - lum(base).mean()) / lum(base).mean() < 0.03
+
+
+def test_furnace_gpu_rr_start_depth(ctx):
+    """The furnace (1 / (1 - a), a = 0.8) with Russian roulette from depth 10."""
+    sc = _furnace(0.8)
+    sc.max_depth = 1000
+    sc.rr_start_depth = 10
+    vals = ctx.scene(sc).pixel_samples(0, 0, 0, 100000)
+    L = lum(vals[:, :3])
+    se = L.std() / np.sqrt(len(L))
+    print(f"furnace rr from depth 10: {L.mean():.5f} +- {se:.5f}")
+    assert abs(L.mean() - 5.0) < 4 * se + 1e-3
+
+
 def test_slot_recycling_and_chunking_do_not_change_bits():
     """Path-slot recycling (tiny wavefront) and sample chunking (tiny sample
     capacity, several segments) leave every film bit unchanged."""

@@ -111,6 +111,29 @@ def test_loader_errors(tmp_path, xml, needle):
     assert r.returncode != 0 and needle in r.stderr, r.stderr
 
 
+_OPT_SCENE = """<scene>
+  <integrator type="gpuPathMIS"><integer name="depth" value="100"/>{opts}</integrator>
+  <camera type="perspective"><integer name="width" value="8"/><integer name="height" value="6"/></camera>
+  <sampler type="independent"><integer name="sampleCount" value="4"/></sampler>
+</scene>"""
+
+
+def test_loader_reads_integrator_options(tmp_path):
+    """queueSize and rrStartDepth, integrator properties the reference does
+    not have, reach the GPU integrator (its description in --dump); both
+    are optional and refuse negative values."""
+    p = tmp_path / "opts.xml"
+    p.write_text(_OPT_SCENE.format(opts=""))
+    assert dump(str(p))["integrator"] == "GPUPathIntegrator[pathMIS, depth = 100]"
+    p.write_text(_OPT_SCENE.format(opts='<integer name="rrStartDepth" value="3"/>'
+                                        '<integer name="queueSize" value="1048576"/>'))
+    assert dump(str(p))["integrator"] == "GPUPathIntegrator[pathMIS, depth = 100, rrStartDepth = 3, queueSize = 1048576]"
+    for opt in ("rrStartDepth", "queueSize"):
+        p.write_text(_OPT_SCENE.format(opts=f'<integer name="{opt}" value="-1"/>'))
+        r = run(str(p), "--dump", check=False)
+        assert r.returncode != 0 and f"{opt} must be >= 0" in r.stderr, r.stderr
+
+
 def test_loader_requires_camera(tmp_path):
     p = tmp_path / "nocam.xml"
     p.write_text('<scene><integrator type="pathMIS"><integer name="depth" value="3"/></integrator></scene>')

@@ -395,6 +395,33 @@ def test_oracle_furnace(albedo, expect):
     assert abs(L.mean() - expect) < 4 * L.std() / np.sqrt(len(L)) + 1e-3
 
 
+@pytest.mark.parametrize("rr_start", [3, 10])
+@pytest.mark.parametrize("integrator", ["pathMIS", "pathEMS", "pathMATS"])
+def test_oracle_furnace_rr_start_depth(integrator, rr_start):
+    """Russian roulette from depth rr_start on (an option the reference does
+    not have; 0 is its schedule) keeps the estimator unbiased: the furnace
+    still gives 1 / (1 - a) with a = 0.8 (scaling the weight below rr_start
+    as well would multiply the deep terms by up to 0.95^-rr_start)."""
+    sc = _furnace(0.8)
+    sc.integrator = integrator
+    sc.max_depth = 1000
+    sc.rr_start_depth = rr_start
+    vals = O.pixel_samples(sc, 0, 0, 0, 40000)
+    L = lum(vals[:, :3])
+    assert abs(L.mean() - 5.0) < 4 * L.std() / np.sqrt(len(L)) + 1e-3, L.mean()
+
+
+def test_oracle_rr_start_zero_is_the_reference_schedule():
+    """rr_start_depth = 0 draws and weighs exactly as before the option existed."""
+    sc = _furnace(0.5)
+    a = O.pixel_samples(sc, 0, 0, 0, 2000)
+    sc.rr_start_depth = 0
+    b = O.pixel_samples(sc, 0, 0, 0, 2000)
+    sc.rr_start_depth = 2
+    c = O.pixel_samples(sc, 0, 0, 0, 2000)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+
+
 @pytest.mark.parametrize("integrator", ["pathMIS", "pathEMS", "pathMATS"])
 def test_oracle_direct_lighting_references(integrator):
     """scene/pa5/tests/test-direct.xml: the five scenes of each integrator against the file's reference values

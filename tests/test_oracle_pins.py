@@ -514,3 +514,16 @@ def test_film_splat_of_one_sample():
 def test_film_splat_catches_slip():
     with planted("film_centre"):
         assert film_splat_error() > 0.1
+
+
+def test_rr_start_furnace_catches_slip():
+    """The furnace check of the roulette start depth (tests/test_oracle.py)
+    fails when the weight 1 / 0.95 is applied below the start depth too."""
+    from tests.test_oracle import _furnace, lum
+    sc = _furnace(0.8)
+    sc.max_depth = 1000
+    sc.rr_start_depth = 10
+    with planted("rr_start_weight"):
+        vals = O.pixel_samples(sc, 0, 0, 0, 40000)
+    L = lum(vals[:, :3])
+    assert abs(L.mean() - 5.0) > 4 * L.std() / np.sqrt(len(L)) + 1e-3, L.mean()
