@@ -311,17 +311,25 @@ std::vector<float> ReadImage(const std::string &path, int &width, int &height);
 
 // Student's t-test harness of the reference (src/core/StudentTTest.cpp):
 // registered as "ttest"; Activate() runs the tests and logs the outcome.
-class StudentTTest : public Object {
+// A test description (<test type="...">): runs at load time (Activate) and
+// keeps its tally for the driver's exit status.
+class TestObject : public Object {
+ public:
+  EClassType GetClassType() const override { return EClassType::ETest; }
+  virtual const char *TestKind() const = 0;
+  int passed = 0, total = 0;
+};
+
+class StudentTTest : public TestObject {
  public:
   explicit StudentTTest(const PropertyList &props);
   ~StudentTTest() override;
-  EClassType GetClassType() const override { return EClassType::ETest; }
+  const char *TestKind() const override { return "ttest"; }
   void AddChild(Object *child, const std::string &name) override;
   void Activate() override;
   std::string ToString() const override;
   const std::vector<Scene *> &GetScenes() const { return scenes_; }
   const std::vector<float> &GetReferences() const { return references_; }
-  int passed = 0, total = 0;
 
  private:
   float significance_;
@@ -340,5 +348,13 @@ bool DryRun();
 // the reference.  `msg` receives a one-line report.
 bool StudentsTTest(double mean, double variance, double reference, long n, double significance, int n_tests,
                    std::string &msg);
+
+// Chi-square distribution function with `dof` degrees of freedom at x.
+double Chi2Cdf(double x, int dof);
+// Pearson's chi^2 test of observed against expected cell frequencies, cells
+// below min_exp pooled, at `significance` Sidak-corrected for n_tests
+// (ext/hypothesis/hypothesis.h:140-240 semantics).
+bool PooledChi2Test(int n_cells, const double *observed, const double *expected, long sample_count, double min_exp,
+                    double significance, int n_tests, std::string &msg);
 
 }  // namespace hikari

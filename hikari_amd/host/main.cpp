@@ -6,13 +6,19 @@
 //   hikari_render FILE.xml [-o OUT.exr|OUT.pfm] [--spp N] [--dump] [--dump-mesh PREFIX]
 //   hikari_render --convert IN.exr|IN.pfm OUT.exr|OUT.pfm
 //   hikari_render --t-test MEAN VARIANCE REFERENCE N LEVEL N_TESTS
+//   hikari_render --chi2-cdf X DOF
+//   hikari_render --chi2-table FILE
 //
 // --dump prints the parsed description as JSON and touches no device;
 // --dump-mesh also writes every mesh's world-space vertex positions and
 // normals as raw FP32 (PREFIX.<mesh>.v.f32, PREFIX.<mesh>.n.f32);
 // --convert rewrites an image (EXR written as uncompressed FLOAT RGB);
 // --t-test runs the Student t-test of the test harness on given statistics
-// and prints its verdict (exit status 0 accepted, 1 rejected).
+// and prints its verdict (exit status 0 accepted, 1 rejected); --chi2-cdf
+// prints the chi-square distribution function of the chi2test harness;
+// --chi2-table runs its pooled chi-square test on a text file holding
+// "cells sample_count min_exp level n_tests", then the observed and the
+// expected frequencies (exit status 0 accepted, 1 rejected).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -100,6 +106,34 @@ int main(int argc, char **argv) {
       return 1;
     }
   }
+  if (argc == 4 && std::strcmp(argv[1], "--chi2-cdf") == 0) {
+    printf("%.17g\n", Chi2Cdf(std::atof(argv[2]), std::atoi(argv[3])));
+    return 0;
+  }
+  if (argc == 3 && std::strcmp(argv[1], "--chi2-table") == 0) {
+    FILE *fp = std::fopen(argv[2], "r");
+    int n = 0, n_tests = 1;
+    long count = 0;
+    double min_exp = 0, level = 0;
+    if (!fp || std::fscanf(fp, "%d %ld %lf %lf %d", &n, &count, &min_exp, &level, &n_tests) != 5 || n <= 0) {
+      fprintf(stderr, "error: bad table %s\n", argv[2]);
+      if (fp) std::fclose(fp);
+      return 2;
+    }
+    std::vector<double> obs(static_cast<size_t>(n)), expv(static_cast<size_t>(n));
+    bool ok = true;
+    for (int i = 0; i < n && ok; i++) ok = std::fscanf(fp, "%lf", &obs[size_t(i)]) == 1;
+    for (int i = 0; i < n && ok; i++) ok = std::fscanf(fp, "%lf", &expv[size_t(i)]) == 1;
+    std::fclose(fp);
+    if (!ok) {
+      fprintf(stderr, "error: short table %s\n", argv[2]);
+      return 2;
+    }
+    std::string msg;
+    bool acc = PooledChi2Test(n, obs.data(), expv.data(), count, min_exp, level, n_tests, msg);
+    printf("%s\n", msg.c_str());
+    return acc ? 0 : 1;
+  }
   if (argc == 8 && std::strcmp(argv[1], "--t-test") == 0) {
     std::string msg;
     bool ok = StudentsTTest(std::atof(argv[2]), std::atof(argv[3]), std::atof(argv[4]), std::atol(argv[5]),
@@ -133,6 +167,14 @@ int main(int argc, char **argv) {
   try {
     std::unique_ptr<Object> root(LoadFromXML(file));
     if (root->GetClassType() == Object::EClassType::ETest) {
+      auto *base = static_cast<TestObject *>(root.get());
+      if (std::strcmp(base->TestKind(), "ttest") != 0) {
+        if (dump) {
+          printf("{\"test\": \"%s\", \"description\": \"%s\"}\n", base->TestKind(), base->ToString().c_str());
+          return 0;
+        }
+        return base->passed == base->total ? 0 : 1;
+      }
       auto *t = static_cast<StudentTTest *>(root.get());
       if (dump) {
         printf("{\"test\": \"ttest\", \"references\": ");

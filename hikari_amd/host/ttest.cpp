@@ -77,7 +77,7 @@ bool StudentsTTest(double mean, double variance, double reference, long n, doubl
   return ok;
 }
 
-StudentTTest::StudentTTest(const PropertyList &p) {
+StudentTTest::StudentTTest(const PropertyList &p) : TestObject() {
   significance_ = p.GetFloat("significanceLevel", 0.01f);
   std::string angles = p.GetString("angles", ""), refs = p.GetString("references", "");
   auto parse = [](const std::string &s, std::vector<float> &out) {

@@ -12,6 +12,8 @@ tests/golden/envmap from the reference's scene files:
     (32 x 32 blocks) of the reference's renders envmap<n>.exr (256 spp);
   * pa4tests/: scene/pa4/tests test-mesh.xml and test-mesh-furnace.xml
     (whitted t-tests with their reference values) and their meshes/;
+  * chi2test-microfacet.xml: scene/pa5/tests' chi-square test of the
+    microfacet BSDF's sampling (three parameter sets);
   * cbox4/ and motto/: scene/pa4 cbox-distributed and motto-diffuse /
     motto-dielectric (whitted over area lights, diffuse and dielectric):
     XML and meshes/, and <name>-blocks<b>.npy, the b x b-pixel block means of
@@ -76,6 +78,7 @@ def main(ref_root):
         np.save(os.path.join(out, f"envmap{n}-blocks16.npy"), blk)
         print("envmap", n, blk.shape, blk.mean(axis=(0, 1)))
     copy(src, out, ["sphere.obj", "plane.obj"])
+    copy(os.path.join(ref_root, "scene", "pa5", "tests"), HERE, ["chi2test-microfacet.xml"])
     src = os.path.join(ref_root, "scene", "pa4")
     copy(os.path.join(src, "tests"), os.path.join(HERE, "pa4tests"), ["test-mesh.xml", "test-mesh-furnace.xml"])
     for sub, out_name, names, b in (("cbox", "cbox4", ["cbox-distributed"], 20),

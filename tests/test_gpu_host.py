@@ -30,12 +30,13 @@ def run(*args):
 
 @pytest.mark.parametrize("name,total", [("test-direct.xml", 15), ("test-furnace.xml", 6),
                                         ("ttest-microfacet.xml", 5), ("pa4tests/test-mesh.xml", 5),
-                                        ("pa4tests/test-mesh-furnace.xml", 2)])
+                                        ("pa4tests/test-mesh-furnace.xml", 2), ("chi2test-microfacet.xml", 15)])
 def test_reference_ttest_files(name, total):
     """scene/pa5/tests/*.xml and scene/pa4/tests/*.xml through the C++ harness
-    (src/core/StudentTTest.cpp semantics): every scene (pathMIS, pathEMS,
-    pathMATS, whitted) and every incidence angle of the microfacet BSDF test
-    passes."""
+    (src/core/StudentTTest.cpp and src/core/Chi2Test.cpp semantics): every
+    scene (pathMIS, pathEMS, pathMATS, whitted), every incidence angle of the
+    microfacet BSDF t-test and all five incident directions of each of the
+    chi-square test's three microfacet BSDFs pass."""
     r = run(os.path.join(GOLDEN, name))
     assert r.returncode == 0 and f"Passed {total}/{total} tests." in r.stdout
 

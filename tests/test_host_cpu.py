@@ -233,3 +233,90 @@ def test_student_t_matches_scipy(case):
     assert float(got["p-value"]) == pytest.approx(p, rel=1e-4, abs=1e-12)
     assert float(got["level"]) == pytest.approx(alpha, rel=1e-5)
     assert r.returncode == (1 if p < alpha else 0)
+
+
+@pytest.mark.parametrize("x,dof", [(0.5, 1), (3.5, 4), (180.0, 199), (250.0, 199), (1e-3, 2), (40.0, 10)])
+def test_chi2_cdf_matches_scipy(x, dof):
+    """The chi-square distribution function of the chi2test harness
+    (hikari_render --chi2-cdf) against scipy's."""
+    from scipy import stats
+    got = float(run("--chi2-cdf", repr(x), str(dof)).stdout)
+    assert got == pytest.approx(stats.chi2.cdf(x, dof), rel=1e-10, abs=1e-14)
+
+
+def _chi2_table(tmp_path, obs, exp, count, min_exp=5, level=0.01, tests=1):
+    p = tmp_path / "table.txt"
+    p.write_text(f"{len(obs)} {count} {min_exp} {level} {tests}\n" + " ".join(map(repr, map(float, obs))) + "\n"
+                 + " ".join(map(repr, map(float, exp))) + "\n")
+    r = run("--chi2-table", str(p), check=False)
+    assert r.returncode in (0, 1), r.stderr
+    return r
+
+
+def _chi2_fields(out):
+    import re
+    m = re.search(r"chi\^2 = ([^,]+), (\d+) d\.o\.f\. \((\d+) cells pooled\), p-value = ([^,]+),", out)
+    return float(m.group(1)), int(m.group(2)), int(m.group(3)), float(m.group(4))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_pooled_chi2_test_matches_scipy(tmp_path, seed):
+    """The pooled chi-square test of the chi2test harness (hikari_render
+    --chi2-table, ext/hypothesis/hypothesis.h:140-240): without low cells it
+    is Pearson's test on k - 1 degrees of freedom (scipy.stats.chisquare);
+    with cells below the minimum expected frequency, those are merged, in
+    increasing expected frequency, until the merged cell reaches it."""
+    from scipy import stats
+    rng = np.random.default_rng(seed)
+    n = 5000
+    # no pooling: 40 cells, all expected frequencies >= 5
+    p = rng.uniform(0.5, 2.0, 40)
+    p /= p.sum()
+    obs = rng.multinomial(n, p)
+    r = _chi2_table(tmp_path, obs, p * n, n)
+    chsq, dof, pooled, pv = _chi2_fields(r.stdout)
+    ref = stats.chisquare(obs, p * n)
+    assert pooled == 0 and dof == 39
+    assert chsq == pytest.approx(ref.statistic, rel=1e-9) and pv == pytest.approx(ref.pvalue, rel=1e-6, abs=1e-12)
+    # pooling: 12 cells of tiny expected frequency among 30
+    q = np.concatenate([rng.uniform(1e-5, 4e-4, 12), rng.uniform(0.5, 2.0, 18)])
+    q /= q.sum()
+    obs = rng.multinomial(n, q)
+    e = q * n
+    order = np.argsort(e, kind="stable")
+    k = 0
+    pool_e = 0.0
+    while k < len(order) and (e[order[k]] < 5 or 0 < pool_e < 5):
+        pool_e += e[order[k]]
+        k += 1
+    keep = order[k:]
+    merged_o = np.concatenate([obs[keep], [obs[order[:k]].sum()]])
+    merged_e = np.concatenate([e[keep], [e[order[:k]].sum()]])
+    ref = stats.chisquare(merged_o, merged_e)
+    r = _chi2_table(tmp_path, obs, e, n)
+    chsq, dof, pooled, pv = _chi2_fields(r.stdout)
+    assert pooled == k and dof == len(merged_o) - 1
+    assert chsq == pytest.approx(ref.statistic, rel=1e-9) and pv == pytest.approx(ref.pvalue, rel=1e-6, abs=1e-12)
+
+
+def test_pooled_chi2_test_rejections(tmp_path):
+    """Samples in a cell of expected frequency 0 beyond 1e-5 of the sample
+    count reject (rounding may leave fewer); a clearly different
+    distribution rejects; too few degrees of freedom reject."""
+    n = 200000
+    exp = [50000.0, 50000.0, 50000.0, 50000.0, 0.0]
+    assert _chi2_table(tmp_path, [49990, 50010, 50000, 49997, 3], exp, n).returncode == 1
+    assert _chi2_table(tmp_path, [49990, 50010, 50000, 49998, 2], exp, n).returncode == 0
+    assert _chi2_table(tmp_path, [52000, 48000, 50000, 50000, 0], exp, n).returncode == 1
+    r = _chi2_table(tmp_path, [10], [10.0], 10)
+    assert r.returncode == 1 and "degrees of freedom" in r.stdout
+
+
+def test_chi2test_file_loads():
+    """scene/pa5/tests/chi2test-microfacet.xml: the chi2test type with the
+    reference's defaults (resolution 10 x 20, 5 incident directions per BSDF,
+    5000 samples per cell)."""
+    d = json.loads(run(os.path.join(GOLDEN, "chi2test-microfacet.xml"), "--dump").stdout)
+    assert d["test"] == "chi2test"
+    assert "resolution = 10 x 20" in d["description"] and "sampleCount = 1000000" in d["description"]
+    assert "testCount = 5" in d["description"] and "minExpFrequency = 5" in d["description"]
