@@ -293,6 +293,7 @@ def bench_serve(a):
     from hikari_amd import scenes, workloads
     box, sphere, field = serve_scenes(scenes.displaced_sphere(alpha=a.alpha))
     ctx = hk.Context(0)
+    ctx.set_option("max_paths", a.max_paths or (1 << 26))  # plain renders at the server lanes' wavefront
     gss = [ctx.scene(box), ctx.scene(sphere), ctx.scene(field)]
     for g in gss:
         g.render(1)
@@ -544,6 +545,9 @@ def main():
     if a.workload == "c1" and not a.no_batch:
         out["batch"] = batch_leg(gs, a.requests or 64, a.schedule, a.detail)
     if a.workload == "c1" and not a.no_serve:
+        # plain renders of the trace and the server's lanes at the same
+        # wavefront (64 M paths, the server lanes' default)
+        ctx.set_option("max_paths", 1 << 26)
         box, sphere, field = serve_scenes(scene)
         gss = [ctx.scene(box), gs, ctx.scene(field)]
         for g in gss:
@@ -555,6 +559,7 @@ def main():
         out["serve_saturated"] = serve_leg(ctx, gss, trace, a.budget_gb, service=out["serve"]["service_s"])
         for k in ("serve", "serve_saturated"):
             out[k].pop("service_s")
+        ctx.set_option("max_paths", 0)
         out["c0"] = c0_leg(gss[0])
         log(f"c0: {out['c0']['value']:.3e} path samples/s, {out['c0']['ms_per_render']:.1f} ms per render")
     if not a.no_cpu_baseline:

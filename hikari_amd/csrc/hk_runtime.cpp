@@ -437,6 +437,7 @@ int hk_set_option(hk_ctx *ctx, const char *name, int64_t value) {
   if (n == "max_paths") {
     if (value < 0 || value > (int64_t(1) << 28)) throw HkError(HK_ERR_INVALID, "max_paths out of range");
     ctx->max_paths = value == 0 ? kDefaultMaxPaths : uint32_t(value);
+    ctx->max_paths_set = value != 0;
   } else if (n == "max_samples") {
     if (value < 0 || value > (int64_t(1) << 31)) throw HkError(HK_ERR_INVALID, "max_samples out of range");
     ctx->max_samples = value == 0 ? kDefaultMaxSamples : uint32_t(value);

@@ -379,7 +379,7 @@ int hk_autotune(hk_scene *s, const hk_request *req, uint64_t calib_samples, hk_t
       }
     };
     uint64_t total = npix * cal.spp;
-    for (uint32_t p : {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26}) {
+    for (uint32_t p : {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27, 1u << 28}) {
       if (p > total && p != (1u << 22)) break;  // a wavefront larger than the work is never filled
       Setting t = best;
       t.paths = p;

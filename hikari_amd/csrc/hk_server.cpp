@@ -2,9 +2,10 @@
 //
 // Every resident scene has two lanes, each an Engine with its own workspace
 // and HIP stream (default priority):
-//   * the bulk lane: the context's wavefront (max_paths), a sample buffer of
-//     4 x max_samples holding up to four chunks of max_samples;
-//   * the express lane: a wavefront of max_paths / 16 and a buffer of
+//   * the bulk lane: a wavefront of max_paths (64 M unless the option is
+//     set), a sample buffer of 4 x max_samples holding up to four chunks of
+//     max_samples;
+//   * the express lane: a wavefront of 1/16 of that and a buffer of
 //     max_samples / 8, for requests of at most half that many samples,
 //     fewest samples left first (context option "express" = 0 turns it off:
 //     every request then waits its turn on the bulk lane).
@@ -369,11 +370,14 @@ int hk_server_add_scene(hk_server *srv, hk_scene *scene, int32_t *scene_id) {
   std::unique_ptr<ServerScene> sc(new ServerScene());
   sc->scene = scene;
   uint64_t bulk_samples = std::min<uint64_t>(uint64_t(ctx->max_samples) * 4, 0x7fffffffu);
-  sc->lane[kBulk] = make_lane(ctx, scene, ctx->max_paths, uint32_t(bulk_samples), ctx->max_samples, false);
+  // a lane's wavefront: max_paths when set, else kDefaultServerLanePaths (the
+  // render default would hold 4 x the HBM per lane and scene)
+  uint32_t lane_paths = ctx->max_paths_set ? ctx->max_paths : kDefaultServerLanePaths;
+  sc->lane[kBulk] = make_lane(ctx, scene, lane_paths, uint32_t(bulk_samples), ctx->max_samples, false);
   // express lane (option "express" = 0: a placeholder that takes nothing, so
   // every request runs on the bulk lane in arrival order)
   uint32_t ex_samples = ctx->express ? std::max<uint32_t>(1, ctx->max_samples / 8) : 1u;
-  uint32_t ex_paths = ctx->express ? std::max<uint32_t>(1, ctx->max_paths / 16) : 1u;
+  uint32_t ex_paths = ctx->express ? std::max<uint32_t>(1, lane_paths / 16) : 1u;
   sc->lane[kExpress] = make_lane(ctx, scene, ex_paths, ex_samples, ex_samples, true);
   sc->lane[kExpress]->max_request = ctx->express ? ex_samples / 2 : 0;
   uint64_t ws_bytes = sc->lane[kBulk]->ws.device_bytes() + sc->lane[kExpress]->ws.device_bytes();
