@@ -293,7 +293,7 @@ def bench_serve(a):
     from hikari_amd import scenes, workloads
     box, sphere, field = serve_scenes(scenes.displaced_sphere(alpha=a.alpha))
     ctx = hk.Context(0)
-    ctx.set_option("max_paths", a.max_paths or (1 << 26))  # plain renders at the server lanes' wavefront
+    ctx.set_option("max_paths", a.max_paths or (1 << 26))  # plain renders at the server's bulk wavefront
     gss = [ctx.scene(box), ctx.scene(sphere), ctx.scene(field)]
     for g in gss:
         g.render(1)
@@ -545,8 +545,8 @@ def main():
     if a.workload == "c1" and not a.no_batch:
         out["batch"] = batch_leg(gs, a.requests or 64, a.schedule, a.detail)
     if a.workload == "c1" and not a.no_serve:
-        # plain renders of the trace and the server's lanes at the same
-        # wavefront (64 M paths, the server lanes' default)
+        # plain renders of the trace at the server's bulk wavefront (64 M
+        # paths, its default; set here, it is also the server's)
         ctx.set_option("max_paths", 1 << 26)
         box, sphere, field = serve_scenes(scene)
         gss = [ctx.scene(box), gs, ctx.scene(field)]

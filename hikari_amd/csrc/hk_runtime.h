@@ -44,11 +44,13 @@ struct DevBvh {
 // floats on the host.  Synchronizes `st`.
 void build_bvh4_device(hipStream_t st, const float *tri9, uint32_t n, DevBvh &out);
 
-// path slots of the wavefront (~250-350 B of HBM each): renders and batches
-// default to 256 M (a larger wavefront spreads each launch's straggler tail
-// over more work); each server lane to 64 M unless max_paths is set
+// path slots of the wavefront (~250-350 B of HBM each): 256 M by default for
+// renders and batches (a larger wavefront spreads each launch's straggler
+// tail over more work); a server's bulk wavefront 64 M (its persistent
+// launches then leave the express lanes room sooner) and each express lane
+// 4 M, unless max_paths is set (bulk max_paths, express 1/16 of it)
 constexpr uint32_t kDefaultMaxPaths = 1u << 28;
-constexpr uint32_t kDefaultServerLanePaths = 1u << 26;
+constexpr uint32_t kDefaultServerBulkPaths = 1u << 26, kDefaultExpressPaths = 1u << 22;
 constexpr uint32_t kDefaultMaxSamples = 1u << 28;   // samples per segment (16 B each in HBM)
 // persistent traversal waves: idle lanes that trigger a refill, and queue
 // entries (in 64s) reserved per atomic on the fetch counter
@@ -102,7 +104,7 @@ struct hk_ctx {
   int n_cu = 0;
   std::string arch;
   uint32_t max_paths = hk::kDefaultMaxPaths;
-  bool max_paths_set = false;  // max_paths given by hk_set_option (servers then use it per lane)
+  bool max_paths_set = false;  // max_paths given by hk_set_option (servers then use it, express 1/16)
   uint32_t max_samples = hk::kDefaultMaxSamples;
   uint32_t grid = 0;  // persistent grid for queue kernels (0: automatic)
   int bvh_builder = 0;  // 0 host binned SAH, 1 device LBVH

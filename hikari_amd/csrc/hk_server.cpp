@@ -1,11 +1,15 @@
 // hk_server.cpp -- request loop with continuous batching over resident scenes.
 //
-// Every resident scene has two lanes, each an Engine with its own workspace
-// and HIP stream (default priority):
-//   * the bulk lane: a wavefront of max_paths (64 M unless the option is
-//     set), a sample buffer of 4 x max_samples holding up to four chunks of
-//     max_samples;
-//   * the express lane: a wavefront of 1/16 of that and a buffer of
+// Every resident scene has two lanes, each an Engine with its own HIP stream
+// (default priority):
+//   * the bulk lane: a wavefront of max_paths (64 M when the option is not
+//     set) and a sample buffer of 4 x max_samples holding up
+//     to four chunks of max_samples.  Bulk lanes never run at the same time
+//     (below), so the scenes' bulk engines share one workspace, sized for the
+//     largest of them when the scenes are added: a resident scene adds its
+//     express lane, not a second wavefront;
+//   * the express lane (its own workspace): a wavefront of max_paths / 16 (4 M
+//     when the option is not set) and a buffer of
 //     max_samples / 8, for requests of at most half that many samples,
 //     fewest samples left first (context option "express" = 0 turns it off:
 //     every request then waits its turn on the bulk lane).
@@ -55,7 +59,7 @@ enum { kBulk = 0, kExpress = 1 };
 struct Lane {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  Workspace ws;
+  Workspace ws;                // the express lane's (bulk lanes use the server's)
   std::unique_ptr<Engine> eng;
   uint32_t chunk_cap = 0;      // samples per chunk at most (prepare_request)
   uint64_t max_request = 0;    // express: largest request (samples) it takes
@@ -89,6 +93,7 @@ struct Ticket {
 struct hk_server {
   hk_ctx *ctx = nullptr;
   uint64_t budget = 0, fixed_bytes = 0, film_bytes = 0;
+  Workspace bulk_ws;  // shared by the bulk lanes (declared before `scenes`: freed after their engines)
   std::vector<std::unique_ptr<ServerScene>> scenes;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -332,13 +337,13 @@ static void worker_main(hk_server *srv) {
   }
 
 static std::unique_ptr<Lane> make_lane(hk_ctx *ctx, hk_scene *scene, uint32_t paths, uint32_t samples,
-                                       uint32_t chunk_cap, bool shortest_first) {
+                                       uint32_t chunk_cap, bool shortest_first, Workspace *shared = nullptr) {
   std::unique_ptr<Lane> l(new Lane());
   hip_check(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipEventCreate(&l->ev0), "hipEventCreate");
   hip_check(hipEventCreate(&l->ev1), "hipEventCreate");
   l->chunk_cap = chunk_cap;
-  l->eng.reset(new Engine(scene, l->ws, l->stream, paths, samples, chunk_cap, false, l->ev0, l->ev1));
+  l->eng.reset(new Engine(scene, shared ? *shared : l->ws, l->stream, paths, samples, chunk_cap, false, l->ev0, l->ev1));
   l->eng->set_shortest_first(shortest_first);
   hip_check(hipStreamSynchronize(l->stream), "workspace");
   return l;
@@ -370,20 +375,34 @@ int hk_server_add_scene(hk_server *srv, hk_scene *scene, int32_t *scene_id) {
   std::unique_ptr<ServerScene> sc(new ServerScene());
   sc->scene = scene;
   uint64_t bulk_samples = std::min<uint64_t>(uint64_t(ctx->max_samples) * 4, 0x7fffffffu);
-  // a lane's wavefront: max_paths when set, else kDefaultServerLanePaths (the
-  // render default would hold 4 x the HBM per lane and scene)
-  uint32_t lane_paths = ctx->max_paths_set ? ctx->max_paths : kDefaultServerLanePaths;
-  sc->lane[kBulk] = make_lane(ctx, scene, lane_paths, uint32_t(bulk_samples), ctx->max_samples, false);
-  // express lane (option "express" = 0: a placeholder that takes nothing, so
-  // every request runs on the bulk lane in arrival order)
-  uint32_t ex_samples = ctx->express ? std::max<uint32_t>(1, ctx->max_samples / 8) : 1u;
-  uint32_t ex_paths = ctx->express ? std::max<uint32_t>(1, lane_paths / 16) : 1u;
-  sc->lane[kExpress] = make_lane(ctx, scene, ex_paths, ex_samples, ex_samples, true);
+  // the bulk lane joins the shared workspace (which grows to this scene's
+  // needs: its emitters' shadow slots, the wavefront); the growth counts
+  // against the budget even when the scene is refused, since it stays held
+  uint64_t shared_before = srv->bulk_ws.device_bytes();
+  uint32_t ex_paths = 1u, ex_samples = 1u;
+  try {
+    uint32_t bulk_paths = ctx->max_paths_set ? ctx->max_paths : kDefaultServerBulkPaths;
+    sc->lane[kBulk] = make_lane(ctx, scene, bulk_paths, uint32_t(bulk_samples), ctx->max_samples, false,
+                                &srv->bulk_ws);
+    // express lane (option "express" = 0: a placeholder that takes nothing,
+    // so every request runs on the bulk lane in arrival order)
+    if (ctx->express) {
+      ex_samples = std::max<uint32_t>(1, ctx->max_samples / 8);
+      ex_paths = ctx->max_paths_set ? std::max<uint32_t>(1, ctx->max_paths / 16) : kDefaultExpressPaths;
+    }
+    sc->lane[kExpress] = make_lane(ctx, scene, ex_paths, ex_samples, ex_samples, true);
+  } catch (...) {
+    srv->fixed_bytes += srv->bulk_ws.device_bytes() - shared_before;
+    throw;
+  }
   sc->lane[kExpress]->max_request = ctx->express ? ex_samples / 2 : 0;
-  uint64_t ws_bytes = sc->lane[kBulk]->ws.device_bytes() + sc->lane[kExpress]->ws.device_bytes();
-  if (srv->budget && srv->fixed_bytes + ws_bytes > srv->budget)
+  uint64_t grown = srv->bulk_ws.device_bytes() - shared_before;
+  uint64_t ws_bytes = grown + sc->lane[kExpress]->ws.device_bytes();
+  if (srv->budget && srv->fixed_bytes + ws_bytes > srv->budget) {
+    srv->fixed_bytes += grown;
     throw HkError(HK_ERR_BUDGET, "scene workspaces (%llu B) exceed the HBM budget (%llu B)",
-                  (unsigned long long)(srv->fixed_bytes + ws_bytes), (unsigned long long)srv->budget);
+                  (unsigned long long)(srv->fixed_bytes + ws_bytes - grown), (unsigned long long)srv->budget);
+  }
   srv->fixed_bytes += ws_bytes;
   if (scene_id) *scene_id = int32_t(srv->scenes.size());
   srv->scenes.push_back(std::move(sc));

@@ -249,9 +249,11 @@ int hk_render_batch(hk_scene *scene, const hk_request *reqs, uint32_t n, float *
  * A request loop with continuous batching (no counterpart in the reference,
  * whose Render() handles one scene file per process).  Each resident scene
  * has two lanes, each a wavefront on its own stream: a bulk lane (64M paths,
- * or the context's max_paths when that option is set; a sample buffer of
- * 4 x max_samples; first come first served) and an express lane (1/16 of the
- * bulk lane's paths, max_samples / 8
+ * or max_paths when that option is set; a sample buffer of 4 x max_samples;
+ * first come first
+ * served; the scenes' bulk lanes, which never run at once, share one
+ * workspace) and an express lane (4M paths, or max_paths / 16 when that
+ * option is set; max_samples / 8
  * samples) for requests of at most max_samples / 16 path samples, fewest
  * samples left first.  A worker thread advances every lane as soon as its
  * previous bounce has finished on the device.  Bulk requests run one at a
@@ -261,7 +263,8 @@ int hk_render_batch(hk_scene *scene, const hk_request *reqs, uint32_t n, float *
  * others run is admitted, first come first served, once its film fits the
  * HBM budget; an express request's paths join its lane's next bounce.
  * hbm_budget bounds the server's path state and sample buffers
- * (fixed per scene, both lanes) plus the films of admitted requests; 0 =
+ * (the shared bulk workspace and each scene's express lane) plus the films
+ * of admitted requests; 0 =
  * unbounded.  Each result equals hk_render of the same request on a context
  * with the same options, bit for bit.  Scenes must stay alive until
  * hk_server_destroy. */
@@ -304,10 +307,10 @@ int hk_server_get_stats(hk_server *server, hk_server_stats *stats);
 int hk_server_destroy(hk_server *server);
 
 /* Tuning knobs (value 0 restores the default):
- *   "max_paths"   paths in flight (default 256M for hk_render and
- *                 hk_render_batch, 64M per server lane; a value set here
- *                 applies to both); at every bounce finished paths are
- *                 replaced by the next samples
+ *   "max_paths"   paths in flight (default 256M for renders and batches; a
+ *                 server's bulk wavefront 64M and express lanes 4M each,
+ *                 or this value and 1/16 of it when set); at every bounce
+ *                 finished paths are replaced by the next samples
  *   "max_samples" samples per segment (default 256M, 16 B of HBM each): a
  *                 request's samples are cut into chunks of
  *                 floor(max_samples / pixels) samples per pixel, resolved

@@ -484,6 +484,39 @@ def test_server_hbm_budget_admission():
     c.close()
 
 
+def test_server_scenes_share_the_bulk_workspace():
+    """Bulk lanes never run at once, so a server's scenes share one bulk
+    workspace: two scenes hold less HBM than two one-scene servers, the
+    sharing scene with more emitters grows it, and requests alternating
+    between the scenes on their bulk lanes equal their renders alone."""
+    box = scenes.cbox(width=32, height=24, sphere_subdiv=2)
+    field = scenes.heightfield(width=28, height=20, n=24)
+    c = hk.Context(0)
+    c.set_option("max_paths", 3000)
+    c.set_option("express", 0)
+    g_box, g_field = c.scene(box), c.scene(field)
+    fixed = []
+    for gl in ([g_box], [g_field], [g_box, g_field]):
+        srv = hk.Server(c)
+        for g in gl:
+            srv.add_scene(g)
+        fixed.append(srv.stats()["fixed_bytes"])
+        srv.close()
+    print("fixed bytes: box", fixed[0], "field", fixed[1], "both", fixed[2])
+    assert max(fixed[0], fixed[1]) <= fixed[2] < fixed[0] + fixed[1]
+    srv = hk.Server(c)
+    srv.add_scene(g_box)
+    srv.add_scene(g_field)
+    reqs = [(i % 2, hk.make_request(box if i % 2 == 0 else field, spp=2 + i, seed=700 + i)) for i in range(6)]
+    tickets = [srv.submit(sid, r) for sid, r in reqs]
+    for t, (sid, r) in zip(tickets, reqs):
+        img, res = srv.wait(t)
+        alone = (g_box if sid == 0 else g_field).render(0, request=r)[0]
+        assert res["lane"] == 0 and np.array_equal(img.view(np.uint32), alone.view(np.uint32)), (sid, r.seed)
+    srv.close()
+    c.close()
+
+
 def test_server_more_requests_than_slots():
     """4097 tiny requests (8 x 8, 1 spp) on one scene with no HBM budget: one
     more than an engine holds at once.  The request beyond the engine's slots
