@@ -200,7 +200,7 @@ static void worker_main(hk_server *srv) {
     // at a time, the next when no bulk lane has one in flight.  A request the
     // engine refuses (e.g. no device memory for its film) fails alone; the
     // engine is left as it was.  Only the worker finishes tickets, so the
-    // accepted ones stay valid until this round ends.
+    // accepted ones stay valid until this pass of the loop ends.
     std::vector<Ticket *> to_submit;
     for (Ticket *tp : admitted) {
       if (tp->lane == kBulk) bulk_wait.push_back(tp);
