@@ -46,3 +46,12 @@ def test_context_creation_fails_loudly_without_device():
     out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, env=env,
                          timeout=120).stdout
     assert "ERR" in out
+
+
+@pytest.mark.skipif(not os.path.exists(hk.LIB_PATH), reason="libhikari_amd.so not built")
+def test_server_entry_points_refuse_null_handles():
+    """The server entry points check their handle before touching a device."""
+    L = hk.lib()
+    assert L.hk_server_cancel(None, 1) == -1
+    assert L.hk_server_wait(None, 1, None) == -1
+    assert L.hk_server_get_stats(None, None) == -1

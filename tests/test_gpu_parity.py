@@ -569,6 +569,7 @@ def test_server_cancel_reclaims_slots():
             srv.cancel(ts[-1])
     time.sleep(0.05)
     srv.cancel(t2)
+    srv.cancel(t2)  # a second cancel of the same ticket changes nothing
     srv.cancel(t1)
     outcome = {"cancelled": 0, "done": 0}
     for i, t in enumerate([t1, t2] + ts):
@@ -587,6 +588,9 @@ def test_server_cancel_reclaims_slots():
     print(outcome, st)
     assert outcome["cancelled"] >= 2 and st["cancelled"] == outcome["cancelled"]
     assert st["requests_in_flight"] == 0 and st["film_bytes"] == 0
+    with pytest.raises(hk.HkError) as e:  # a ticket that was waited for is gone
+        srv.cancel(t1)
+    assert e.value.code == -1
     # the slots and films came back: new requests render as they do alone
     after = [hk.make_request(box, spp=2, seed=12000 + i) for i in range(3)] + [long1]
     for r in after:
