@@ -195,7 +195,13 @@ void Chi2Test::Activate() {
   constexpr int kIntervals = 128;  // per axis and cell
   constexpr int kPts = kIntervals + 1;
   constexpr int kCellsPerBatch = 16;
-  HostPcg32 rng;
+  // the reference's stream: per test cos(theta) and phi of the incident
+  // direction, then two numbers per sample (src/core/Chi2Test.cpp:84-100),
+  // so the incident directions are the reference's; the further numbers a
+  // BSDF may draw come from a second stream
+  HostPcg32 rng, extra;
+  extra.state = 0x5bd1e995u;
+  extra.inc = (0x2545f4914f6cdd1dull << 1) | 1u;
   for (BSDF *b : bsdfs_) {
     std::vector<float> records;
     int idx = b->AppendRecords(records);
@@ -211,8 +217,8 @@ void Chi2Test::Activate() {
       for (size_t k = 0; k < n; k++) {
         u[4 * k] = rng.NextFloat();
         u[4 * k + 1] = rng.NextFloat();
-        u[4 * k + 2] = rng.NextFloat();  // for BSDFs that draw a further number
-        u[4 * k + 3] = rng.NextFloat();
+        u[4 * k + 2] = extra.NextFloat();  // for BSDFs that draw a further number
+        u[4 * k + 3] = extra.NextFloat();
       }
       Batch(records, idx, n, wi, wo, u, pdf, f, swo);
       std::vector<double> obs(size_t(cells), 0.0), expv(size_t(cells), 0.0);

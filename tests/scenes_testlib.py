@@ -156,3 +156,43 @@ def materials_box(width=48, height=48, integrator="pathMIS"):
     for k, m in ((0, floor), (2, back), (3, left), (7, mir), (8, shiny)):
         sc.meshes[k].material = m
     return sc
+
+
+class Pcg32:
+    """pcg32 (XSH-RR 64 -> 32) of the PCG family, with the default state and
+    stream of ext/pcg32/pcg32.h, seed(), floats in [0, 1) from the mantissa
+    and jump-ahead (advance) by the LCG's closed form; checked against the
+    reference's pcg32-demo.out."""
+    MASK = (1 << 64) - 1
+    MULT = 0x5851F42D4C957F2D
+
+    def __init__(self):
+        self.state, self.inc = 0x853C49E6748FEA9B, 0xDA3E39CB94B95BDB
+
+    def seed(self, initstate, initseq):
+        self.state, self.inc = 0, ((initseq << 1) | 1) & self.MASK
+        self.next_uint()
+        self.state = (self.state + initstate) & self.MASK
+        self.next_uint()
+
+    def next_uint(self):
+        old = self.state
+        self.state = (old * self.MULT + self.inc) & self.MASK
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def next_float(self):
+        u = (self.next_uint() >> 9) | 0x3F800000
+        return np.float32(np.array([u], np.uint32).view(np.float32)[0] - np.float32(1.0))
+
+    def advance(self, delta):
+        cur_mult, cur_plus, acc_mult, acc_plus = self.MULT, self.inc, 1, 0
+        while delta > 0:
+            if delta & 1:
+                acc_mult = (acc_mult * cur_mult) & self.MASK
+                acc_plus = (acc_plus * cur_mult + cur_plus) & self.MASK
+            cur_plus = ((cur_mult + 1) * cur_plus) & self.MASK
+            cur_mult = (cur_mult * cur_mult) & self.MASK
+            delta >>= 1
+        self.state = (acc_mult * self.state + acc_plus) & self.MASK

@@ -162,3 +162,24 @@ def test_whitted_scenes_match_reference_render(tmp_path, sub, name, b):
     ratio, med, p90, mx = _block_check(img, ref, b)
     print(name, "mean ratio", ratio, "block rel median", med, "p90", p90, "max", mx)
     assert abs(ratio - 1) < 0.003 and med < 0.005 and p90 < 0.012
+
+
+def test_chi2test_incident_directions_are_the_references():
+    """The chi2test harness draws each test's incident direction from the
+    reference's pcg32 stream in the reference's order (cos theta, phi, then
+    two numbers per sample, src/core/Chi2Test.cpp:84-100): the directions it
+    prints are those of the reference's test run."""
+    from tests.scenes_testlib import Pcg32
+    r = run(os.path.join(GOLDEN, "chi2test-microfacet.xml"))
+    assert r.returncode == 0, r.stdout[-2000:]
+    got = [tuple(float(x) for x in line.split("wi = ")[1].split(")")[0].split(", "))
+           for line in r.stdout.splitlines() if line.startswith("Testing (wi = ")]
+    rng, want = Pcg32(), []
+    for _ in range(15):  # 3 BSDFs x testCount 5
+        ct = rng.next_float()
+        st = np.sqrt(max(np.float32(0), np.float32(1) - ct * ct), dtype=np.float32)
+        ph = np.float32(2.0) * np.float32(np.pi) * rng.next_float()
+        want.append((np.cos(ph) * st, np.sin(ph) * st, ct))
+        rng.advance(2 * 1000000)  # sampleCount 1,000,000, two numbers each
+    assert len(got) == 15
+    assert np.allclose(np.array(got), np.array(want, np.float64), atol=1e-4), (got[:3], want[:3])

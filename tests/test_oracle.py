@@ -531,3 +531,21 @@ def test_cbox_oracle_event_counts():
     _, _, st = O.render(sc, req_seed=1, spp=(0, 16))
     ext, sh, paths = (float(x) for x in st[:3])
     assert 11.0 < ext / paths < 16.0 and 9.0 < sh / paths < 13.0
+
+
+def test_test_side_pcg32_matches_reference_demo():
+    """The tests' own pcg32 (tests/scenes_testlib.Pcg32) reproduces the
+    reference's pcg32-demo.out (seed(42, 54)), and its jump-ahead equals
+    stepping."""
+    from tests.scenes_testlib import Pcg32
+    with open(os.path.join(GOLDEN, "pcg32-demo.out")) as fh:
+        line = next(l for l in fh if l.strip().startswith("32bit:"))
+    want = [int(x, 16) for x in line.split()[1:]]
+    r = Pcg32()
+    r.seed(42, 54)
+    assert [r.next_uint() for _ in range(6)] == want
+    a, b = Pcg32(), Pcg32()
+    for _ in range(1000):
+        a.next_uint()
+    b.advance(1000)
+    assert a.state == b.state
