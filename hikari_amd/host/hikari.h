@@ -13,6 +13,7 @@
 #pragma once
 #include <cstdarg>
 #include <cstdint>
+#include <cstring>
 #include <functional>
 #include <map>
 #include <memory>
@@ -348,6 +349,25 @@ bool DryRun();
 // the reference.  `msg` receives a one-line report.
 bool StudentsTTest(double mean, double variance, double reference, long n, double significance, int n_tests,
                    std::string &msg);
+
+// pcg32 (XSH-RR 64 -> 32, O'Neill's PCG family) with its default state and
+// stream; NextFloat fills the mantissa of [1, 2) (ext/pcg32/pcg32.h).
+struct HostPcg32 {
+  uint64_t state = 0x853c49e6748fea9bull, inc = 0xda3e39cb94b95bdbull;
+  uint32_t NextUint() {
+    uint64_t old = state;
+    state = old * 0x5851f42d4c957f2dull + inc;
+    uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = uint32_t(old >> 59u);
+    return (xs >> rot) | (xs << ((~rot + 1u) & 31u));
+  }
+  float NextFloat() {
+    uint32_t u = (NextUint() >> 9) | 0x3f800000u;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f - 1.0f;
+  }
+};
 
 // Chi-square distribution function with `dof` degrees of freedom at x.
 double Chi2Cdf(double x, int dof);

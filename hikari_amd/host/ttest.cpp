@@ -123,11 +123,13 @@ void StudentTTest::Activate() {
       throw HikariException("Specified a different number of angles and reference values!");
     if (!scenes_.empty()) throw HikariException("Cannot test BSDFs and scenes at the same time!");
     size_t ref = 0;
-    uint64_t state = 0x853c49e6748fea9bull;
-    auto next = [&]() {  // host uniforms for the GPU sampler (any good generator will do)
-      state = state * 6364136223846793005ull + 1442695040888963407ull;
-      return float((state >> 40) & 0xffffff) * (1.0f / 16777216.0f);
-    };
+    // the reference's stream: two numbers per sample from a default-seeded
+    // pcg32 (src/core/StudentTTest.cpp:105), so the harness draws the
+    // reference's sample points; the further numbers a BSDF may draw come
+    // from a second stream
+    HostPcg32 rng, extra;
+    extra.state = 0x5bd1e995u;
+    extra.inc = (0x2545f4914f6cdd1dull << 1) | 1u;
     for (BSDF *b : bsdfs_) {
       std::vector<float> records;
       int idx = b->AppendRecords(records);
@@ -141,7 +143,10 @@ void StudentTTest::Activate() {
           wi[3 * k] = std::sin(th);
           wi[3 * k + 1] = 0.0f;
           wi[3 * k + 2] = std::cos(th);
-          for (int j = 0; j < 4; j++) u[4 * k + j] = next();
+          u[4 * k] = rng.NextFloat();
+          u[4 * k + 1] = rng.NextFloat();
+          u[4 * k + 2] = extra.NextFloat();
+          u[4 * k + 3] = extra.NextFloat();
         }
         CheckHk(hk_bsdf_batch(GpuContext(), records.data(), uint32_t(records.size() / HK_MATERIAL_STRIDE),
                               uint32_t(idx), nullptr, uint32_t(n), wi.data(), wo.data(), meas.data(), u.data(),
