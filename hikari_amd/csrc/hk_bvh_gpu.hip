@@ -14,7 +14,12 @@
 //   4. a bottom-up pass (the second child to arrive builds its parent) that
 //      sets boxes and SAH costs and turns subtrees of at most kLeafMax
 //      triangles into leaves where that is cheaper;
-//   5. a top-down, level-synchronous collapse into Bvh4Node: each 4-wide
+//   5. above the cells of the top 12 Morton bits (4 per axis, as the
+//      reference's treelets, HLBVHAcceleration.cpp:118-191) the radix tree's
+//      spatial-median nodes are replaced by a bucketed SAH tree over the
+//      cells' subtree roots, built on the host (at most 4096 roots) into the
+//      same node slots;
+//   6. a top-down, level-synchronous collapse into Bvh4Node: each 4-wide
 //      node opens the internal child of largest area until it has four
 //      children, as the host builder does.
 //
@@ -126,6 +131,7 @@ __global__ __launch_bounds__(kB) void k_bvh_morton(const float *__restrict__ tri
 struct Tree {
   float4 *lo, *hi;        // boxes (2n-1)
   int2 *child;            // internal: (left, right)
+  int32_t *prefix;        // internal: common-prefix length of the node's keys
   int32_t *parent;        // (2n-1); root: -1
   uint2 *range;           // (2n-1): first, last sorted position
   float *cost;            // SAH cost of the subtree as built
@@ -200,6 +206,7 @@ __global__ __launch_bounds__(kB) void k_bvh_karras(const uint64_t *__restrict__ 
   int left = first == gamma ? n - 1 + gamma : gamma;
   int right = last == gamma + 1 ? n - 1 + gamma + 1 : gamma + 1;
   T.child[i] = make_int2(left, right);
+  T.prefix[i] = dnode;
   T.range[i] = make_uint2(uint32_t(first), uint32_t(last));
   T.parent[left] = i;
   T.parent[right] = i;
@@ -239,6 +246,30 @@ __global__ __launch_bounds__(kB) void k_bvh_refit(Tree T) {
 struct Todo {
   int32_t bnode, slot;
 };
+
+// Boxes and ranges of the nodes idx[0 .. m).
+__global__ __launch_bounds__(kB) void k_bvh_gather(const int32_t *__restrict__ idx, uint32_t m, Tree T, float4 *lo,
+                                                  float4 *hi, uint2 *range) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  int32_t b = idx[k];
+  lo[k] = T.lo[b];
+  hi[k] = T.hi[b];
+  range[k] = T.range[b];
+}
+
+// Internal nodes idx[j] become (child[j], box lo[j] / hi[j]).
+__global__ __launch_bounds__(kB) void k_bvh_splice(const int32_t *__restrict__ idx, const int2 *__restrict__ child,
+                                                  const float4 *__restrict__ lo, const float4 *__restrict__ hi,
+                                                  uint32_t m, Tree T) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  int32_t b = idx[j];
+  T.child[b] = child[j];
+  T.lo[b] = lo[j];
+  T.hi[b] = hi[j];
+  T.leaf[b] = 0u;
+}
 
 // One level of the collapse.  counters[0]: 4-wide nodes allocated,
 // counters[1]: entries written to `out`, counters[2]: leaves.
@@ -292,6 +323,94 @@ __global__ __launch_bounds__(kB) void k_bvh_collapse(Tree T, const Todo *__restr
   nodes[t.slot] = nd;
 }
 
+// Bucketed SAH tree over the cell roots it[s .. e) (box, centroid, triangle
+// count), built on the host.  Returns an upper-node number (>= 0; node 0 is
+// the first call's) or ~item for a single root; kids / boxes receive the
+// upper nodes.
+struct UpperItem {
+  float lo[3], hi[3], cen[3];
+  double count;
+  int32_t node;  // radix-tree node index
+};
+struct UpperBox {
+  float lo[3], hi[3];
+};
+float box_area(const float *lo, const float *hi) {
+  if (!(lo[0] <= hi[0])) return 0.0f;
+  float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+  return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+void box_grow(UpperBox &b, const float *lo, const float *hi) {
+  for (int a = 0; a < 3; a++) {
+    b.lo[a] = std::min(b.lo[a], lo[a]);
+    b.hi[a] = std::max(b.hi[a], hi[a]);
+  }
+}
+UpperBox box_empty() {
+  UpperBox b;
+  for (int a = 0; a < 3; a++) {
+    b.lo[a] = INFINITY;
+    b.hi[a] = -INFINITY;
+  }
+  return b;
+}
+int32_t upper_build(std::vector<UpperItem> &it, int s, int e, std::vector<int2> &kids, std::vector<UpperBox> &boxes) {
+  if (e - s == 1) return ~s;
+  int32_t u = int32_t(kids.size());
+  kids.push_back(make_int2(0, 0));
+  boxes.push_back(box_empty());
+  UpperBox cb = box_empty();
+  for (int i = s; i < e; i++) box_grow(cb, it[size_t(i)].cen, it[size_t(i)].cen);
+  int axis = 0;
+  for (int a = 1; a < 3; a++)
+    if (cb.hi[a] - cb.lo[a] > cb.hi[axis] - cb.lo[axis]) axis = a;
+  float ext = cb.hi[axis] - cb.lo[axis], lo0 = cb.lo[axis];
+  constexpr int kNB = 16;
+  auto bucket = [&](const UpperItem &x) {
+    return std::min(kNB - 1, std::max(0, int((x.cen[axis] - lo0) / ext * float(kNB))));
+  };
+  int mid = -1;
+  if (ext > 0.0f) {
+    UpperBox bb[kNB];
+    double cnt[kNB] = {0};
+    int num[kNB] = {0};
+    for (int b = 0; b < kNB; b++) bb[b] = box_empty();
+    for (int i = s; i < e; i++) {
+      int b = bucket(it[size_t(i)]);
+      box_grow(bb[b], it[size_t(i)].lo, it[size_t(i)].hi);
+      cnt[b] += it[size_t(i)].count;
+      num[b]++;
+    }
+    double best = INFINITY;
+    int split = -1;
+    for (int sp = 0; sp + 1 < kNB; sp++) {
+      UpperBox l = box_empty(), r = box_empty();
+      double cl = 0, cr = 0;
+      int nl = 0, nr = 0;
+      for (int b = 0; b <= sp; b++) { box_grow(l, bb[b].lo, bb[b].hi); cl += cnt[b]; nl += num[b]; }
+      for (int b = sp + 1; b < kNB; b++) { box_grow(r, bb[b].lo, bb[b].hi); cr += cnt[b]; nr += num[b]; }
+      if (nl == 0 || nr == 0) continue;
+      double c = double(box_area(l.lo, l.hi)) * cl + double(box_area(r.lo, r.hi)) * cr;
+      if (c < best) { best = c; split = sp; }
+    }
+    if (split >= 0)
+      mid = int(std::partition(it.begin() + s, it.begin() + e, [&](const UpperItem &x) { return bucket(x) <= split; }) -
+                it.begin());
+  }
+  if (mid <= s || mid >= e) {  // equal centroids: the median on the widest axis
+    mid = s + (e - s) / 2;
+    std::nth_element(it.begin() + s, it.begin() + mid, it.begin() + e,
+                     [&](const UpperItem &x, const UpperItem &y) { return x.cen[axis] < y.cen[axis]; });
+  }
+  int32_t l = upper_build(it, s, mid, kids, boxes), r = upper_build(it, mid, e, kids, boxes);
+  kids[size_t(u)] = make_int2(l, r);
+  for (int32_t c : {l, r}) {
+    if (c >= 0) box_grow(boxes[size_t(u)], boxes[size_t(c)].lo, boxes[size_t(c)].hi);
+    else box_grow(boxes[size_t(u)], it[size_t(~c)].lo, it[size_t(~c)].hi);
+  }
+  return u;
+}
+
 __global__ void k_bvh_init(uint32_t *bounds, Todo *q, uint32_t *counters) {
   if (threadIdx.x < 12) bounds[threadIdx.x] = (threadIdx.x % 6) < 3 ? 0xffffffffu : 0u;
   if (threadIdx.x == 0) {
@@ -309,7 +428,7 @@ uint32_t blocks(uint32_t n) { return (n + kB - 1) / kB; }
 void build_bvh4_device(hipStream_t st, const float *tri9_host, uint32_t n, DevBvh &out) {
   if (n < 2) throw HkError(HK_ERR_INVALID, "the device BVH builder needs at least two triangles");
   if (n >= (1u << 28)) throw HkError(HK_ERR_INVALID, "too many triangles for the leaf encoding");
-  DBuf tri9, bounds, keys[2], vals[2], lo, hi, child, parent, range, cost, leaf, visit, q[2], counters, temp;
+  DBuf tri9, bounds, keys[2], vals[2], lo, hi, child, prefix, parent, range, cost, leaf, visit, q[2], counters, temp;
   tri9.upload(tri9_host, size_t(n) * 36, st);
   size_t nn = 2 * size_t(n) - 1;
   bounds.alloc(12 * 4);
@@ -348,6 +467,7 @@ void build_bvh4_device(hipStream_t st, const float *tri9_host, uint32_t n, DevBv
   lo.alloc(nn * 16);
   hi.alloc(nn * 16);
   child.alloc(size_t(n) * 8);
+  prefix.alloc(size_t(n) * 4);
   parent.alloc(nn * 4);
   range.alloc(nn * 8);
   cost.alloc(nn * 4);
@@ -355,13 +475,88 @@ void build_bvh4_device(hipStream_t st, const float *tri9_host, uint32_t n, DevBv
   visit.alloc(size_t(n) * 4);
   hip_check(hipMemsetAsync(visit.p, 0, size_t(n) * 4, st), "BVH visit reset");
   hip_check(hipMemsetAsync(leaf.p, 0, nn * 4, st), "BVH leaf reset");
-  Tree T{lo.as<float4>(), hi.as<float4>(), child.as<int2>(), parent.as<int32_t>(), range.as<uint2>(), cost.as<float>(),
-         leaf.as<uint32_t>(), visit.as<uint32_t>(), n};
+  Tree T{lo.as<float4>(), hi.as<float4>(), child.as<int2>(), prefix.as<int32_t>(), parent.as<int32_t>(),
+         range.as<uint2>(), cost.as<float>(), leaf.as<uint32_t>(), visit.as<uint32_t>(), n};
   out.tris.alloc(size_t(n) * sizeof(TraceTri));
   hipLaunchKernelGGL(k_bvh_leaves, dim3(blocks(n)), dim3(kB), 0, st, tri9.as<float>(), vals[1].as<uint32_t>(),
                      scene_ext, T, out.tris.as<TraceTri>());
   hipLaunchKernelGGL(k_bvh_karras, dim3(blocks(n - 1)), dim3(kB), 0, st, keys[1].as<uint64_t>(), T);
   hipLaunchKernelGGL(k_bvh_refit, dim3(blocks(n)), dim3(kB), 0, st, T);
+  // The SAH upper tree: the radix tree's nodes above the cells of the top 12
+  // Morton bits (common prefix < 13 bits of the 64-bit keys, whose top bit is
+  // always 0) are a binary tree over the cells' subtree roots; a bucketed SAH
+  // tree over the same roots has as many internal nodes and takes their
+  // slots (its root node 0).
+  {
+    std::vector<int2> hchild(n - 1);
+    std::vector<int32_t> hpre(n - 1);
+    hip_check(hipMemcpyAsync(hchild.data(), child.p, sizeof(int2) * (n - 1), hipMemcpyDeviceToHost, st),
+              "BVH children");
+    hip_check(hipMemcpyAsync(hpre.data(), prefix.p, sizeof(int32_t) * (n - 1), hipMemcpyDeviceToHost, st),
+              "BVH prefixes");
+    hip_check(hipStreamSynchronize(st), "BVH tree");
+    constexpr int32_t kCellPrefix = 13;
+    std::vector<int32_t> top, roots, stack{0};
+    while (!stack.empty()) {
+      int32_t b = stack.back();
+      stack.pop_back();
+      if (b >= int32_t(n - 1) || hpre[size_t(b)] >= kCellPrefix) {
+        roots.push_back(b);
+        continue;
+      }
+      top.push_back(b);
+      stack.push_back(hchild[size_t(b)].y);
+      stack.push_back(hchild[size_t(b)].x);
+    }
+    if (roots.size() >= 3) {
+      uint32_t m = uint32_t(roots.size());
+      DBuf d_idx, d_lo, d_hi, d_rng;
+      d_idx.upload(roots.data(), size_t(m) * 4, st);
+      d_lo.alloc(size_t(m) * 16);
+      d_hi.alloc(size_t(m) * 16);
+      d_rng.alloc(size_t(m) * 8);
+      hipLaunchKernelGGL(k_bvh_gather, dim3(blocks(m)), dim3(kB), 0, st, d_idx.as<int32_t>(), m, T, d_lo.as<float4>(),
+                         d_hi.as<float4>(), d_rng.as<uint2>());
+      std::vector<float4> hlo(m), hhi(m);
+      std::vector<uint2> hrng(m);
+      hip_check(hipMemcpyAsync(hlo.data(), d_lo.p, size_t(m) * 16, hipMemcpyDeviceToHost, st), "BVH root boxes");
+      hip_check(hipMemcpyAsync(hhi.data(), d_hi.p, size_t(m) * 16, hipMemcpyDeviceToHost, st), "BVH root boxes");
+      hip_check(hipMemcpyAsync(hrng.data(), d_rng.p, size_t(m) * 8, hipMemcpyDeviceToHost, st), "BVH root ranges");
+      hip_check(hipStreamSynchronize(st), "BVH roots");
+      std::vector<UpperItem> items(m);
+      for (uint32_t k = 0; k < m; k++) {
+        UpperItem &x = items[k];
+        float l4[3] = {hlo[k].x, hlo[k].y, hlo[k].z}, h4[3] = {hhi[k].x, hhi[k].y, hhi[k].z};
+        for (int a = 0; a < 3; a++) {
+          x.lo[a] = l4[a];
+          x.hi[a] = h4[a];
+          x.cen[a] = 0.5f * (l4[a] + h4[a]);
+        }
+        x.count = double(hrng[k].y - hrng[k].x + 1u);
+        x.node = roots[k];
+      }
+      std::vector<int2> kids;
+      std::vector<UpperBox> boxes;
+      upper_build(items, 0, int(m), kids, boxes);
+      if (kids.size() != top.size()) throw HkError(HK_ERR_INVALID, "BVH upper tree: node count mismatch");
+      std::vector<int2> rec_child(kids.size());
+      std::vector<float4> rec_lo(kids.size()), rec_hi(kids.size());
+      auto slot = [&](int32_t c) { return c >= 0 ? top[size_t(c)] : items[size_t(~c)].node; };
+      for (size_t u = 0; u < kids.size(); u++) {
+        rec_child[u] = make_int2(slot(kids[u].x), slot(kids[u].y));
+        rec_lo[u] = make_float4(boxes[u].lo[0], boxes[u].lo[1], boxes[u].lo[2], 0.0f);
+        rec_hi[u] = make_float4(boxes[u].hi[0], boxes[u].hi[1], boxes[u].hi[2], 0.0f);
+      }
+      DBuf d_top, d_child, d_ulo, d_uhi;
+      d_top.upload(top.data(), top.size() * 4, st);
+      d_child.upload(rec_child.data(), rec_child.size() * 8, st);
+      d_ulo.upload(rec_lo.data(), rec_lo.size() * 16, st);
+      d_uhi.upload(rec_hi.data(), rec_hi.size() * 16, st);
+      hipLaunchKernelGGL(k_bvh_splice, dim3(blocks(uint32_t(top.size()))), dim3(kB), 0, st, d_top.as<int32_t>(),
+                         d_child.as<int2>(), d_ulo.as<float4>(), d_uhi.as<float4>(), uint32_t(top.size()), T);
+      hip_check(hipStreamSynchronize(st), "BVH upper tree");
+    }
+  }
   // The root's children always form a 4-wide node, also when the root could
   // be one leaf (n <= kLeafMax).
   hip_check(hipMemsetAsync(leaf.p, 0, 4, st), "BVH root");
