@@ -79,6 +79,8 @@ def parse():
                          "the defaults are its usual choice on these scenes")
     ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
                     help="BVH construction: binned SAH on the host or linear BVH on the device")
+    ap.add_argument("--bvh-cell-bits", type=int, default=-1,
+                    help="device BVH: Morton bits of the cells under the SAH upper tree (-1: library default)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["c1", "c2", "batch", "serve", "accel", "copy"], default="c1")
     ap.add_argument("--requests", type=int, default=0, help="batch/serve: number of requests (default 64 / 40)")
@@ -435,6 +437,8 @@ def main():
     if a.max_paths:
         ctx.set_option("max_paths", a.max_paths)
     ctx.set_option("bvh_builder", 1 if a.bvh_builder == "device" else 0)
+    if a.bvh_cell_bits >= 0:
+        ctx.set_option("bvh_cell_bits", a.bvh_cell_bits)
     ctx.set_option("graph", a.graph)
     ctx.set_option("refill", a.refill)
     ctx.set_option("refill_shadow", a.refill_shadow)

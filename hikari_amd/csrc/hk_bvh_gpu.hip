@@ -14,11 +14,11 @@
 //   4. a bottom-up pass (the second child to arrive builds its parent) that
 //      sets boxes and SAH costs and turns subtrees of at most kLeafMax
 //      triangles into leaves where that is cheaper;
-//   5. above the cells of the top 12 Morton bits (4 per axis, as the
-//      reference's treelets, HLBVHAcceleration.cpp:118-191) the radix tree's
-//      spatial-median nodes are replaced by a bucketed SAH tree over the
-//      cells' subtree roots, built on the host (at most 4096 roots) into the
-//      same node slots;
+//   5. above the cells of the top `cell_bits` Morton bits (the reference's
+//      treelets, HLBVHAcceleration.cpp:118-191, use 12: 4 per axis) the radix
+//      tree's spatial-median nodes are replaced by a bucketed SAH tree over
+//      the cells' subtree roots, built on the host into the same node slots
+//      (those nodes cover no contiguous range and never become leaves);
 //   6. a top-down, level-synchronous collapse into Bvh4Node: each 4-wide
 //      node opens the internal child of largest area until it has four
 //      children, as the host builder does.
@@ -354,7 +354,11 @@ UpperBox box_empty() {
   }
   return b;
 }
-int32_t upper_build(std::vector<UpperItem> &it, int s, int e, std::vector<int2> &kids, std::vector<UpperBox> &boxes) {
+// Below depth kUpperSahDepth the splits are medians, which bounds the
+// recursion at kUpperSahDepth + log2(items) frames whatever the boxes.
+constexpr int kUpperSahDepth = 48;
+int32_t upper_build(std::vector<UpperItem> &it, int s, int e, int depth, std::vector<int2> &kids,
+                    std::vector<UpperBox> &boxes) {
   if (e - s == 1) return ~s;
   int32_t u = int32_t(kids.size());
   kids.push_back(make_int2(0, 0));
@@ -370,7 +374,7 @@ int32_t upper_build(std::vector<UpperItem> &it, int s, int e, std::vector<int2> 
     return std::min(kNB - 1, std::max(0, int((x.cen[axis] - lo0) / ext * float(kNB))));
   };
   int mid = -1;
-  if (ext > 0.0f) {
+  if (ext > 0.0f && depth < kUpperSahDepth) {
     UpperBox bb[kNB];
     double cnt[kNB] = {0};
     int num[kNB] = {0};
@@ -381,28 +385,43 @@ int32_t upper_build(std::vector<UpperItem> &it, int s, int e, std::vector<int2> 
       cnt[b] += it[size_t(i)].count;
       num[b]++;
     }
+    // right-to-left sweep: the cost terms of buckets sp+1 .. kNB-1; then a
+    // left-to-right sweep adds those of 0 .. sp
+    double right_cost[kNB];
+    int right_num[kNB];
+    UpperBox r = box_empty();
+    double cr = 0;
+    int nr = 0;
+    for (int b = kNB - 1; b > 0; b--) {
+      box_grow(r, bb[b].lo, bb[b].hi);
+      cr += cnt[b];
+      nr += num[b];
+      right_cost[b - 1] = double(box_area(r.lo, r.hi)) * cr;
+      right_num[b - 1] = nr;
+    }
     double best = INFINITY;
     int split = -1;
+    UpperBox l = box_empty();
+    double cl = 0;
+    int nl = 0;
     for (int sp = 0; sp + 1 < kNB; sp++) {
-      UpperBox l = box_empty(), r = box_empty();
-      double cl = 0, cr = 0;
-      int nl = 0, nr = 0;
-      for (int b = 0; b <= sp; b++) { box_grow(l, bb[b].lo, bb[b].hi); cl += cnt[b]; nl += num[b]; }
-      for (int b = sp + 1; b < kNB; b++) { box_grow(r, bb[b].lo, bb[b].hi); cr += cnt[b]; nr += num[b]; }
-      if (nl == 0 || nr == 0) continue;
-      double c = double(box_area(l.lo, l.hi)) * cl + double(box_area(r.lo, r.hi)) * cr;
+      box_grow(l, bb[sp].lo, bb[sp].hi);
+      cl += cnt[sp];
+      nl += num[sp];
+      if (nl == 0 || right_num[sp] == 0) continue;
+      double c = double(box_area(l.lo, l.hi)) * cl + right_cost[sp];
       if (c < best) { best = c; split = sp; }
     }
     if (split >= 0)
       mid = int(std::partition(it.begin() + s, it.begin() + e, [&](const UpperItem &x) { return bucket(x) <= split; }) -
                 it.begin());
   }
-  if (mid <= s || mid >= e) {  // equal centroids: the median on the widest axis
+  if (mid <= s || mid >= e) {  // equal centroids, or deep in the tree: the median on the widest axis
     mid = s + (e - s) / 2;
     std::nth_element(it.begin() + s, it.begin() + mid, it.begin() + e,
                      [&](const UpperItem &x, const UpperItem &y) { return x.cen[axis] < y.cen[axis]; });
   }
-  int32_t l = upper_build(it, s, mid, kids, boxes), r = upper_build(it, mid, e, kids, boxes);
+  int32_t l = upper_build(it, s, mid, depth + 1, kids, boxes), r = upper_build(it, mid, e, depth + 1, kids, boxes);
   kids[size_t(u)] = make_int2(l, r);
   for (int32_t c : {l, r}) {
     if (c >= 0) box_grow(boxes[size_t(u)], boxes[size_t(c)].lo, boxes[size_t(c)].hi);
@@ -425,7 +444,7 @@ uint32_t blocks(uint32_t n) { return (n + kB - 1) / kB; }
 
 }  // namespace
 
-void build_bvh4_device(hipStream_t st, const float *tri9_host, uint32_t n, DevBvh &out) {
+void build_bvh4_device(hipStream_t st, const float *tri9_host, uint32_t n, uint32_t cell_bits, DevBvh &out) {
   if (n < 2) throw HkError(HK_ERR_INVALID, "the device BVH builder needs at least two triangles");
   if (n >= (1u << 28)) throw HkError(HK_ERR_INVALID, "too many triangles for the leaf encoding");
   DBuf tri9, bounds, keys[2], vals[2], lo, hi, child, prefix, parent, range, cost, leaf, visit, q[2], counters, temp;
@@ -482,12 +501,12 @@ void build_bvh4_device(hipStream_t st, const float *tri9_host, uint32_t n, DevBv
                      scene_ext, T, out.tris.as<TraceTri>());
   hipLaunchKernelGGL(k_bvh_karras, dim3(blocks(n - 1)), dim3(kB), 0, st, keys[1].as<uint64_t>(), T);
   hipLaunchKernelGGL(k_bvh_refit, dim3(blocks(n)), dim3(kB), 0, st, T);
-  // The SAH upper tree: the radix tree's nodes above the cells of the top 12
-  // Morton bits (common prefix < 13 bits of the 64-bit keys, whose top bit is
-  // always 0) are a binary tree over the cells' subtree roots; a bucketed SAH
-  // tree over the same roots has as many internal nodes and takes their
-  // slots (its root node 0).
-  {
+  // The SAH upper tree: the radix tree's nodes above the cells of the top
+  // `cell_bits` Morton bits (common prefix <= cell_bits bits of the 64-bit
+  // keys, whose top bit is always 0) are a binary tree over the cells' subtree
+  // roots; a bucketed SAH tree over the same roots has as many internal nodes
+  // and takes their slots (its root node 0).
+  if (cell_bits > 0) {
     std::vector<int2> hchild(n - 1);
     std::vector<int32_t> hpre(n - 1);
     hip_check(hipMemcpyAsync(hchild.data(), child.p, sizeof(int2) * (n - 1), hipMemcpyDeviceToHost, st),
@@ -495,7 +514,7 @@ void build_bvh4_device(hipStream_t st, const float *tri9_host, uint32_t n, DevBv
     hip_check(hipMemcpyAsync(hpre.data(), prefix.p, sizeof(int32_t) * (n - 1), hipMemcpyDeviceToHost, st),
               "BVH prefixes");
     hip_check(hipStreamSynchronize(st), "BVH tree");
-    constexpr int32_t kCellPrefix = 13;
+    const int32_t kCellPrefix = int32_t(cell_bits) + 1;
     std::vector<int32_t> top, roots, stack{0};
     while (!stack.empty()) {
       int32_t b = stack.back();
@@ -537,7 +556,7 @@ void build_bvh4_device(hipStream_t st, const float *tri9_host, uint32_t n, DevBv
       }
       std::vector<int2> kids;
       std::vector<UpperBox> boxes;
-      upper_build(items, 0, int(m), kids, boxes);
+      upper_build(items, 0, int(m), 0, kids, boxes);
       if (kids.size() != top.size()) throw HkError(HK_ERR_INVALID, "BVH upper tree: node count mismatch");
       std::vector<int2> rec_child(kids.size());
       std::vector<float4> rec_lo(kids.size()), rec_hi(kids.size());

@@ -470,6 +470,9 @@ int hk_set_option(hk_ctx *ctx, const char *name, int64_t value) {
   } else if (n == "bvh_builder") {
     if (value < 0 || value > 1) throw HkError(HK_ERR_INVALID, "bvh_builder: 0 (host SAH) or 1 (device LBVH)");
     ctx->bvh_builder = int(value);
+  } else if (n == "bvh_cell_bits") {
+    if (value < 0 || value > 30) throw HkError(HK_ERR_INVALID, "bvh_cell_bits: 0..30");
+    ctx->bvh_cell_bits = uint32_t(value);
   } else {
     throw HkError(HK_ERR_INVALID, "unknown option \"%s\"", name);
   }

@@ -26,7 +26,7 @@ extern thread_local std::string g_create_error;
 
 void upload_bvh(hk_ctx *ctx, const float *tri9, uint32_t nt, DevBvh &out) {
   if (ctx->bvh_builder == 1 && nt >= 2) {
-    build_bvh4_device(ctx->stream, tri9, nt, out);
+    build_bvh4_device(ctx->stream, tri9, nt, ctx->bvh_cell_bits, out);
     if (3 * out.max_depth + 1 > 96)
       throw HkError(HK_ERR_UNSUPPORTED, "device BVH depth %u exceeds the traversal stack (use the host builder)",
                     out.max_depth);

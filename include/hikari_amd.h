@@ -341,9 +341,14 @@ int hk_server_destroy(hk_server *server);
  *   "resolve_tiles" 0: resolve films with the plain per-pixel gather instead
  *                 of LDS tiles (same bits; default 1)
  *   "bvh_builder" 0: binned-SAH build on the host (default); 1: linear BVH
- *                 built on the device (Morton order, SAH leaf collapse), for
- *                 callers that add scenes while rendering.  Both give the
- *                 same closest hits bit for bit. */
+ *                 built on the device (Morton order, SAH leaf collapse, a
+ *                 bucketed SAH tree over the Morton cells), for callers that
+ *                 add scenes while rendering.  Both give the same closest hits
+ *                 bit for bit.
+ *   "bvh_cell_bits" device builder: Morton bits of the cells under its SAH
+ *                 upper tree, 0..30 (0: the radix tree all the way up;
+ *                 default 24; the reference's hlbvh uses 12).  More bits
+ *                 trace faster and build slower; hits do not change. */
 int hk_set_option(hk_ctx *ctx, const char *name, int64_t value);
 
 #ifdef __cplusplus

@@ -4,7 +4,7 @@ identical to those over the host-built BVH, and the tree's shape."""
 import numpy as np
 import pytest
 
-from hikari_amd import scenes
+from hikari_amd import HkError, scenes
 from oracle import pyoracle as O
 from tests.scenes_testlib import box_rays, grazing_rays, scene_triangles, scene_vf
 
@@ -49,6 +49,27 @@ def test_device_bvh_sphere_bitexact(dctx):
     assert ok.all()
     assert info["n_triangles"] == len(f) and 0 < info["n_nodes"] < len(f)
     assert info["n_leaves"] >= len(f) / 8 and info["max_depth"] <= 31
+
+
+@pytest.mark.parametrize("cell_bits", [0, 3, 18])
+def test_device_bvh_cell_bits_bitexact(dctx, cell_bits):
+    """The SAH upper tree over Morton cells of any size (0: none, 3: a few
+    cells, 18: cells of a triangle or two) changes the tree, never the hits."""
+    rng = np.random.default_rng(40 + cell_bits)
+    v, f = _sphere(5)
+    dctx.set_option("bvh_cell_bits", cell_bits)
+    try:
+        acc = dctx.accel(v, f)
+    finally:
+        dctx.set_option("bvh_cell_bits", 24)
+    info = acc.info()
+    print("cell bits", cell_bits, "device BVH:", info)
+    tris = v[f].reshape(-1, 9).astype(np.float32)
+    rays = np.concatenate([box_rays(rng, 40000, -1.5, 1.5), grazing_rays(rng, tris, 40000)])
+    assert _hits_equal(acc.intersect(rays), O.intersect(tris, rays, use_bvh=True)).all()
+    assert info["n_triangles"] == len(f) and info["max_depth"] <= 31
+    with pytest.raises(HkError):
+        dctx.set_option("bvh_cell_bits", 31)
 
 
 def test_device_bvh_cbox_closest_and_any_hit(dctx):
