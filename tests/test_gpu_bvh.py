@@ -51,10 +51,11 @@ def test_device_bvh_sphere_bitexact(dctx):
     assert info["n_leaves"] >= len(f) / 8 and info["max_depth"] <= 31
 
 
-@pytest.mark.parametrize("cell_bits", [0, 3, 18])
+@pytest.mark.parametrize("cell_bits", [0, 3, 18, 30])
 def test_device_bvh_cell_bits_bitexact(dctx, cell_bits):
     """The SAH upper tree over Morton cells of any size (0: none, 3: a few
-    cells, 18: cells of a triangle or two) changes the tree, never the hits."""
+    cells, 18: cells of a triangle or two, 30: the largest setting) changes
+    the tree, never the hits."""
     rng = np.random.default_rng(40 + cell_bits)
     v, f = _sphere(5)
     dctx.set_option("bvh_cell_bits", cell_bits)
