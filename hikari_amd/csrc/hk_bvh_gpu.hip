@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <vector>
 
 #include "hk_build.h"
@@ -357,6 +358,9 @@ UpperBox box_empty() {
 // Below depth kUpperSahDepth the splits are medians, which bounds the
 // recursion at kUpperSahDepth + log2(items) frames whatever the boxes.
 constexpr int kUpperSahDepth = 48;
+// The two halves of a node above depth kUpperParDepth with at least
+// kUpperParItems roots are built on two host threads (at most 8 at once).
+constexpr int kUpperParDepth = 3, kUpperParItems = 1 << 14;
 int32_t upper_build(std::vector<UpperItem> &it, int s, int e, int depth, std::vector<int2> &kids,
                     std::vector<UpperBox> &boxes) {
   if (e - s == 1) return ~s;
@@ -421,7 +425,27 @@ int32_t upper_build(std::vector<UpperItem> &it, int s, int e, int depth, std::ve
     std::nth_element(it.begin() + s, it.begin() + mid, it.begin() + e,
                      [&](const UpperItem &x, const UpperItem &y) { return x.cen[axis] < y.cen[axis]; });
   }
-  int32_t l = upper_build(it, s, mid, depth + 1, kids, boxes), r = upper_build(it, mid, e, depth + 1, kids, boxes);
+  int32_t l, r;
+  if (depth < kUpperParDepth && e - s >= kUpperParItems) {
+    // each half into vectors of its own (its root at 0), then appended with
+    // its upper-node numbers shifted; item numbers are global already
+    std::vector<int2> kl, kr;
+    std::vector<UpperBox> bl, br;
+    auto left = std::async(std::launch::async, [&] { return upper_build(it, s, mid, depth + 1, kl, bl); });
+    int32_t rr = upper_build(it, mid, e, depth + 1, kr, br);
+    int32_t ll = left.get();
+    auto append = [&](const std::vector<int2> &k, const std::vector<UpperBox> &b, int32_t root) {
+      int32_t base = int32_t(kids.size());
+      for (int2 c : k) kids.push_back(make_int2(c.x >= 0 ? c.x + base : c.x, c.y >= 0 ? c.y + base : c.y));
+      boxes.insert(boxes.end(), b.begin(), b.end());
+      return root >= 0 ? root + base : root;
+    };
+    l = append(kl, bl, ll);
+    r = append(kr, br, rr);
+  } else {
+    l = upper_build(it, s, mid, depth + 1, kids, boxes);
+    r = upper_build(it, mid, e, depth + 1, kids, boxes);
+  }
   kids[size_t(u)] = make_int2(l, r);
   for (int32_t c : {l, r}) {
     if (c >= 0) box_grow(boxes[size_t(u)], boxes[size_t(c)].lo, boxes[size_t(c)].hi);
